@@ -1,0 +1,177 @@
+/*
+ * psmma.h - C ABI of libpsmma_hip.so, the MI355X (gfx950) implementation of the
+ * PSMMA structured x dense products.
+ *
+ * Two layers:
+ *
+ *  1. Fortran drop-ins.  Lower-case names with one trailing underscore and every
+ *     argument by reference, exactly as gfortran/flang/ifort emit calls to the
+ *     reference's implicit-interface externals.  A Fortran program that calls
+ *     PSMMA_TOEPLITZ(...) links against this library unchanged.  They take host
+ *     arrays, run on the current HIP device and return when the result is back
+ *     in host memory.  This build serves one GPU's share: the BLACS grid must be
+ *     1 x 1 (if a BLACS library is linked, BLACS_GRIDINFO is consulted and a
+ *     larger grid is rejected).  INTEGER is 4 bytes, LOGICAL is 4 bytes
+ *     (nonzero = .TRUE.).  Errors are reported the reference's way - a message
+ *     on stderr and a plain return - and additionally through psmma_last_status().
+ *
+ *  2. Device API.  Device pointers, a hipStream_t passed as void*, asynchronous
+ *     on that stream, status returned (0 = success, -i = argument i invalid,
+ *     > 0 = HIP/runtime error; see psmma_last_error()).  It exposes the offsets
+ *     (r0, c0) of the block of the global structured matrix that is multiplied,
+ *     so a caller can drive one block of a larger product.
+ *
+ * Matrices are column-major with explicit leading dimensions.  Complex data is
+ * interleaved complex128 (re, im), leading dimensions in complex elements.
+ */
+#ifndef PSMMA_H
+#define PSMMA_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------------------
+ * Numeric forms.
+ *   PSMMA_PREC_FP64     plain FP64 on FP64 MFMA (v_mfma_f64_16x16x4f64).
+ *   PSMMA_PREC_SPLIT_I8 error-free split: both operands cut into `slices`
+ *                       int8 slices with power-of-two row/column scales, slice
+ *                       products on int8 MFMA with exact int32 accumulation,
+ *                       recombined in FP64.
+ *   PSMMA_PREC_SPLIT_BF16 the same with bf16 slices on bf16 MFMA, FP32
+ *                       accumulation over exactly representable K-chunks and
+ *                       FP64 recombination.
+ * The pole differences of the Cauchy kinds are always formed in FP64.
+ */
+#define PSMMA_PREC_FP64 0
+#define PSMMA_PREC_SPLIT_I8 1
+#define PSMMA_PREC_SPLIT_BF16 2
+
+/* Kinds of structured operand (psmma_structured_dev, batched API). */
+#define PSMMA_KIND_CAUCHY 0
+#define PSMMA_KIND_TOEPLITZ 1
+#define PSMMA_KIND_HANKEL 2
+#define PSMMA_KIND_CAUCHYEIG 3
+
+/* ===========================================================================
+ * 1. Fortran drop-in entry points (host pointers).
+ * ======================================================================== */
+
+/* Q = alpha*A*H + beta*Q, H(i,j) = U_i V_j / (D_i - F_j), generators B(LDB,4)=[D F U V].
+ * Replaces PSMMA_CAUCHY, SRC/psmma_cauchy.f90:2-3. */
+void psmma_cauchy_(const int* n, const int* k, const double* alpha, const double* a,
+                   const int* ia, const int* ja, const int* desca, const double* b,
+                   const int* ldb, const double* beta, double* q, const int* iq, const int* jq,
+                   const int* descq, double* work, const int* redist);
+
+/* Q = alpha*A*H + beta*Q, H(i,j) = B(K+j-i), B of length 2K-1.
+ * Replaces PSMMA_TOEPLITZ, SRC/psmma_toeplitz.f90:2-3. */
+void psmma_toeplitz_(const int* n, const int* k, const double* alpha, const double* a,
+                     const int* ia, const int* ja, const int* desca, const double* b,
+                     const int* ldb, const double* beta, double* q, const int* iq, const int* jq,
+                     const int* descq, double* work, const int* redist);
+
+/* Q = alpha*A*H + beta*Q, H(i,j) = B(i+j-1).
+ * Replaces PSMMA_HANKEL, SRC/psmma_hankel.f90:2-3. */
+void psmma_hankel_(const int* n, const int* k, const double* alpha, const double* a,
+                   const int* ia, const int* ja, const int* desca, const double* b,
+                   const int* ldb, const double* beta, double* q, const int* iq, const int* jq,
+                   const int* descq, double* work, const int* redist);
+
+/* Q(IQ:,JQ:) = alpha*A*H + beta*Q(IQ:,JQ:), H(i,j) = U_i V_j / (F_i - D_j) with the
+ * pole differences formed from DIFL/DIFR (DIFL_j = D_j - F_j, DIFR_j = D_j - F_{j+1}).
+ * Replaces PSMMA_CAUCHYEIG, SRC/psmma_cauchyeig.f90:2-3. */
+void psmma_cauchyeig_(const int* n, const int* k, const int* id, const double* alpha,
+                      const double* a, const int* ia, const int* ja, const int* desca,
+                      const double* f, const double* d, const double* u, const double* v,
+                      const double* difl, const double* difr, const double* beta, double* q,
+                      const int* iq, const int* jq, const int* descq, double* work,
+                      const int* redist);
+
+/* C = alpha*F*B + beta*C, F the unnormalised N x N DFT matrix, complex128 B and C.
+ * Replaces PZMDFTL1, SRC/pzmdftL1.f90:1-2 (RRQR variant), and PZMDFTL,
+ * SRC/pzmdftL.f90 (SVD variant); on one GPU both reduce to the same product. */
+void pzmdftl1_(const int* n, const int* k, const int* nb, const int* kb, const double* alpha,
+               double* b, const int* ldb, const double* beta, double* c, const int* ldc,
+               const int* imrow, const int* imcol, double* work, double* rwork);
+void pzmdftl_(const int* n, const int* k, const int* nb, const int* kb, const double* alpha,
+              double* b, const int* ldb, const double* beta, double* c, const int* ldc,
+              const int* imrow, const int* imcol, double* work, double* rwork);
+
+/* Compute-level routines (reference internals, also external symbols there).
+ * Replace PSCAUCHY_COMPUTE (SRC/pscauchy_compute.f90:2-3), PSTOEPLITZ_COMPUTE
+ * (SRC/pstoeplitz_compute.f90:2-3), PSHANKEL_COMPUTE (SRC/pshankel_compute.f90:2-3)
+ * and PSCAUCHYEIG_COMPUTE (SRC/pscauchyeig_compute.f90:2-3). */
+void pscauchy_compute_(const int* n, const int* k, const double* alpha, const double* a,
+                       const int* lda, const int* desca, const double* b, const int* ldb,
+                       const double* beta, double* c, const int* ldc, const int* imrow,
+                       const int* imcol, double* work);
+void pstoeplitz_compute_(const int* n, const int* k, const double* alpha, const double* a,
+                         const int* lda, const int* desca, const double* b, const int* ldb,
+                         const double* beta, double* c, const int* ldc, const int* imrow,
+                         const int* imcol, double* work);
+void pshankel_compute_(const int* n, const int* k, const double* alpha, const double* a,
+                       const int* lda, const int* desca, const double* b, const int* ldb,
+                       const double* beta, double* c, const int* ldc, const int* imrow,
+                       const int* imcol, double* work);
+void pscauchyeig_compute_(const int* n, const int* k, const int* mb, const int* nb,
+                          const double* alpha, const double* a, const int* lda,
+                          const int* desca, const double* f, const double* d, const double* u,
+                          const double* v, const double* difl, const double* difr,
+                          const double* beta, double* c, const int* ldc, const int* imrow,
+                          const int* imcol, double* work);
+
+/* Numeric form used by the Fortran entry points (default PSMMA_PREC_FP64).
+ * Returns 0, or -1 / -2 for an invalid prec / slice count. */
+int psmma_set_precision(int prec, int slices);
+/* Status of the last Fortran entry call on this thread: 0 or the reference's
+ * INFO code (argument number), or a negative runtime error. */
+int psmma_last_status(void);
+/* Text of the last error on this thread ("" if none). */
+const char* psmma_last_error(void);
+
+/* ===========================================================================
+ * 2. Device API (device pointers, asynchronous on `stream`).
+ *
+ * C(m x n) = alpha * A(m x k) * H(r0:r0+k, c0:c0+n) + beta * C
+ * ======================================================================== */
+
+/* Toeplitz block: H(r,c) = g[korder - 1 + c - r] (0-based), lg = length of g.
+ * Requires 0 <= korder-1+c-r < lg over the block. */
+int psmma_toeplitz_dev(int m, int n, int k, double alpha, const double* a, int lda,
+                       const double* g, int lg, int korder, int r0, int c0, double beta, double* c,
+                       int ldc, int prec, int slices, void* stream);
+
+/* Hankel block: H(r,c) = g[r + c] (0-based).  Requires r0+k-1 + c0+n-1 < lg. */
+int psmma_hankel_dev(int m, int n, int k, double alpha, const double* a, int lda, const double* g,
+                     int lg, int r0, int c0, double beta, double* c, int ldc, int prec, int slices,
+                     void* stream);
+
+/* Cauchy-like block: H(r,c) = u[r] v[c] / (d[r] - f[c]).  d,u need r0+k entries,
+ * f,v need c0+n entries. */
+int psmma_cauchy_dev(int m, int n, int k, double alpha, const double* a, int lda, const double* d,
+                     const double* f, const double* u, const double* v, int r0, int c0,
+                     double beta, double* c, int ldc, int prec, int slices, void* stream);
+
+/* Eigenvector-Cauchy block: H(r,c) = u[r] v[c] / (fpole[r] - lambda_c) with the
+ * difference formed as (fpole[r]-fpole[c]) - difl[c] for r <= c and
+ * (fpole[r]-fpole[c+1]) - difr[c] otherwise.  nglob = global order (length of
+ * fpole, difl, difr, u, v). */
+int psmma_cauchyeig_dev(int m, int n, int k, double alpha, const double* a, int lda,
+                        const double* fpole, const double* u, const double* v, const double* difl,
+                        const double* difr, int nglob, int r0, int c0, double beta, double* c,
+                        int ldc, int prec, int slices, void* stream);
+
+/* DFT block (left-structured, complex128):
+ * C(m x n) = alpha * F(r0:r0+m, c0:c0+k) * B(k x n) + beta * C,
+ * F(r,c) = exp(-2*pi*i * (r*c mod nd) / nd).  prec: PSMMA_PREC_FP64 only. */
+int pzmdft_dev(int m, int n, int k, int nd, double alpha, const double* b, int ldb, int r0, int c0,
+               double beta, double* c, int ldc, int prec, void* stream);
+
+/* Library version string. */
+const char* psmma_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PSMMA_H */

@@ -1,0 +1,235 @@
+"""CPU oracle for the PSMMA structured x dense products.
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by ``__graft_entry__.smoke()`` and
+by the CPU-baseline leg of bench.py, always as the checker, never as the thing
+measured or shipped.
+
+Two restatements of the reference algorithm live here:
+
+* ``psmma_oracle.c`` (loaded through ctypes): the tile generators, index maps
+  and the block-cyclic ring loop of PS*_COMPUTE on an emulated P x Q grid
+  (reference files cited function by function in that source);
+* numpy functions below that materialise a structured matrix explicitly from
+  its generators and multiply it with FP64 BLAS.  They restate the same
+  formulas (cited per function) and serve as the FP64 oracle at sizes where
+  the C ring emulation would be slow.
+
+Seeded input generators for the five benchmark configurations are in
+``inputs.py`` next to this file.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB_PATH = HERE / "_build" / "libpsmma_oracle.so"
+
+CAUCHY, TOEPLITZ, HANKEL, CAUCHYEIG = 0, 1, 2, 3
+MODE_DENSE, MODE_FAITHFUL = 0, 1
+
+
+def build() -> Path:
+    """Compile the C oracle with its Makefile (gcc only)."""
+    subprocess.run(["make", "-s", "-C", str(HERE)], check=True)
+    return LIB_PATH
+
+
+_lib = None
+
+
+def lib() -> ctypes.CDLL:
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            build()
+        _lib = ctypes.CDLL(str(LIB_PATH))
+        _declare(_lib)
+    return _lib
+
+
+class OraProblem(ctypes.Structure):
+    _fields_ = [
+        ("kind", ctypes.c_int), ("nprow", ctypes.c_int), ("npcol", ctypes.c_int),
+        ("nb", ctypes.c_int), ("n", ctypes.c_int), ("k", ctypes.c_int),
+        ("alpha", ctypes.c_double), ("beta", ctypes.c_double),
+        ("a", ctypes.c_void_p), ("lda", ctypes.c_int),
+        ("gen", ctypes.c_void_p), ("ldgen", ctypes.c_int),
+        ("c", ctypes.c_void_p), ("ldc", ctypes.c_int), ("mode", ctypes.c_int),
+    ]
+
+
+def _declare(L):
+    I, D, P = ctypes.c_int, ctypes.c_double, ctypes.c_void_p
+    L.ora_numroc.argtypes = [I, I, I, I]
+    L.ora_numroc.restype = I
+    L.ora_constindex.argtypes = [I, I, I, I, P]
+    L.ora_constindex.restype = I
+    L.ora_is_intersect.argtypes = [I, P, I, P]
+    L.ora_is_intersect.restype = I
+    L.ora_dgemm.argtypes = [I, I, I, D, P, I, P, I, D, P, I]
+    L.ora_dgemm.restype = None
+    L.ora_ps_compute.argtypes = [ctypes.POINTER(OraProblem), I, P]
+    L.ora_ps_compute.restype = I
+    L.ora_structured_gemm.argtypes = [I, I, I, I, D, P, I, P, I, I, I, I, D, P, I]
+    L.ora_structured_gemm.restype = I
+    L.ora_mdgeqp3.argtypes = [I, I, P, I, P, P, D, I, P]
+    L.ora_mdgeqp3.restype = I
+
+
+def _f64(x):
+    return np.asfortranarray(x, dtype=np.float64)
+
+
+def _ptr(x):
+    return x.ctypes.data_as(ctypes.c_void_p)
+
+
+# ---------------------------------------------------------------------------
+# Index maps (C restatement).
+
+def constindex(n: int, nb: int, mycol: int, npcol: int) -> np.ndarray:
+    """SRC/Auxil_2p5D.f90:2-66 (ConstIndex), 1-based global indices."""
+    out = np.zeros(n + nb + 1, dtype=np.int32)
+    ln = lib().ora_constindex(n, nb, mycol, npcol, _ptr(out))
+    return out[:ln].copy()
+
+
+def is_intersect(a, b) -> bool:
+    """SRC/Auxil_2p5D.f90:752-792: TRUE when the sets are DISJOINT (reference sense)."""
+    a = np.ascontiguousarray(a, dtype=np.int32)
+    b = np.ascontiguousarray(b, dtype=np.int32)
+    return bool(lib().ora_is_intersect(len(a), _ptr(a), len(b), _ptr(b)))
+
+
+def numroc(n: int, nb: int, iproc: int, nprocs: int) -> int:
+    return int(lib().ora_numroc(n, nb, iproc, nprocs))
+
+
+# ---------------------------------------------------------------------------
+# Explicit structured matrices (numpy restatement), 0-based block offsets.
+
+def toeplitz_matrix(g, korder: int, r0: int, k: int, c0: int, n: int) -> np.ndarray:
+    """SRC/Auxil_2p5D.f90:224-231: H(i,j) = G(K + j - i), 1-based."""
+    r = np.arange(r0, r0 + k)[:, None]
+    c = np.arange(c0, c0 + n)[None, :]
+    return np.asarray(g, dtype=np.float64)[korder - 1 + c - r]
+
+
+def hankel_matrix(g, r0: int, k: int, c0: int, n: int) -> np.ndarray:
+    """SRC/Auxil_2p5D.f90:309-316: H(i,j) = G(i + j - 1), 1-based."""
+    r = np.arange(r0, r0 + k)[:, None]
+    c = np.arange(c0, c0 + n)[None, :]
+    return np.asarray(g, dtype=np.float64)[r + c]
+
+
+def cauchy_matrix(d, f, u, v, r0: int, k: int, c0: int, n: int) -> np.ndarray:
+    """SRC/Cauchylowrank.f90:509-521: H(i,j) = U_i V_j / (D_i - F_j)."""
+    d = np.asarray(d, np.float64)[r0:r0 + k, None]
+    u = np.asarray(u, np.float64)[r0:r0 + k, None]
+    f = np.asarray(f, np.float64)[None, c0:c0 + n]
+    v = np.asarray(v, np.float64)[None, c0:c0 + n]
+    return (u * v) / (d - f)
+
+
+def cauchyeig_matrix(fpole, u, v, difl, difr, r0: int, k: int, c0: int, n: int) -> np.ndarray:
+    """SRC/CauchyHssEig.f90:1758-1770 (CauchylikeEig) with PL=i, PU=j."""
+    F = np.asarray(fpole, np.float64)
+    r = np.arange(r0, r0 + k)[:, None]
+    c = np.arange(c0, c0 + n)[None, :]
+    cp1 = np.minimum(c + 1, len(F) - 1)
+    den = np.where(r <= c, (F[r] - F[c]) - np.asarray(difl)[c],
+                   (F[r] - F[cp1]) - np.asarray(difr)[c])
+    return (np.asarray(u, np.float64)[r] * np.asarray(v, np.float64)[c]) / den
+
+
+def dft_matrix(nd: int, r0: int, m: int, c0: int, k: int) -> np.ndarray:
+    """F(r,c) = exp(-2 pi i (r c mod nd) / nd), unnormalised (SRC/pzmdftL1.f90:329-337).
+
+    The exponent is reduced modulo nd in exact integer arithmetic before the
+    angle is formed, so every entry is correctly rounded."""
+    r = np.arange(r0, r0 + m, dtype=np.int64)[:, None]
+    c = np.arange(c0, c0 + k, dtype=np.int64)[None, :]
+    e = (r * c) % nd
+    ang = 2.0 * np.pi * e.astype(np.float64) / nd
+    return np.cos(ang) - 1j * np.sin(ang)
+
+
+def structured_gemm_np(kind: int, alpha, a, gens: dict, r0: int, c0: int, n: int, beta=0.0,
+                       c=None) -> np.ndarray:
+    """C = alpha * A * H(r0:r0+k, c0:c0+n) + beta * C in FP64 with an explicit H."""
+    a = np.asarray(a, np.float64)
+    k = a.shape[1]
+    if kind == TOEPLITZ:
+        h = toeplitz_matrix(gens["g"], gens["korder"], r0, k, c0, n)
+    elif kind == HANKEL:
+        h = hankel_matrix(gens["g"], r0, k, c0, n)
+    elif kind == CAUCHY:
+        h = cauchy_matrix(gens["d"], gens["f"], gens["u"], gens["v"], r0, k, c0, n)
+    else:
+        h = cauchyeig_matrix(gens["f"], gens["u"], gens["v"], gens["difl"], gens["difr"],
+                             r0, k, c0, n)
+    out = alpha * (a @ h)
+    if beta != 0.0:
+        out = out + beta * np.asarray(c, np.float64)
+    return out
+
+
+# ---------------------------------------------------------------------------
+# C restatement entry points.
+
+def structured_gemm_c(kind: int, alpha, a, gen, korder: int, r0: int, c0: int, n: int,
+                      beta=0.0, c=None) -> np.ndarray:
+    """Dense step of PS*_COMPUTE on a 1 x 1 grid (C restatement)."""
+    a = _f64(a)
+    m, k = a.shape
+    gen = _f64(gen)
+    ldgen = gen.shape[0]
+    out = _f64(c) if c is not None else np.zeros((m, n), order="F")
+    out = out.copy(order="F")
+    st = lib().ora_structured_gemm(kind, m, n, k, float(alpha), _ptr(a), max(1, m), _ptr(gen),
+                                   ldgen, korder, r0, c0, float(beta), _ptr(out), max(1, m))
+    if st != 0:
+        raise RuntimeError(f"ora_structured_gemm failed ({st})")
+    return out
+
+
+def ps_compute(kind: int, nprow: int, npcol: int, nb: int, alpha, a, gen, beta=0.0, c=None,
+               mode: int = MODE_DENSE, nthreads: int = 1):
+    """PS*_COMPUTE over an emulated nprow x npcol grid (C restatement).
+
+    Returns (C, stats) with stats = [dense, lowrank, fullrank, stale] step counts."""
+    a = _f64(a)
+    n, k = a.shape
+    gen = _f64(gen)
+    out = _f64(c).copy(order="F") if c is not None else np.zeros((n, k), order="F")
+    prob = OraProblem(kind, nprow, npcol, nb, n, k, float(alpha), float(beta), _ptr(a).value,
+                      max(1, n), _ptr(gen).value, gen.shape[0], _ptr(out).value, max(1, n), mode)
+    stats = np.zeros(4, dtype=np.int64)
+    st = lib().ora_ps_compute(ctypes.byref(prob), int(nthreads), _ptr(stats))
+    if st != 0:
+        raise RuntimeError(f"ora_ps_compute failed ({st})")
+    return out, stats
+
+
+def dgemm_c(alpha, a, b, beta=0.0, c=None) -> np.ndarray:
+    a = _f64(a)
+    b = _f64(b)
+    m, k = a.shape
+    n = b.shape[1]
+    out = _f64(c).copy(order="F") if c is not None else np.zeros((m, n), order="F")
+    lib().ora_dgemm(m, n, k, float(alpha), _ptr(a), max(1, m), _ptr(b), max(1, k), float(beta),
+                    _ptr(out), max(1, m))
+    return out
+
+
+def rel_fro(x, ref) -> float:
+    """Relative Frobenius error ||x - ref||_F / ||ref||_F."""
+    x = np.asarray(x)
+    ref = np.asarray(ref)
+    den = np.linalg.norm(ref)
+    return float(np.linalg.norm(x - ref) / (den if den > 0 else 1.0))

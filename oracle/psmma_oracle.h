@@ -1,0 +1,55 @@
+/* psmma_oracle.h - CPU restatement of the reference PSMMA algorithm.
+ * TEST INFRASTRUCTURE ONLY: used by tests/ and by bench.py's CPU baseline. */
+#ifndef PSMMA_ORACLE_H
+#define PSMMA_ORACLE_H
+
+enum { ORA_CAUCHY = 0, ORA_TOEPLITZ = 1, ORA_HANKEL = 2, ORA_CAUCHYEIG = 3 };
+enum { ORA_MODE_DENSE = 0, ORA_MODE_FAITHFUL = 1 };
+enum { ORA_TILE_OK = 0, ORA_TILE_STALE = 1 };
+
+/* One PS*_COMPUTE call on an emulated nprow x npcol grid with global arrays. */
+typedef struct {
+  int kind;           /* ORA_* */
+  int nprow, npcol;   /* emulated BLACS grid */
+  int nb;             /* block size (MB = NB = KB, as the reference assumes) */
+  int n, k;           /* A is n x k, H is k x k, C is n x k */
+  double alpha, beta;
+  const double* a;    /* global A, leading dimension lda */
+  int lda;
+  const double* gen;  /* Cauchy: B(ldgen,4)=[D F U V]; Toeplitz/Hankel: G;
+                         CauchyEig: [F D U V DIFL DIFR] stacked with stride ldgen */
+  int ldgen;
+  double* c;          /* global C, leading dimension ldc */
+  int ldc;
+  int mode;           /* ORA_MODE_DENSE or ORA_MODE_FAITHFUL */
+} ora_problem;
+
+int ora_numroc(int n, int nb, int iproc, int nprocs);
+int ora_constindex(int n, int nb, int mycol, int npcol, int* idx);
+int ora_is_intersect(int m, const int* seta, int n, const int* setb);
+void ora_const_cauchy(int lm, int ln, const double* b, int ldb, double* t, const int* ridx,
+                      const int* cidx);
+void ora_const_toeplitz(int lm, int ln, const double* g, int k, double* t, const int* ridx,
+                        const int* cidx);
+void ora_const_hankel(int lm, int ln, const double* g, double* t, const int* ridx,
+                      const int* cidx);
+void ora_const_cauchyeig(int lm, int ln, double* t, const double* f, const double* u,
+                         const double* v, const double* difl, const double* difr,
+                         const int* ridx, const int* cidx);
+void ora_dgemm(int m, int n, int k, double alpha, const double* a, int lda, const double* b,
+               int ldb, double beta, double* c, int ldc);
+int ora_ps_compute(const ora_problem* p, int nthreads, long long* stats);
+int ora_structured_gemm(int kind, int m, int n, int k, double alpha, const double* a, int lda,
+                        const double* gen, int ldgen, int korder, int r0, int c0, double beta,
+                        double* c, int ldc);
+
+/* Low-rank branch of one off-diagonal ring step (faithful mode): fills
+ * hbuf with [X(kbp x rk) | Y(rk x nq)] or, at full rank, with the tile the
+ * reference would multiply.  Returns ORA_TILE_OK / ORA_TILE_STALE or < 0. */
+int ora_lowrank_tile(const ora_problem* p, int kbp, int nq, const int* ridx, const int* cidx,
+                     double* hbuf, unsigned long hcap, int* rk);
+
+/* Truncated pivoted QR (SRC/mdgeqp3.f): see psmma_oracle_lowrank.c. */
+int ora_mdgeqp3(int m, int n, double* a, int lda, int* jpvt, double* tau, double tol, int nb,
+                int* rk);
+#endif

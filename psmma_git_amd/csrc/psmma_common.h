@@ -1,0 +1,98 @@
+// Shared device-side definitions for the PSMMA structured x dense kernels (gfx950).
+//
+// Conventions (all kernels):
+//   * Matrices are column-major with an explicit leading dimension, as in the
+//     Fortran entry points they serve.
+//   * A structured matrix H is never materialised in HBM: each kernel evaluates
+//     the entries of the K-tile it is about to consume from the O(n) generators
+//     and writes them straight into LDS, where the MFMA fragments read them.
+//   * Row/column offsets (r0, c0) select a sub-block of the global structured
+//     matrix, so a caller can process one block of a larger product (a ring step,
+//     one problem of a batch, one HSS block).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+typedef double v4d __attribute__((ext_vector_type(4)));
+typedef float v4f __attribute__((ext_vector_type(4)));
+typedef int v4i __attribute__((ext_vector_type(4)));
+
+namespace psmma {
+
+// Kinds of structured operand.  Values are part of the C ABI (include/psmma.h).
+enum Kind : int {
+  KIND_CAUCHY = 0,     // H(i,j) = U_i V_j / (D_i - F_j)                 psmma_cauchy
+  KIND_TOEPLITZ = 1,   // H(i,j) = G(K + j - i)  (1-based, G length 2K-1) psmma_toeplitz
+  KIND_HANKEL = 2,     // H(i,j) = G(i + j - 1)                           psmma_hankel
+  KIND_CAUCHYEIG = 3,  // H(i,j) = U_i V_j / (F_i - lambda_j), stable form psmma_cauchyeig
+};
+
+// Generators of one structured operand.  Pointers are device pointers.
+//   Cauchy:     p0 = D, p1 = F, p2 = U, p3 = V                  (columns of B(LDB,4))
+//   Toeplitz:   p0 = G (length 2K-1), korder = K
+//   Hankel:     p0 = G (length >= rows + cols - 1)
+//   CauchyEig:  p0 = F (old eigenvalues, poles), p1 = U, p2 = V, p3 = DIFL, p4 = DIFR
+struct GenPtrs {
+  const double* p0;
+  const double* p1;
+  const double* p2;
+  const double* p3;
+  const double* p4;
+  int korder;  // order K of the Toeplitz matrix (index shift); unused otherwise
+  int nrows;   // global row count of H (bounds for generator reads)
+  int ncols;   // global column count of H
+};
+
+// Entry H(r, c) of the structured matrix, 0-based global indices.
+template <int KIND>
+__device__ __forceinline__ double gen_entry(const GenPtrs& g, int r, int c);
+
+// SRC/Cauchylowrank.f90:509-521 (Cauchylike), via SRC/Auxil_2p5D.f90:138-144 (ConstCauchy).
+template <>
+__device__ __forceinline__ double gen_entry<KIND_CAUCHY>(const GenPtrs& g, int r, int c) {
+  return (g.p2[r] * g.p3[c]) / (g.p0[r] - g.p1[c]);
+}
+
+// SRC/Auxil_2p5D.f90:224-231 (ConstToeplitz): T(i,j) = G(K+J1-I1), 1-based.
+template <>
+__device__ __forceinline__ double gen_entry<KIND_TOEPLITZ>(const GenPtrs& g, int r, int c) {
+  return g.p0[g.korder - 1 + c - r];
+}
+
+// SRC/Auxil_2p5D.f90:309-316 (ConstHankel): T(i,j) = G(I1+J1-1), 1-based.
+template <>
+__device__ __forceinline__ double gen_entry<KIND_HANKEL>(const GenPtrs& g, int r, int c) {
+  return g.p0[r + c];
+}
+
+// SRC/CauchyHssEig.f90:1758-1770 (CauchylikeEig) with PL = r+1, PU = c+1:
+// the pole difference d_r - lambda_c is formed as (F_r - F_c) - DIFL_c when
+// r <= c and as (F_r - F_{c+1}) - DIFR_c otherwise, never from lambda itself.
+template <>
+__device__ __forceinline__ double gen_entry<KIND_CAUCHYEIG>(const GenPtrs& g, int r, int c) {
+  const double* F = g.p0;
+  double den = (r <= c) ? ((F[r] - F[c]) - g.p3[c]) : ((F[r] - F[c + 1]) - g.p4[c]);
+  return (g.p1[r] * g.p2[c]) / den;
+}
+
+__device__ __forceinline__ double gen_entry_rt(int kind, const GenPtrs& g, int r, int c) {
+  switch (kind) {
+    case KIND_CAUCHY: return gen_entry<KIND_CAUCHY>(g, r, c);
+    case KIND_TOEPLITZ: return gen_entry<KIND_TOEPLITZ>(g, r, c);
+    case KIND_HANKEL: return gen_entry<KIND_HANKEL>(g, r, c);
+    default: return gen_entry<KIND_CAUCHYEIG>(g, r, c);
+  }
+}
+
+// Remap a linear workgroup id so that consecutive output tiles land on the same
+// XCD.  Workgroups are dispatched round-robin over the 8 XCDs of an MI355X; tiles
+// in one row of tiles share the same rows of the dense operand, so keeping them
+// on one XCD keeps those rows in that XCD's L2.
+__device__ __forceinline__ int xcd_remap(int bid, int nblocks) {
+  constexpr int NXCD = 8;
+  if (nblocks % NXCD != 0) return bid;
+  int per = nblocks / NXCD;
+  return (bid % NXCD) * per + bid / NXCD;
+}
+
+}  // namespace psmma

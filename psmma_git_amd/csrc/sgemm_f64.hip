@@ -1,0 +1,395 @@
+// Plain FP64 structured x dense products on gfx950 FP64 MFMA.
+//
+//   right-structured (psmma_cauchy/toeplitz/hankel/cauchyeig):
+//       C(MxN) = alpha * A(MxK) * H(KxN) + beta * C,   H generated from generators
+//   left-structured  (pzmdftL):
+//       C(MxN) = alpha * F(MxK) * B(KxN) + beta * C,   F(r,c) = w^(r*c mod Nd), complex
+//
+// Both replace the reference's "build tile into WORK(IPB), then DGEMM/ZGEMM"
+// step (SRC/pscauchy_compute.f90:395-407, SRC/pstoeplitz_compute.f90:290-293,
+// SRC/pshankel_compute.f90:249-250, SRC/pscauchyeig_compute.f90:399-402,
+// SRC/pzmdftL1.f90:490-496) with one kernel in which the structured tile only
+// ever exists in LDS.
+//
+// Tiling: a workgroup of NW = WGM*WGN wave64s owns a BM x BN tile of C.  Each
+// wave owns a (BM/WGM) x (BN/WGN) sub-tile held in v_mfma_f64_16x16x4f64
+// accumulators.  Per BK-slab of the contraction the dense operand is loaded
+// global->registers (prefetched one slab ahead), the structured operand is
+// evaluated from its generators, and both are written to a double-buffered LDS
+// pair; one barrier per slab.
+#include "psmma_common.h"
+
+namespace psmma {
+
+// MFMA f64 16x16x4 operand/result lane layout used throughout:
+//   A fragment: lane l supplies A[l%16][k-slot l/16]
+//   B fragment: lane l supplies B[k-slot l/16][l%16]
+//   D:          lane l, element r holds D[4*(l/16)+r][l%16]
+// (The k-slot assignment only has to agree between A and B.)
+
+template <int KIND, int BM, int BN, int BK, int WGM, int WGN>
+__global__ __launch_bounds__(WGM* WGN * 64, 2) void sgemm_f64_right_kernel(
+    int M, int N, int K, double alpha, const double* __restrict__ A, int lda, GenPtrs g, int r0,
+    int c0, double beta, double* __restrict__ C, int ldc) {
+  constexpr int NT = WGM * WGN * 64;
+  constexpr int WM = BM / WGM, WN = BN / WGN;
+  constexpr int TM = WM / 16, TN = WN / 16;
+  constexpr int APAD = 16, HPAD = 16;
+  constexpr int ALD = BM + APAD, HLD = BN + HPAD;
+  constexpr int AREG = BM * BK / NT;  // dense elements staged per thread per slab
+  constexpr int HREG = BK * BN / NT;  // structured elements per thread per slab
+  static_assert(BM * BK % NT == 0 && BK * BN % NT == 0, "tile/threads mismatch");
+  static_assert(WM % 16 == 0 && WN % 16 == 0 && BK % 4 == 0, "bad wave tile");
+
+  __shared__ double As[2][BK * ALD];
+  __shared__ double Hs[2][BK * HLD];
+
+  const int nbn = (N + BN - 1) / BN;
+  const int nbm = (M + BM - 1) / BM;
+  const int bid = xcd_remap(blockIdx.x, nbn * nbm);
+  const int bm = bid / nbn, bn = bid % nbn;
+  const int m0 = bm * BM, n0 = bn * BN;
+
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
+  const int wm0 = (wave / WGN) * WM, wn0 = (wave % WGN) * WN;
+  const int lr = lane & 15, lk = lane >> 4;
+
+  v4d acc[TM][TN];
+#pragma unroll
+  for (int i = 0; i < TM; ++i)
+#pragma unroll
+    for (int j = 0; j < TN; ++j) acc[i][j] = (v4d){0.0, 0.0, 0.0, 0.0};
+
+  double areg[AREG];
+  double hreg[HREG];
+
+  auto load_a = [&](int k0) {
+#pragma unroll
+    for (int q = 0; q < AREG; ++q) {
+      int idx = tid + q * NT;
+      int row = idx % BM, col = idx / BM;
+      int gr = m0 + row, gc = k0 + col;
+      areg[q] = (gr < M && gc < K) ? A[(size_t)gc * lda + gr] : 0.0;
+    }
+  };
+  auto gen_h = [&](int k0) {
+#pragma unroll
+    for (int q = 0; q < HREG; ++q) {
+      int idx = tid + q * NT;
+      int col = idx % BN, row = idx / BN;
+      int gk = k0 + row, gn = n0 + col;
+      hreg[q] = (gk < K && gn < N) ? gen_entry<KIND>(g, r0 + gk, c0 + gn) : 0.0;
+    }
+  };
+  auto store = [&](int buf) {
+#pragma unroll
+    for (int q = 0; q < AREG; ++q) {
+      int idx = tid + q * NT;
+      As[buf][(idx / BM) * ALD + idx % BM] = areg[q];
+    }
+#pragma unroll
+    for (int q = 0; q < HREG; ++q) {
+      int idx = tid + q * NT;
+      Hs[buf][(idx / BN) * HLD + idx % BN] = hreg[q];
+    }
+  };
+
+  const int nkt = (K + BK - 1) / BK;
+  if (nkt > 0) {
+    load_a(0);
+    gen_h(0);
+    store(0);
+  }
+  __syncthreads();
+
+  for (int kt = 0; kt < nkt; ++kt) {
+    const int cur = kt & 1;
+    const bool more = (kt + 1) < nkt;
+    if (more) load_a((kt + 1) * BK);
+    const double* as = As[cur];
+    const double* hs = Hs[cur];
+#pragma unroll
+    for (int kk = 0; kk < BK; kk += 4) {
+      double af[TM], bf[TN];
+#pragma unroll
+      for (int i = 0; i < TM; ++i) af[i] = as[(kk + lk) * ALD + wm0 + i * 16 + lr];
+#pragma unroll
+      for (int j = 0; j < TN; ++j) bf[j] = hs[(kk + lk) * HLD + wn0 + j * 16 + lr];
+#pragma unroll
+      for (int i = 0; i < TM; ++i)
+#pragma unroll
+        for (int j = 0; j < TN; ++j)
+          acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[i], bf[j], acc[i][j], 0, 0, 0);
+    }
+    if (more) {
+      gen_h((kt + 1) * BK);
+      store(cur ^ 1);
+    }
+    __syncthreads();
+  }
+
+  // Epilogue: C = alpha*acc + beta*C.  C is not read when beta == 0 (BLAS rule).
+#pragma unroll
+  for (int i = 0; i < TM; ++i) {
+#pragma unroll
+    for (int j = 0; j < TN; ++j) {
+      int gc = n0 + wn0 + j * 16 + lr;
+      if (gc >= N) continue;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        int gr = m0 + wm0 + i * 16 + lk * 4 + r;
+        if (gr < M) {
+          double* p = C + (size_t)gc * ldc + gr;
+          double v = alpha * acc[i][j][r];
+          if (beta != 0.0) v += beta * (*p);
+          *p = v;
+        }
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Left-structured complex DFT product.  F(r,c) = w^((r0+r)*(c0+c) mod Nd) with
+// w = exp(-2*pi*i/Nd) (SRC/pzmdftL1.f90:329-337, unnormalised); the twiddles
+// come from an FP64 table tw[e] = (cos(2*pi*e/Nd), -sin(2*pi*e/Nd)) so every
+// entry is correctly rounded instead of accumulating the error of Wn**PowK.
+// B and C are complex128, interleaved (re, im), column-major, ld in complex
+// elements.  alpha and beta are real, as in the reference's argument list.
+template <int BM, int BN, int BK, int WGM, int WGN>
+__global__ __launch_bounds__(WGM* WGN * 64, 2) void dft_f64_left_kernel(
+    int M, int N, int K, int Nd, double alpha, const double2* __restrict__ tw,
+    const double2* __restrict__ B, int ldb, int r0, int c0, double beta, double2* __restrict__ C,
+    int ldc) {
+  constexpr int NT = WGM * WGN * 64;
+  constexpr int WM = BM / WGM, WN = BN / WGN;
+  constexpr int TM = WM / 16, TN = WN / 16;
+  constexpr int PAD = 16;
+  constexpr int FLD = BM + PAD, BLD = BN + PAD;
+  constexpr int FREG = BM * BK / NT;
+  constexpr int BREG = BK * BN / NT;
+  static_assert(BM * BK % NT == 0 && BK * BN % NT == 0, "tile/threads mismatch");
+  static_assert(NT % BM == 0, "F generator assumes fixed row per thread");
+
+  __shared__ double Fr[2][BK * FLD];
+  __shared__ double Fi[2][BK * FLD];
+  __shared__ double Br[2][BK * BLD];
+  __shared__ double Bi[2][BK * BLD];
+
+  const int nbn = (N + BN - 1) / BN;
+  const int nbm = (M + BM - 1) / BM;
+  const int bid = xcd_remap(blockIdx.x, nbn * nbm);
+  const int bm = bid / nbn, bn = bid % nbn;
+  const int m0 = bm * BM, n0 = bn * BN;
+
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
+  const int wm0 = (wave / WGN) * WM, wn0 = (wave % WGN) * WN;
+  const int lr = lane & 15, lk = lane >> 4;
+
+  v4d accr[TM][TN], acci[TM][TN];
+#pragma unroll
+  for (int i = 0; i < TM; ++i)
+#pragma unroll
+    for (int j = 0; j < TN; ++j) {
+      accr[i][j] = (v4d){0.0, 0.0, 0.0, 0.0};
+      acci[i][j] = (v4d){0.0, 0.0, 0.0, 0.0};
+    }
+
+  // Each thread generates F for one fixed row (tid % BM) and FREG columns
+  // (tid / BM + q * NT/BM).  It keeps the exponents e_q = row*col mod Nd and
+  // advances them by row*BK mod Nd per slab with one add and one compare.
+  const int frow = tid % BM;
+  const int grow = m0 + frow;
+  const unsigned nd = (unsigned)Nd;
+  const unsigned rmod = (unsigned)(((long long)(r0 + grow) % Nd + Nd) % Nd);
+  const unsigned step = (unsigned)(((unsigned long long)rmod * BK) % nd);
+  unsigned ex[FREG];
+#pragma unroll
+  for (int q = 0; q < FREG; ++q) {
+    int col = tid / BM + q * (NT / BM);
+    ex[q] = (unsigned)(((unsigned long long)rmod * (unsigned long long)(c0 + col)) % nd);
+  }
+
+  double2 freg[FREG];
+  double2 breg[BREG];
+  auto gen_f = [&](int k0) {
+#pragma unroll
+    for (int q = 0; q < FREG; ++q) {
+      int col = tid / BM + q * (NT / BM);
+      bool ok = (grow < M) && (k0 + col < K);
+      freg[q] = ok ? tw[ex[q]] : make_double2(0.0, 0.0);
+    }
+  };
+  auto advance_f = [&]() {
+#pragma unroll
+    for (int q = 0; q < FREG; ++q) {
+      unsigned e = ex[q] + step;
+      ex[q] = (e >= nd) ? e - nd : e;
+    }
+  };
+  auto load_b = [&](int k0) {
+#pragma unroll
+    for (int q = 0; q < BREG; ++q) {
+      int idx = tid + q * NT;
+      int row = idx % BK, col = idx / BK;  // consecutive threads walk down a column
+      int gk = k0 + row, gn = n0 + col;
+      breg[q] = (gk < K && gn < N) ? B[(size_t)gn * ldb + gk] : make_double2(0.0, 0.0);
+    }
+  };
+  auto store = [&](int buf) {
+#pragma unroll
+    for (int q = 0; q < FREG; ++q) {
+      int col = tid / BM + q * (NT / BM);
+      Fr[buf][col * FLD + frow] = freg[q].x;
+      Fi[buf][col * FLD + frow] = freg[q].y;
+    }
+#pragma unroll
+    for (int q = 0; q < BREG; ++q) {
+      int idx = tid + q * NT;
+      int row = idx % BK, col = idx / BK;
+      Br[buf][row * BLD + col] = breg[q].x;
+      Bi[buf][row * BLD + col] = breg[q].y;
+    }
+  };
+
+  const int nkt = (K + BK - 1) / BK;
+  if (nkt > 0) {
+    gen_f(0);
+    load_b(0);
+    store(0);
+  }
+  __syncthreads();
+
+  for (int kt = 0; kt < nkt; ++kt) {
+    const int cur = kt & 1;
+    const bool more = (kt + 1) < nkt;
+    if (more) {
+      advance_f();
+      gen_f((kt + 1) * BK);
+      load_b((kt + 1) * BK);
+    }
+#pragma unroll
+    for (int kk = 0; kk < BK; kk += 4) {
+      double ar[TM], ai[TM], an[TM], br[TN], bi[TN];
+#pragma unroll
+      for (int i = 0; i < TM; ++i) {
+        ar[i] = Fr[cur][(kk + lk) * FLD + wm0 + i * 16 + lr];
+        ai[i] = Fi[cur][(kk + lk) * FLD + wm0 + i * 16 + lr];
+        an[i] = -ai[i];
+      }
+#pragma unroll
+      for (int j = 0; j < TN; ++j) {
+        br[j] = Br[cur][(kk + lk) * BLD + wn0 + j * 16 + lr];
+        bi[j] = Bi[cur][(kk + lk) * BLD + wn0 + j * 16 + lr];
+      }
+#pragma unroll
+      for (int i = 0; i < TM; ++i)
+#pragma unroll
+        for (int j = 0; j < TN; ++j) {
+          accr[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[i], br[j], accr[i][j], 0, 0, 0);
+          accr[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(an[i], bi[j], accr[i][j], 0, 0, 0);
+          acci[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[i], bi[j], acci[i][j], 0, 0, 0);
+          acci[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[i], br[j], acci[i][j], 0, 0, 0);
+        }
+    }
+    if (more) store(cur ^ 1);
+    __syncthreads();
+  }
+
+#pragma unroll
+  for (int i = 0; i < TM; ++i) {
+#pragma unroll
+    for (int j = 0; j < TN; ++j) {
+      int gc = n0 + wn0 + j * 16 + lr;
+      if (gc >= N) continue;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        int gr = m0 + wm0 + i * 16 + lk * 4 + r;
+        if (gr < M) {
+          double2* p = C + (size_t)gc * ldc + gr;
+          double2 v = make_double2(alpha * accr[i][j][r], alpha * acci[i][j][r]);
+          if (beta != 0.0) {
+            double2 o = *p;
+            v.x += beta * o.x;
+            v.y += beta * o.y;
+          }
+          *p = v;
+        }
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Host-side launchers (called from the C ABI in psmma_capi.hip).
+
+struct RightTile {
+  int bm, bn;
+};
+
+template <int KIND, int BM, int BN, int BK, int WGM, int WGN>
+static hipError_t launch_right_t(int M, int N, int K, double alpha, const double* A, int lda,
+                                 const GenPtrs& g, int r0, int c0, double beta, double* C, int ldc,
+                                 hipStream_t s) {
+  int nb = ((M + BM - 1) / BM) * ((N + BN - 1) / BN);
+  hipLaunchKernelGGL((sgemm_f64_right_kernel<KIND, BM, BN, BK, WGM, WGN>), dim3(nb),
+                     dim3(WGM * WGN * 64), 0, s, M, N, K, alpha, A, lda, g, r0, c0, beta, C, ldc);
+  return hipGetLastError();
+}
+
+template <int KIND>
+static hipError_t launch_right_kind(int cfg, int M, int N, int K, double alpha, const double* A,
+                                    int lda, const GenPtrs& g, int r0, int c0, double beta,
+                                    double* C, int ldc, hipStream_t s) {
+  switch (cfg) {
+    case 0:  // large problems: 128x128 workgroup tile, 4 waves of 64x64
+      return launch_right_t<KIND, 128, 128, 16, 2, 2>(M, N, K, alpha, A, lda, g, r0, c0, beta, C,
+                                                      ldc, s);
+    case 1:  // 64x256: fewer dense-operand re-reads per flop
+      return launch_right_t<KIND, 64, 256, 16, 1, 4>(M, N, K, alpha, A, lda, g, r0, c0, beta, C,
+                                                     ldc, s);
+    default:  // small problems: 64x64 workgroup tile, 4 waves of 32x32
+      return launch_right_t<KIND, 64, 64, 16, 2, 2>(M, N, K, alpha, A, lda, g, r0, c0, beta, C,
+                                                    ldc, s);
+  }
+}
+
+// Picks a tile shape: the big tile when it still yields >= 2 workgroups per CU.
+int choose_right_cfg(int M, int N) {
+  long long big = (long long)((M + 127) / 128) * ((N + 127) / 128);
+  if (big >= 512) return 0;
+  return 2;
+}
+
+hipError_t launch_right_f64(int kind, int cfg, int M, int N, int K, double alpha, const double* A,
+                            int lda, const GenPtrs& g, int r0, int c0, double beta, double* C,
+                            int ldc, hipStream_t s) {
+  if (cfg < 0) cfg = choose_right_cfg(M, N);
+  switch (kind) {
+    case KIND_CAUCHY:
+      return launch_right_kind<KIND_CAUCHY>(cfg, M, N, K, alpha, A, lda, g, r0, c0, beta, C, ldc, s);
+    case KIND_TOEPLITZ:
+      return launch_right_kind<KIND_TOEPLITZ>(cfg, M, N, K, alpha, A, lda, g, r0, c0, beta, C, ldc,
+                                              s);
+    case KIND_HANKEL:
+      return launch_right_kind<KIND_HANKEL>(cfg, M, N, K, alpha, A, lda, g, r0, c0, beta, C, ldc, s);
+    case KIND_CAUCHYEIG:
+      return launch_right_kind<KIND_CAUCHYEIG>(cfg, M, N, K, alpha, A, lda, g, r0, c0, beta, C, ldc,
+                                               s);
+  }
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_dft_f64(int M, int N, int K, int Nd, double alpha, const double2* tw,
+                          const double2* B, int ldb, int r0, int c0, double beta, double2* C, int ldc,
+                          hipStream_t s) {
+  constexpr int BM = 64, BN = 64, BK = 16, WGM = 2, WGN = 2;
+  int nb = ((M + BM - 1) / BM) * ((N + BN - 1) / BN);
+  hipLaunchKernelGGL((dft_f64_left_kernel<BM, BN, BK, WGM, WGN>), dim3(nb), dim3(WGM * WGN * 64), 0,
+                     s, M, N, K, Nd, alpha, tw, B, ldb, r0, c0, beta, C, ldc);
+  return hipGetLastError();
+}
+
+}  // namespace psmma
