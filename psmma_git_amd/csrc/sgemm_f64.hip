@@ -24,7 +24,8 @@ namespace psmma {
 // MFMA f64 16x16x4 operand/result lane layout used throughout:
 //   A fragment: lane l supplies A[l%16][k-slot l/16]
 //   B fragment: lane l supplies B[k-slot l/16][l%16]
-//   D:          lane l, element r holds D[4*(l/16)+r][l%16]
+//   D:          lane l, element r holds D[l/16 + 4*r][l%16]
+// (measured on gfx950; the bf16/int8 16x16 forms instead hold D[4*(l/16)+r][l%16]).
 // (The k-slot assignment only has to agree between A and B.)
 
 template <int KIND, int BM, int BN, int BK, int WGM, int WGN>
@@ -138,7 +139,7 @@ __global__ __launch_bounds__(WGM* WGN * 64, 2) void sgemm_f64_right_kernel(
       if (gc >= N) continue;
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        int gr = m0 + wm0 + i * 16 + lk * 4 + r;
+        int gr = m0 + wm0 + i * 16 + lk + 4 * r;
         if (gr < M) {
           double* p = C + (size_t)gc * ldc + gr;
           double v = alpha * acc[i][j][r];
@@ -306,7 +307,7 @@ __global__ __launch_bounds__(WGM* WGN * 64, 2) void dft_f64_left_kernel(
       if (gc >= N) continue;
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        int gr = m0 + wm0 + i * 16 + lk * 4 + r;
+        int gr = m0 + wm0 + i * 16 + lk + 4 * r;
         if (gr < M) {
           double2* p = C + (size_t)gc * ldc + gr;
           double2 v = make_double2(alpha * accr[i][j][r], alpha * acci[i][j][r]);
