@@ -17,6 +17,8 @@
 // global->registers (prefetched one slab ahead), the structured operand is
 // evaluated from its generators, and both are written to a double-buffered LDS
 // pair; one barrier per slab.
+#include <cstdlib>
+
 #include "psmma_common.h"
 
 namespace psmma {
@@ -38,12 +40,19 @@ __global__ __launch_bounds__(WGM* WGN * 64, 2) void sgemm_f64_right_kernel(
   constexpr int APAD = 16, HPAD = 16;
   constexpr int ALD = BM + APAD, HLD = BN + HPAD;
   constexpr int AREG = BM * BK / NT;  // dense elements staged per thread per slab
-  constexpr int HREG = BK * BN / NT;  // structured elements per thread per slab
+  // Toeplitz and Hankel tiles are functions of (col - row) / (row + col): one
+  // slab needs only a window of BK+BN-1 generator values.
+  constexpr bool WIN = (KIND == KIND_TOEPLITZ || KIND == KIND_HANKEL);
+  constexpr int WLEN = BK + BN - 1;
+  constexpr int WREG = (WLEN + NT - 1) / NT;
+  constexpr int HREG = WIN ? 1 : BK * BN / NT;  // generated entries per thread per slab
+  constexpr int HSZ = WIN ? WLEN + 1 : BK * HLD;
   static_assert(BM * BK % NT == 0 && BK * BN % NT == 0, "tile/threads mismatch");
+  static_assert(WIN || NT % BN == 0, "generated tiles assume a fixed column per thread");
   static_assert(WM % 16 == 0 && WN % 16 == 0 && BK % 4 == 0, "bad wave tile");
 
   __shared__ double As[2][BK * ALD];
-  __shared__ double Hs[2][BK * HLD];
+  __shared__ double Hs[2][HSZ];
 
   const int nbn = (N + BN - 1) / BN;
   const int nbm = (M + BM - 1) / BM;
@@ -63,7 +72,41 @@ __global__ __launch_bounds__(WGM* WGN * 64, 2) void sgemm_f64_right_kernel(
     for (int j = 0; j < TN; ++j) acc[i][j] = (v4d){0.0, 0.0, 0.0, 0.0};
 
   double areg[AREG];
+  double wreg[WREG];     // window values (Toeplitz/Hankel)
+  double rowa[HREG], rowb[HREG];  // row generators (Cauchy kinds)
   double hreg[HREG];
+
+  // Generator index range of the whole block: window slots outside it are only
+  // used by padded (row, col) pairs and are set to 0 so 0 * value stays 0.
+  long long glo = 0, ghi = -1;
+  if constexpr (KIND == KIND_TOEPLITZ) {
+    glo = (long long)g.korder - 1 + c0 - (r0 + K - 1);
+    ghi = (long long)g.korder - 1 + (c0 + N - 1) - r0;
+  } else if constexpr (KIND == KIND_HANKEL) {
+    glo = (long long)r0 + c0;
+    ghi = (long long)r0 + K - 1 + c0 + N - 1;
+  }
+
+  // Column invariants of the Cauchy kinds: this thread always generates column
+  // hcol of every slab.
+  const int hcol = tid % BN;
+  const int gcol = c0 + n0 + hcol;
+  const bool colok = (n0 + hcol) < N;
+  double cv0 = 0.0, cv1 = 0.0, cv2 = 0.0, cv3 = 0.0, cv4 = 0.0;
+  if constexpr (KIND == KIND_CAUCHY) {
+    if (colok) {
+      cv0 = g.p1[gcol];  // F_c
+      cv1 = g.p3[gcol];  // V_c
+    }
+  } else if constexpr (KIND == KIND_CAUCHYEIG) {
+    if (colok) {
+      cv0 = g.p0[gcol];                                   // F_c
+      cv1 = (gcol + 1 < g.korder) ? g.p0[gcol + 1] : 0.0;  // F_{c+1}
+      cv2 = g.p2[gcol];                                   // V_c
+      cv3 = g.p3[gcol];                                   // DIFL_c
+      cv4 = g.p4[gcol];                                   // DIFR_c
+    }
+  }
 
   auto load_a = [&](int k0) {
 #pragma unroll
@@ -74,13 +117,60 @@ __global__ __launch_bounds__(WGM* WGN * 64, 2) void sgemm_f64_right_kernel(
       areg[q] = (gr < M && gc < K) ? A[(size_t)gc * lda + gr] : 0.0;
     }
   };
-  auto gen_h = [&](int k0) {
+  // Issue the generator loads of slab k0 (consumed after the MFMA loop).
+  auto fetch_h = [&](int k0) {
+    if constexpr (WIN) {
+      long long base;
+      if constexpr (KIND == KIND_TOEPLITZ)
+        base = (long long)g.korder - 1 + (c0 + n0) - (r0 + k0) - (BK - 1);
+      else
+        base = (long long)(r0 + k0) + (c0 + n0);
 #pragma unroll
-    for (int q = 0; q < HREG; ++q) {
-      int idx = tid + q * NT;
-      int col = idx % BN, row = idx / BN;
-      int gk = k0 + row, gn = n0 + col;
-      hreg[q] = (gk < K && gn < N) ? gen_entry<KIND>(g, r0 + gk, c0 + gn) : 0.0;
+      for (int q = 0; q < WREG; ++q) {
+        int t = tid + q * NT;
+        long long gi = base + t;
+        wreg[q] = (t < WLEN && gi >= glo && gi <= ghi) ? g.p0[gi] : 0.0;
+      }
+    } else {
+#pragma unroll
+      for (int q = 0; q < HREG; ++q) {
+        int row = tid / BN + q * (NT / BN);
+        int gk = k0 + row;
+        if (gk < K) {
+          if constexpr (KIND == KIND_CAUCHY) {
+            rowa[q] = g.p0[r0 + gk];  // D_r
+            rowb[q] = g.p2[r0 + gk];  // U_r
+          } else {
+            rowa[q] = g.p0[r0 + gk];  // F_r
+            rowb[q] = g.p1[r0 + gk];  // U_r
+          }
+        } else {
+          rowa[q] = 0.0;
+          rowb[q] = 0.0;
+        }
+      }
+    }
+  };
+  // Evaluate the prefetched slab (the FP64 divisions of the Cauchy kinds).
+  auto finish_h = [&](int k0) {
+    if constexpr (!WIN) {
+#pragma unroll
+      for (int q = 0; q < HREG; ++q) {
+        int row = tid / BN + q * (NT / BN);
+        int gk = k0 + row;
+        double v = 0.0;
+        if (gk < K && colok) {
+          if constexpr (KIND == KIND_CAUCHY) {
+            v = (rowb[q] * cv1) / (rowa[q] - cv0);
+          } else {
+            // SRC/CauchyHssEig.f90:1764-1768: d_r - lambda_c from stored differences
+            int gr = r0 + gk;
+            double den = (gr <= gcol) ? ((rowa[q] - cv0) - cv3) : ((rowa[q] - cv1) - cv4);
+            v = (rowb[q] * cv2) / den;
+          }
+        }
+        hreg[q] = v;
+      }
     }
   };
   auto store = [&](int buf) {
@@ -89,17 +179,26 @@ __global__ __launch_bounds__(WGM* WGN * 64, 2) void sgemm_f64_right_kernel(
       int idx = tid + q * NT;
       As[buf][(idx / BM) * ALD + idx % BM] = areg[q];
     }
+    if constexpr (WIN) {
 #pragma unroll
-    for (int q = 0; q < HREG; ++q) {
-      int idx = tid + q * NT;
-      Hs[buf][(idx / BN) * HLD + idx % BN] = hreg[q];
+      for (int q = 0; q < WREG; ++q) {
+        int t = tid + q * NT;
+        if (t < WLEN) Hs[buf][t] = wreg[q];
+      }
+    } else {
+#pragma unroll
+      for (int q = 0; q < HREG; ++q) {
+        int row = tid / BN + q * (NT / BN);
+        Hs[buf][row * HLD + hcol] = hreg[q];
+      }
     }
   };
 
   const int nkt = (K + BK - 1) / BK;
   if (nkt > 0) {
     load_a(0);
-    gen_h(0);
+    fetch_h(0);
+    finish_h(0);
     store(0);
   }
   __syncthreads();
@@ -107,7 +206,10 @@ __global__ __launch_bounds__(WGM* WGN * 64, 2) void sgemm_f64_right_kernel(
   for (int kt = 0; kt < nkt; ++kt) {
     const int cur = kt & 1;
     const bool more = (kt + 1) < nkt;
-    if (more) load_a((kt + 1) * BK);
+    if (more) {
+      load_a((kt + 1) * BK);
+      fetch_h((kt + 1) * BK);
+    }
     const double* as = As[cur];
     const double* hs = Hs[cur];
 #pragma unroll
@@ -116,7 +218,15 @@ __global__ __launch_bounds__(WGM* WGN * 64, 2) void sgemm_f64_right_kernel(
 #pragma unroll
       for (int i = 0; i < TM; ++i) af[i] = as[(kk + lk) * ALD + wm0 + i * 16 + lr];
 #pragma unroll
-      for (int j = 0; j < TN; ++j) bf[j] = hs[(kk + lk) * HLD + wn0 + j * 16 + lr];
+      for (int j = 0; j < TN; ++j) {
+        int jj = wn0 + j * 16 + lr, kr = kk + lk;
+        if constexpr (KIND == KIND_TOEPLITZ)
+          bf[j] = hs[(BK - 1) - kr + jj];
+        else if constexpr (KIND == KIND_HANKEL)
+          bf[j] = hs[kr + jj];
+        else
+          bf[j] = hs[kr * HLD + jj];
+      }
 #pragma unroll
       for (int i = 0; i < TM; ++i)
 #pragma unroll
@@ -124,7 +234,7 @@ __global__ __launch_bounds__(WGM* WGN * 64, 2) void sgemm_f64_right_kernel(
           acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[i], bf[j], acc[i][j], 0, 0, 0);
     }
     if (more) {
-      gen_h((kt + 1) * BK);
+      finish_h((kt + 1) * BK);
       store(cur ^ 1);
     }
     __syncthreads();
@@ -345,10 +455,13 @@ static hipError_t launch_right_kind(int cfg, int M, int N, int K, double alpha, 
                                     int lda, const GenPtrs& g, int r0, int c0, double beta,
                                     double* C, int ldc, hipStream_t s) {
   switch (cfg) {
-    case 0:  // large problems: 128x128 workgroup tile, 4 waves of 64x64
+    case 0:  // large problems: 128x128 workgroup tile, 4 waves of 64x64, BK = 16
       return launch_right_t<KIND, 128, 128, 16, 2, 2>(M, N, K, alpha, A, lda, g, r0, c0, beta, C,
                                                       ldc, s);
-    case 1:  // 64x256: fewer dense-operand re-reads per flop
+    case 1:  // as 0 with BK = 32: half the barriers per flop
+      return launch_right_t<KIND, 128, 128, 32, 2, 2>(M, N, K, alpha, A, lda, g, r0, c0, beta, C,
+                                                      ldc, s);
+    case 3:  // 64x256: fewer dense-operand re-reads per flop
       return launch_right_t<KIND, 64, 256, 16, 1, 4>(M, N, K, alpha, A, lda, g, r0, c0, beta, C,
                                                      ldc, s);
     default:  // small problems: 64x64 workgroup tile, 4 waves of 32x32
@@ -358,16 +471,22 @@ static hipError_t launch_right_kind(int cfg, int M, int N, int K, double alpha, 
 }
 
 // Picks a tile shape: the big tile when it still yields >= 2 workgroups per CU.
-int choose_right_cfg(int M, int N) {
+// PSMMA_F64_CFG (0..3) forces one, for tuning sweeps.
+int choose_right_cfg(int kind, int M, int N) {
+  static int forced = [] {
+    const char* e = getenv("PSMMA_F64_CFG");
+    return e ? atoi(e) : -1;
+  }();
+  if (forced >= 0) return forced;
   long long big = (long long)((M + 127) / 128) * ((N + 127) / 128);
-  if (big >= 512) return 0;
+  if (big >= 512) return (kind == KIND_TOEPLITZ || kind == KIND_HANKEL) ? 3 : 0;
   return 2;
 }
 
 hipError_t launch_right_f64(int kind, int cfg, int M, int N, int K, double alpha, const double* A,
                             int lda, const GenPtrs& g, int r0, int c0, double beta, double* C,
                             int ldc, hipStream_t s) {
-  if (cfg < 0) cfg = choose_right_cfg(M, N);
+  if (cfg < 0) cfg = choose_right_cfg(kind, M, N);
   switch (kind) {
     case KIND_CAUCHY:
       return launch_right_kind<KIND_CAUCHY>(cfg, M, N, K, alpha, A, lda, g, r0, c0, beta, C, ldc, s);
