@@ -154,7 +154,7 @@ def main():
     if not args.no_split:
         try:
             best = None
-            for slices in (6, 7):
+            for slices in (12, 13, 14):
                 def sstep(sl=slices):
                     api.toeplitz_dev(1.0, a, g, n, n, c, prec=api.PREC_SPLIT_I8, slices=sl)
                 sw, sper = time_steps(sstep, max(3, args.steps // 2), 1, torch)

@@ -34,10 +34,13 @@ extern "C" {
 /* ---------------------------------------------------------------------------
  * Numeric forms.
  *   PSMMA_PREC_FP64     plain FP64 on FP64 MFMA (v_mfma_f64_16x16x4f64).
- *   PSMMA_PREC_SPLIT_I8 error-free split: both operands cut into `slices`
- *                       int8 slices with power-of-two row/column scales, slice
- *                       products on int8 MFMA with exact int32 accumulation,
- *                       recombined in FP64.
+ *   PSMMA_PREC_SPLIT_I8 error-free modular split: rows of A and columns of H are
+ *                       scaled by powers of two and rounded to integers, the
+ *                       integer product is computed exactly modulo `slices`
+ *                       (2..16) pairwise-coprime moduli <= 256 on int8 MFMA
+ *                       (exact int32 accumulation) and rebuilt by the Chinese
+ *                       remainder theorem in double-double FP64.  13 moduli keep
+ *                       43 bits per operand at K = 8192, 14 keep 47.
  *   PSMMA_PREC_SPLIT_BF16 the same with bf16 slices on bf16 MFMA, FP32
  *                       accumulation over exactly representable K-chunks and
  *                       FP64 recombination.

@@ -156,7 +156,7 @@ static int check_common(int m, int n, int k, const double* a, int lda, const dou
   if (ldc < (m > 1 ? m : 1)) return -14;
   if (prec != PSMMA_PREC_FP64 && prec != PSMMA_PREC_SPLIT_I8 && prec != PSMMA_PREC_SPLIT_BF16)
     return -15;
-  if (prec != PSMMA_PREC_FP64 && (slices < 1 || slices > 8)) return -16;
+  if (prec != PSMMA_PREC_FP64 && (slices < 1 || slices > 16)) return -16;
   if (m > 0 && n > 0 && (c == nullptr || (k > 0 && a == nullptr))) return -5;
   return 0;
 }
@@ -271,7 +271,7 @@ extern "C" int psmma_last_status(void) { return g_status; }
 extern "C" int psmma_set_precision(int prec, int slices) {
   if (prec != PSMMA_PREC_FP64 && prec != PSMMA_PREC_SPLIT_I8 && prec != PSMMA_PREC_SPLIT_BF16)
     return -1;
-  if (prec != PSMMA_PREC_FP64 && (slices < 1 || slices > 8)) return -2;
+  if (prec != PSMMA_PREC_FP64 && (slices < 1 || slices > 16)) return -2;
   g_prec = prec;
   g_slices = slices;
   return 0;

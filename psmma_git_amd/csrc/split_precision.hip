@@ -1,0 +1,805 @@
+// Split-precision structured x dense products on gfx950: an int8 modular form
+// (the fast one) and a bf16-slice form.
+//
+// ---- int8 modular form (PSMMA_PREC_SPLIT_I8) ----
+//
+// C = alpha * A * H + beta * C with FP64 inputs and an FP64 result, computed
+// from exact integer products (an error-free modular decomposition, the
+// "Ozaki scheme II" family):
+//
+//  1. scale each row i of A by 2^sa_i and each column j of H by 2^sh_j so the
+//     largest entry is below 2^abits, and round to integers A', H'
+//     (|A'|, |H'| <= 2^abits; this rounding is the only approximation);
+//  2. for t pairwise-coprime moduli m_l <= 256 form the symmetric residues
+//     A'_l = A' mod m_l, H'_l = H' mod m_l as int8 matrices;
+//  3. compute C'_l = A'_l * H'_l exactly on v_mfma_i32_16x16x64_i8 (int32
+//     accumulation cannot overflow: |residue| <= 128, K <= 2^17) and keep
+//     C'_l mod m_l;
+//  4. rebuild X = A' * H' from its residues (Chinese remainder theorem, Garner's
+//     mixed-radix form with signed digits, evaluated in double-double FP64); X is
+//     determined exactly because M = prod m_l > 2 K 2^(2 abits) > 2 |X|; scale
+//     back: C = alpha * X * 2^-(sa_i + sh_j) + beta * C.
+//
+// The structured operand is evaluated from its generators (FP64, divisions of
+// the Cauchy kinds included) once per entry in the residue pass; only the
+// residues travel to the int8 GEMM.  Error: |C - AH| <= 2^-abits-ish relative
+// to row/column maxima; t = 13 gives abits = 43 at K = 8192, t = 14 gives 47.
+#include <hip/hip_runtime.h>
+
+#include <climits>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <mutex>
+#include <unordered_map>
+
+#include "psmma_common.h"
+
+typedef long v2l __attribute__((ext_vector_type(2)));
+
+namespace psmma {
+
+constexpr int kMaxModuli = 16;
+// Exponent of an all-zero row/column (below any FP64 exponent; sums stay in int).
+constexpr int kNoExp = -10000;
+// Pairwise coprime (distinct prime factors), all <= 256 so a symmetric
+// residue fits int8.
+constexpr int kModuli[kMaxModuli] = {256, 255, 253, 251, 247, 241, 239, 233,
+                                     229, 227, 223, 217, 211, 199, 197, 193};
+
+struct CrtParams {
+  int t;
+  int m[kMaxModuli];
+  int y[kMaxModuli];         // (M/m_l)^-1 mod m_l
+  double inv_hi[kMaxModuli];  // 1/m_l as a double-double
+  double inv_lo[kMaxModuli];
+  double M_hi, M_lo;          // M = prod m_l as a double-double
+  double rm[kMaxModuli];      // 1/m_l rounded (for the residue quotient)
+  unsigned char minv[kMaxModuli][kMaxModuli];  // minv[j][l] = m_j^-1 mod m_l (j < l)
+};
+
+// ---------------------------------------------------------------------------
+// Host: CRT constants.
+
+static long long inv_mod(long long a, long long m) {
+  long long t = 0, nt = 1, r = m, nr = ((a % m) + m) % m;
+  while (nr) {
+    long long q = r / nr, tmp;
+    tmp = t - q * nt; t = nt; nt = tmp;
+    tmp = r - q * nr; r = nr; nr = tmp;
+  }
+  return (t % m + m) % m;
+}
+
+static CrtParams make_crt(int t) {
+  CrtParams p{};
+  p.t = t;
+  unsigned __int128 M = 1;
+  for (int l = 0; l < t; ++l) M *= (unsigned)kModuli[l];
+  for (int l = 0; l < t; ++l) {
+    int m = kModuli[l];
+    p.m[l] = m;
+    unsigned __int128 Ml = M / (unsigned)m;
+    long long r = (long long)(Ml % (unsigned)m);
+    p.y[l] = (int)inv_mod(r, m);
+    double hi = 1.0 / m;
+    double lo = std::fma(-hi, (double)m, 1.0) / m;  // 1 - hi*m is exact in the fma
+    p.inv_hi[l] = hi;
+    p.inv_lo[l] = lo;
+    p.rm[l] = 1.0 / m;
+    for (int j = 0; j < l; ++j) p.minv[j][l] = (unsigned char)inv_mod(kModuli[j] % m, m);
+  }
+  // M as hi + lo (M < 2^128): split the 128-bit integer exactly.
+  unsigned long long top = (unsigned long long)(M >> 64), bot = (unsigned long long)M;
+  long double Ml = (long double)top * 18446744073709551616.0L + (long double)bot;
+  double hi = (double)Ml;
+  // exact remainder M - hi in 128-bit integer arithmetic
+  __int128 hi_i = 0;
+  {
+    int e;
+    double f = frexp(hi, &e);  // hi = f * 2^e, f in [0.5,1)
+    long long mant = (long long)ldexp(f, 53);
+    hi_i = (__int128)mant << (e - 53);
+  }
+  __int128 rem = (__int128)M - hi_i;
+  double lo = (double)(long double)rem;
+  p.M_hi = hi;
+  p.M_lo = lo;
+  return p;
+}
+
+// Largest abits with 2 * K * 2^(2 abits) < M, capped so A' stays exact in FP64.
+static int choose_abits(int t, int K) {
+  double lg = 0.0;
+  for (int l = 0; l < t; ++l) lg += std::log2((double)kModuli[l]);
+  int lk = 0;
+  while ((1LL << lk) < (long long)(K > 1 ? K : 1)) ++lk;
+  int ab = ((int)std::floor(lg) - 2 - lk) / 2;
+  if (ab > 52) ab = 52;
+  return ab;
+}
+
+// ---------------------------------------------------------------------------
+// Device helpers.
+
+// Symmetric residue of an integer-valued double |v| < 2^53 modulo m:
+// q = rint(v/m) may be off by one (v/m is rounded); r = v - q*m is exact.
+__device__ __forceinline__ int sym_residue(double v, int m, double rm) {
+  double q = rint(v * rm);
+  double r = fma(-q, (double)m, v);
+  int ri = (int)r;
+  int half = m >> 1;
+  if (ri >= half) ri -= m;
+  if (ri < -half) ri += m;
+  return ri;  // in [-m/2, m/2) subset of [-128, 127]
+}
+
+__device__ __forceinline__ int exponent_of(double x) {
+  int e;
+  frexp(fabs(x), &e);
+  return x == 0.0 ? kNoExp : e;
+}
+
+__global__ void fill_int_kernel(int* p, int n, int v) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+// Max exponent per row of A (column-major): threads walk rows, blocks split K.
+__global__ void row_exp_kernel(int M, int K, const double* __restrict__ A, int lda, int kchunk,
+                               int* __restrict__ ea) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int k0 = blockIdx.y * kchunk;
+  if (i >= M) return;
+  int k1 = min(K, k0 + kchunk);
+  double mx = 0.0;
+  for (int k = k0; k < k1; ++k) mx = fmax(mx, fabs(A[(size_t)k * lda + i]));
+  if (mx > 0.0) atomicMax(&ea[i], exponent_of(mx));
+}
+
+// Max exponent per column of H: threads walk columns, blocks split K.
+template <int KIND>
+__global__ void col_exp_kernel(int K, int N, GenPtrs g, int r0, int c0, int kchunk,
+                               int* __restrict__ eh) {
+  int j = blockIdx.x * blockDim.x + threadIdx.x;
+  int k0 = blockIdx.y * kchunk;
+  if (j >= N) return;
+  int k1 = min(K, k0 + kchunk);
+  double mx = 0.0;
+  for (int k = k0; k < k1; ++k) mx = fmax(mx, fabs(gen_entry<KIND>(g, r0 + k, c0 + j)));
+  if (mx > 0.0) atomicMax(&eh[j], exponent_of(mx));
+}
+
+// Residues of A: reads a 64 x 64 tile of column-major A, writes t row-major
+// int8 planes Ares[l][i * kp + k] (zero for k >= K; kp multiple of 128).
+__global__ __launch_bounds__(256) void split_a_kernel(int M, int K, const double* __restrict__ A,
+                                                      int lda, const int* __restrict__ ea,
+                                                      int abits, CrtParams cp,
+                                                      signed char* __restrict__ res, int kp,
+                                                      size_t plane) {
+  __shared__ double tile[64][65];
+  const int i0 = blockIdx.x * 64, k0 = blockIdx.y * 64;
+  const int t = threadIdx.x;
+#pragma unroll
+  for (int q = 0; q < 16; ++q) {
+    int ii = t % 64, kk = t / 64 + 4 * q;
+    int gi = i0 + ii, gk = k0 + kk;
+    tile[kk][ii] = (gi < M && gk < K) ? A[(size_t)gk * lda + gi] : 0.0;
+  }
+  __syncthreads();
+  // thread -> row i0 + t/4, 16 consecutive k
+  const int ii = t / 4, kb = (t % 4) * 16;
+  const int gi = i0 + ii;
+  if (gi >= M) return;
+  const int sh = abits - ea[gi];
+  double v[16];
+#pragma unroll
+  for (int e = 0; e < 16; ++e) v[e] = rint(ldexp(tile[kb + e][ii], sh));
+  for (int l = 0; l < cp.t; ++l) {
+    int m = cp.m[l];
+    double rm = cp.rm[l];
+    unsigned w[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      unsigned word = 0;
+#pragma unroll
+      for (int b = 0; b < 4; ++b) {
+        int r = sym_residue(v[c * 4 + b], m, rm);
+        word |= ((unsigned)(r & 0xff)) << (8 * b);
+      }
+      w[c] = word;
+    }
+    uint4* dst = (uint4*)(res + l * plane + (size_t)gi * kp + k0 + kb);
+    *dst = make_uint4(w[0], w[1], w[2], w[3]);
+  }
+}
+
+// Residues of H: column-major int8 planes Hres[l][j * kp + k] (k contiguous),
+// entries generated from the generators.  Thread -> (column, 16 consecutive k).
+template <int KIND>
+__global__ __launch_bounds__(256) void split_h_kernel(int K, int N, GenPtrs g, int r0, int c0,
+                                                      const int* __restrict__ eh, int hbits,
+                                                      CrtParams cp, signed char* __restrict__ res,
+                                                      int kp, size_t plane) {
+  const int t = threadIdx.x;
+  const int j = blockIdx.x * 16 + t / 16;
+  const int kb = blockIdx.y * 256 + (t % 16) * 16;
+  if (j >= N || kb >= kp) return;
+  const int sh = hbits - eh[j];
+  double v[16];
+#pragma unroll
+  for (int e = 0; e < 16; ++e) {
+    int k = kb + e;
+    v[e] = (k < K) ? rint(ldexp(gen_entry<KIND>(g, r0 + k, c0 + j), sh)) : 0.0;
+  }
+  for (int l = 0; l < cp.t; ++l) {
+    int m = cp.m[l];
+    double rm = cp.rm[l];
+    unsigned w[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      unsigned word = 0;
+#pragma unroll
+      for (int b = 0; b < 4; ++b) {
+        int r = sym_residue(v[c * 4 + b], m, rm);
+        word |= ((unsigned)(r & 0xff)) << (8 * b);
+      }
+      w[c] = word;
+    }
+    uint4* dst = (uint4*)(res + l * plane + (size_t)j * kp + kb);
+    *dst = make_uint4(w[0], w[1], w[2], w[3]);
+  }
+}
+
+// int8 GEMM per modulus (blockIdx.y = modulus index):
+//   R_l = (Ares_l * Hres_l) mod m_l, stored as uint8, column-major M x N.
+// Ares_l: M x kp row-major, Hres_l: kp x N column-major; kp % BK == 0.
+// v_mfma_i32_16x16x64_i8 lane layout (measured): A lane l supplies
+// A[l%16][16*(l/16) .. +15], B lane l supplies B[16*(l/16) .. +15][l%16],
+// D lane l element r holds D[4*(l/16)+r][l%16].
+template <int BM, int BN, int BK, int WGM, int WGN>
+__global__ __launch_bounds__(WGM* WGN * 64, 2) void gemm_i8_mod_kernel(
+    int M, int N, int kp, const signed char* __restrict__ Ares, const signed char* __restrict__ Hres,
+    size_t aplane, size_t hplane, CrtParams cp, unsigned char* __restrict__ out, size_t oplane) {
+  constexpr int NT = WGM * WGN * 64;
+  constexpr int WM = BM / WGM, WN = BN / WGN;
+  constexpr int TM = WM / 16, TN = WN / 16;
+  constexpr int LDB = BK + 16;  // bytes per staged row (conflict-free 16-byte reads)
+  constexpr int CH = BK / 16;   // 16-byte chunks per row
+  constexpr int AQ = BM * CH / NT, BQ = BN * CH / NT;
+  static_assert(BM * CH % NT == 0 && BN * CH % NT == 0, "tile/threads mismatch");
+  static_assert(BK % 64 == 0, "BK must be a multiple of the MFMA depth");
+
+  __shared__ __align__(16) signed char As[2][BM * LDB];
+  __shared__ __align__(16) signed char Bs[2][BN * LDB];
+
+  const int l = blockIdx.y;
+  const int m = cp.m[l];
+  const signed char* Al = Ares + l * aplane;
+  const signed char* Hl = Hres + l * hplane;
+  unsigned char* Ol = out + l * oplane;
+
+  const int nbn = (N + BN - 1) / BN, nbm = (M + BM - 1) / BM;
+  const int bid = xcd_remap(blockIdx.x, nbn * nbm);
+  const int m0 = (bid / nbn) * BM, n0 = (bid % nbn) * BN;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm0 = (wave / WGN) * WM, wn0 = (wave % WGN) * WN;
+  const int lr = lane & 15, lg = lane >> 4;
+
+  v4i acc[TM][TN];
+#pragma unroll
+  for (int i = 0; i < TM; ++i)
+#pragma unroll
+    for (int j = 0; j < TN; ++j) acc[i][j] = (v4i){0, 0, 0, 0};
+
+  uint4 ar[AQ], br[BQ];
+  auto load = [&](int k0) {
+#pragma unroll
+    for (int q = 0; q < AQ; ++q) {
+      int c = tid + q * NT, row = c / CH, part = c % CH;
+      int gr = m0 + row;
+      ar[q] = (gr < M) ? *(const uint4*)(Al + (size_t)gr * kp + k0 + part * 16)
+                       : make_uint4(0, 0, 0, 0);
+    }
+#pragma unroll
+    for (int q = 0; q < BQ; ++q) {
+      int c = tid + q * NT, col = c / CH, part = c % CH;
+      int gc = n0 + col;
+      br[q] = (gc < N) ? *(const uint4*)(Hl + (size_t)gc * kp + k0 + part * 16)
+                       : make_uint4(0, 0, 0, 0);
+    }
+  };
+  auto store = [&](int buf) {
+#pragma unroll
+    for (int q = 0; q < AQ; ++q) {
+      int c = tid + q * NT, row = c / CH, part = c % CH;
+      *(uint4*)(&As[buf][row * LDB + part * 16]) = ar[q];
+    }
+#pragma unroll
+    for (int q = 0; q < BQ; ++q) {
+      int c = tid + q * NT, col = c / CH, part = c % CH;
+      *(uint4*)(&Bs[buf][col * LDB + part * 16]) = br[q];
+    }
+  };
+
+  const int nkt = kp / BK;
+  load(0);
+  store(0);
+  __syncthreads();
+  for (int kt = 0; kt < nkt; ++kt) {
+    const int cur = kt & 1;
+    const bool more = kt + 1 < nkt;
+    if (more) load((kt + 1) * BK);
+#pragma unroll
+    for (int ks = 0; ks < BK; ks += 64) {
+      v2l af[TM], bf[TN];
+#pragma unroll
+      for (int i = 0; i < TM; ++i)
+        af[i] = *(const v2l*)(&As[cur][(wm0 + i * 16 + lr) * LDB + ks + lg * 16]);
+#pragma unroll
+      for (int j = 0; j < TN; ++j)
+        bf[j] = *(const v2l*)(&Bs[cur][(wn0 + j * 16 + lr) * LDB + ks + lg * 16]);
+#pragma unroll
+      for (int i = 0; i < TM; ++i)
+#pragma unroll
+        for (int j = 0; j < TN; ++j)
+          acc[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(af[i], bf[j], acc[i][j], 0, 0, 0);
+    }
+    if (more) store(cur ^ 1);
+    __syncthreads();
+  }
+
+#pragma unroll
+  for (int i = 0; i < TM; ++i)
+#pragma unroll
+    for (int j = 0; j < TN; ++j) {
+      int gc = n0 + wn0 + j * 16 + lr;
+      int gr = m0 + wm0 + i * 16 + lg * 4;
+      if (gc >= N || gr >= M) continue;
+      unsigned word = 0;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        int x = acc[i][j][r] % m;
+        if (x < 0) x += m;
+        word |= (unsigned)x << (8 * r);
+      }
+      unsigned char* dst = Ol + (size_t)gc * M + gr;
+      if (gr + 3 < M && ((M & 3) == 0)) {
+        *(unsigned*)dst = word;
+      } else {
+        for (int r = 0; r < 4 && gr + r < M; ++r) dst[r] = (unsigned char)(word >> (8 * r));
+      }
+    }
+}
+
+__device__ __forceinline__ void two_sum(double a, double b, double& s, double& e) {
+  s = a + b;
+  double bb = s - a;
+  e = (a - (s - bb)) + (b - bb);
+}
+
+// CRT recombination + scaling: C(i,j) = alpha * X * 2^-(sa_i + sh_j) + beta * C(i,j).
+// X is rebuilt from its residues by Garner's mixed-radix conversion with signed
+// digits d_l in [-m_l/2, m_l/2) (exact small-integer arithmetic, moduli known at
+// compile time), X = d_0 + m_0 (d_1 + m_1 (d_2 + ...)), evaluated by Horner in
+// double-double from the most significant digit: a small |X| has leading zero
+// digits, so its relative precision does not depend on the size of M.
+template <int T>
+__global__ __launch_bounds__(256) void crt_kernel(int M, int N, const unsigned char* __restrict__ res,
+                                                  size_t oplane, CrtParams cp,
+                                                  const int* __restrict__ ea,
+                                                  const int* __restrict__ eh, int abits, int hbits,
+                                                  double alpha, double beta, double* __restrict__ C,
+                                                  int ldc) {
+  size_t off = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (off >= (size_t)M * N) return;
+  int i = (int)(off % M), j = (int)(off / M);
+  int d[T];
+#pragma unroll
+  for (int l = 0; l < T; ++l) {
+    const int m = kModuli[l];
+    int v = res[l * oplane + off];  // in [0, m)
+#pragma unroll
+    for (int q = 0; q < l; ++q) v = ((v - d[q] + 2 * m) * (int)cp.minv[q][l]) % m;
+    d[l] = (v >= (m + 1) / 2) ? v - m : v;
+  }
+  double yh = (double)d[T - 1], yl = 0.0;
+#pragma unroll
+  for (int l = T - 2; l >= 0; --l) {
+    const double m = (double)kModuli[l];
+    double ph = yh * m;
+    double pl = fma(yh, m, -ph) + yl * m;
+    double sh, se;
+    two_sum(ph, (double)d[l], sh, se);
+    double sl = pl + se;
+    yh = sh + sl;
+    yl = sl - (yh - sh);
+  }
+  double x = yh + yl;
+  int s_a = abits - ea[i];
+  int s_h = hbits - eh[j];
+  double v = alpha * ldexp(x, -(s_a + s_h));
+  double* p = C + (size_t)j * ldc + i;
+  if (beta != 0.0) v += beta * (*p);
+  *p = v;
+}
+
+// ---------------------------------------------------------------------------
+// Workspace (grow-only, per device).
+struct SplitWs {
+  void* p = nullptr;
+  size_t cap = 0;
+};
+static std::mutex g_ws_mu;
+static std::unordered_map<int, SplitWs> g_ws;
+
+static void* split_workspace(size_t bytes) {
+  int dev = 0;
+  hipGetDevice(&dev);
+  std::lock_guard<std::mutex> lk(g_ws_mu);
+  SplitWs& w = g_ws[dev];
+  if (w.cap < bytes) {
+    if (w.p) {
+      hipDeviceSynchronize();
+      hipFree(w.p);
+    }
+    w.p = nullptr;
+    w.cap = 0;
+    if (hipMalloc(&w.p, bytes) != hipSuccess) return nullptr;
+    w.cap = bytes;
+  }
+  return w.p;
+}
+
+template <int KIND>
+static int run_split_i8(int t, int M, int N, int K, double alpha, const double* A, int lda,
+                        const GenPtrs& g, int r0, int c0, double beta, double* C, int ldc,
+                        hipStream_t s) {
+  const CrtParams cp = make_crt(t);
+  const int abits = choose_abits(t, K);
+  if (abits < 8) return -16;
+  const int kp = ((K + 127) / 128) * 128;
+  const size_t aplane = (size_t)M * kp, hplane = (size_t)kp * N, oplane = (size_t)M * N;
+  auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
+  size_t bytes = al(sizeof(int) * M) + al(sizeof(int) * N) + al(t * aplane) + al(t * hplane) +
+                 al(t * oplane);
+  char* ws = (char*)split_workspace(bytes);
+  if (!ws) return 1002;
+  int* ea = (int*)ws;
+  int* eh = (int*)(ws + al(sizeof(int) * M));
+  signed char* ares = (signed char*)(ws + al(sizeof(int) * M) + al(sizeof(int) * N));
+  signed char* hres = ares + al(t * aplane);
+  unsigned char* ores = (unsigned char*)(hres + al(t * hplane));
+
+  hipLaunchKernelGGL(fill_int_kernel, dim3((M + 255) / 256), dim3(256), 0, s, ea, M, kNoExp);
+  hipLaunchKernelGGL(fill_int_kernel, dim3((N + 255) / 256), dim3(256), 0, s, eh, N, kNoExp);
+  const int kchunk = 256;
+  if (K > 0) {
+    hipLaunchKernelGGL(row_exp_kernel, dim3((M + 255) / 256, (K + kchunk - 1) / kchunk), dim3(256),
+                       0, s, M, K, A, lda, kchunk, ea);
+    hipLaunchKernelGGL(col_exp_kernel<KIND>, dim3((N + 255) / 256, (K + kchunk - 1) / kchunk),
+                       dim3(256), 0, s, K, N, g, r0, c0, kchunk, eh);
+  }
+  hipLaunchKernelGGL(split_a_kernel, dim3((M + 63) / 64, kp / 64), dim3(256), 0, s, M, K, A, lda,
+                     ea, abits, cp, ares, kp, aplane);
+  hipLaunchKernelGGL(split_h_kernel<KIND>, dim3((N + 15) / 16, (kp + 255) / 256), dim3(256), 0, s, K, N, g,
+                     r0, c0, eh, abits, cp, hres, kp, hplane);
+  constexpr int BM = 128, BN = 128, BK = 128;
+  int nb = ((M + BM - 1) / BM) * ((N + BN - 1) / BN);
+  hipLaunchKernelGGL((gemm_i8_mod_kernel<BM, BN, BK, 2, 2>), dim3(nb, t), dim3(256), 0, s, M, N, kp,
+                     ares, hres, aplane, hplane, cp, ores, oplane);
+  size_t nel = (size_t)M * N;
+  dim3 cg((unsigned)((nel + 255) / 256));
+#define PSMMA_CRT_CASE(TT)                                                                     \
+  case TT:                                                                                     \
+    hipLaunchKernelGGL(crt_kernel<TT>, cg, dim3(256), 0, s, M, N, ores, oplane, cp, ea, eh,     \
+                       abits, abits, alpha, beta, C, ldc);                                     \
+    break;
+  switch (t) {
+    PSMMA_CRT_CASE(2) PSMMA_CRT_CASE(3) PSMMA_CRT_CASE(4) PSMMA_CRT_CASE(5) PSMMA_CRT_CASE(6)
+    PSMMA_CRT_CASE(7) PSMMA_CRT_CASE(8) PSMMA_CRT_CASE(9) PSMMA_CRT_CASE(10) PSMMA_CRT_CASE(11)
+    PSMMA_CRT_CASE(12) PSMMA_CRT_CASE(13) PSMMA_CRT_CASE(14) PSMMA_CRT_CASE(15) PSMMA_CRT_CASE(16)
+    default: return -16;
+  }
+#undef PSMMA_CRT_CASE
+  hipError_t e = hipGetLastError();
+  return e == hipSuccess ? 0 : 1000 + (int)e;
+}
+
+hipError_t launch_right_f64(int kind, int cfg, int M, int N, int K, double alpha, const double* A,
+                            int lda, const GenPtrs& g, int r0, int c0, double beta, double* C,
+                            int ldc, hipStream_t s);
+
+static int launch_right_split_i8(int kind, int t, int M, int N, int K, double alpha,
+                                 const double* A, int lda, const GenPtrs& g, int r0, int c0,
+                                 double beta, double* C, int ldc, hipStream_t s) {
+  if (t < 2 || t > kMaxModuli) return -16;
+  switch (kind) {
+    case KIND_CAUCHY: return run_split_i8<KIND_CAUCHY>(t, M, N, K, alpha, A, lda, g, r0, c0, beta, C, ldc, s);
+    case KIND_TOEPLITZ: return run_split_i8<KIND_TOEPLITZ>(t, M, N, K, alpha, A, lda, g, r0, c0, beta, C, ldc, s);
+    case KIND_HANKEL: return run_split_i8<KIND_HANKEL>(t, M, N, K, alpha, A, lda, g, r0, c0, beta, C, ldc, s);
+    case KIND_CAUCHYEIG: return run_split_i8<KIND_CAUCHYEIG>(t, M, N, K, alpha, A, lda, g, r0, c0, beta, C, ldc, s);
+  }
+  return -1;
+}
+
+// ===========================================================================
+// bf16-slice form (PSMMA_PREC_SPLIT_BF16).
+//
+// With the same power-of-two row/column scales (|x| < 1 after scaling), each
+// value is cut into s slices on a fixed-point grid: x_1 = rint(x 2^8) 2^-8,
+// x_p = rint(r_{p-1} 2^(8p)) 2^(-8p) with r_p the remainder.  Every slice has
+// at most 8 significant bits, so it is exact in bf16, and |x - sum x_p| <= 2^(-8s-1).
+// Products x_p y_q are exact in FP32; all pairs with p + q = d sit on the grid
+// 2^(-8d) with |x_p y_q| <= 2^16 grid units, so their FP32 sum over a K-chunk of
+// 256 stays below 2^24 units and is exact.  Pairs with p + q <= s + 1 are kept,
+// one FP32 accumulator per class d, flushed into FP64 accumulators per chunk.
+
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
+
+__device__ __forceinline__ unsigned bf16_bits_exact(double v) {
+  // v has <= 8 significant bits: the float conversion and the truncation to
+  // the upper 16 bits of the float are both exact.
+  float f = (float)v;
+  return __float_as_uint(f) >> 16;
+}
+
+// A slices: s planes of bf16, row-major M x kp (k contiguous).
+__global__ __launch_bounds__(256) void split_a_bf16_kernel(int M, int K, const double* __restrict__ A,
+                                                           int lda, const int* __restrict__ ea,
+                                                           int s, unsigned short* __restrict__ out,
+                                                           int kp, size_t plane) {
+  __shared__ double tile[64][65];
+  const int i0 = blockIdx.x * 64, k0 = blockIdx.y * 64;
+  const int t = threadIdx.x;
+#pragma unroll
+  for (int q = 0; q < 16; ++q) {
+    int ii = t % 64, kk = t / 64 + 4 * q;
+    int gi = i0 + ii, gk = k0 + kk;
+    tile[kk][ii] = (gi < M && gk < K) ? A[(size_t)gk * lda + gi] : 0.0;
+  }
+  __syncthreads();
+  const int ii = t / 4, kb = (t % 4) * 16;
+  const int gi = i0 + ii;
+  if (gi >= M) return;
+  const int sh = -ea[gi];
+  double r[16];
+#pragma unroll
+  for (int e = 0; e < 16; ++e) r[e] = ldexp(tile[kb + e][ii], sh);
+  for (int p = 1; p <= s; ++p) {
+    unsigned w[8];
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+      unsigned word = 0;
+#pragma unroll
+      for (int b = 0; b < 2; ++b) {
+        int e = c * 2 + b;
+        double xp = ldexp(rint(ldexp(r[e], 8 * p)), -8 * p);
+        r[e] -= xp;
+        word |= bf16_bits_exact(xp) << (16 * b);
+      }
+      w[c] = word;
+    }
+    uint4* dst = (uint4*)(out + (size_t)(p - 1) * plane + (size_t)gi * kp + k0 + kb);
+    dst[0] = make_uint4(w[0], w[1], w[2], w[3]);
+    dst[1] = make_uint4(w[4], w[5], w[6], w[7]);
+  }
+}
+
+// H slices: s planes of bf16, column-major kp x N (k contiguous).
+template <int KIND>
+__global__ __launch_bounds__(256) void split_h_bf16_kernel(int K, int N, GenPtrs g, int r0, int c0,
+                                                           const int* __restrict__ eh, int s,
+                                                           unsigned short* __restrict__ out, int kp,
+                                                           size_t plane) {
+  const int t = threadIdx.x;
+  const int j = blockIdx.x * 16 + t / 16;
+  const int kb = blockIdx.y * 256 + (t % 16) * 16;
+  if (j >= N || kb >= kp) return;
+  const int sh = -eh[j];
+  double r[16];
+#pragma unroll
+  for (int e = 0; e < 16; ++e) {
+    int k = kb + e;
+    r[e] = (k < K) ? ldexp(gen_entry<KIND>(g, r0 + k, c0 + j), sh) : 0.0;
+  }
+  for (int p = 1; p <= s; ++p) {
+    unsigned w[8];
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+      unsigned word = 0;
+#pragma unroll
+      for (int b = 0; b < 2; ++b) {
+        int e = c * 2 + b;
+        double xp = ldexp(rint(ldexp(r[e], 8 * p)), -8 * p);
+        r[e] -= xp;
+        word |= bf16_bits_exact(xp) << (16 * b);
+      }
+      w[c] = word;
+    }
+    uint4* dst = (uint4*)(out + (size_t)(p - 1) * plane + (size_t)j * kp + kb);
+    dst[0] = make_uint4(w[0], w[1], w[2], w[3]);
+    dst[1] = make_uint4(w[4], w[5], w[6], w[7]);
+  }
+}
+
+// bf16 slice GEMM: workgroup 64 x 64 (4 waves of 32 x 32), fragments read from
+// global memory (L1/L2), v_mfma_f32_16x16x32_bf16.  Lane layout (measured):
+// A lane l supplies A[l%16][8*(l/16) .. +7], B lane l supplies B[8*(l/16) .. +7][l%16],
+// D lane l element r holds D[4*(l/16)+r][l%16].
+template <int S>
+__global__ __launch_bounds__(256) void gemm_bf16_slices_kernel(
+    int M, int N, int kp, const unsigned short* __restrict__ As,
+    const unsigned short* __restrict__ Hs, size_t aplane, size_t hplane,
+    const int* __restrict__ ea, const int* __restrict__ eh, double alpha, double beta,
+    double* __restrict__ C, int ldc) {
+  constexpr int TM = 2, TN = 2;
+  const int nbn = (N + 63) / 64, nbm = (M + 63) / 64;
+  const int bid = xcd_remap(blockIdx.x, nbn * nbm);
+  const int m0 = (bid / nbn) * 64, n0 = (bid % nbn) * 64;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm0 = (wave / 2) * 32, wn0 = (wave % 2) * 32;
+  const int lr = lane & 15, lg = lane >> 4;
+
+  double acc64[TM][TN][4];
+  v4f acc[S][TM][TN];
+#pragma unroll
+  for (int i = 0; i < TM; ++i)
+#pragma unroll
+    for (int j = 0; j < TN; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) acc64[i][j][r] = 0.0;
+
+  int arow[TM], hcol[TN];
+  bool aok[TM], hok[TN];
+#pragma unroll
+  for (int i = 0; i < TM; ++i) {
+    arow[i] = m0 + wm0 + i * 16 + lr;
+    aok[i] = arow[i] < M;
+  }
+#pragma unroll
+  for (int j = 0; j < TN; ++j) {
+    hcol[j] = n0 + wn0 + j * 16 + lr;
+    hok[j] = hcol[j] < N;
+  }
+
+  for (int kc = 0; kc < kp; kc += 256) {
+#pragma unroll
+    for (int d = 0; d < S; ++d)
+#pragma unroll
+      for (int i = 0; i < TM; ++i)
+#pragma unroll
+        for (int j = 0; j < TN; ++j) acc[d][i][j] = (v4f){0.f, 0.f, 0.f, 0.f};
+    for (int kk = kc; kk < kc + 256; kk += 32) {
+      v8bf af[S][TM], bf[S][TN];
+#pragma unroll
+      for (int p = 0; p < S; ++p) {
+#pragma unroll
+        for (int i = 0; i < TM; ++i) {
+          uint4 u = aok[i] ? *(const uint4*)(As + p * aplane + (size_t)arow[i] * kp + kk + lg * 8)
+                           : make_uint4(0, 0, 0, 0);
+          af[p][i] = __builtin_bit_cast(v8bf, u);
+        }
+#pragma unroll
+        for (int j = 0; j < TN; ++j) {
+          uint4 u = hok[j] ? *(const uint4*)(Hs + p * hplane + (size_t)hcol[j] * kp + kk + lg * 8)
+                           : make_uint4(0, 0, 0, 0);
+          bf[p][j] = __builtin_bit_cast(v8bf, u);
+        }
+      }
+#pragma unroll
+      for (int p = 0; p < S; ++p)
+#pragma unroll
+        for (int q = 0; q < S - p; ++q)  // 0-based p + q <= S - 1: class d = p + q
+#pragma unroll
+          for (int i = 0; i < TM; ++i)
+#pragma unroll
+            for (int j = 0; j < TN; ++j)
+              acc[p + q][i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[p][i], bf[q][j],
+                                                                         acc[p + q][i][j], 0, 0, 0);
+    }
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+      for (int j = 0; j < TN; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          double sum = 0.0;
+#pragma unroll
+          for (int d = S - 1; d >= 0; --d) sum += (double)acc[d][i][j][r];
+          acc64[i][j][r] += sum;
+        }
+  }
+
+#pragma unroll
+  for (int i = 0; i < TM; ++i)
+#pragma unroll
+    for (int j = 0; j < TN; ++j) {
+      int gc = n0 + wn0 + j * 16 + lr;
+      if (gc >= N) continue;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        int gr = m0 + wm0 + i * 16 + lg * 4 + r;
+        if (gr >= M) continue;
+        double v = alpha * ldexp(acc64[i][j][r], ea[gr] + eh[gc]);
+        double* p = C + (size_t)gc * ldc + gr;
+        if (beta != 0.0) v += beta * (*p);
+        *p = v;
+      }
+    }
+}
+
+template <int KIND>
+static int run_split_bf16(int s, int M, int N, int K, double alpha, const double* A, int lda,
+                          const GenPtrs& g, int r0, int c0, double beta, double* C, int ldc,
+                          hipStream_t st) {
+  const int kp = ((K + 255) / 256) * 256;
+  const size_t aplane = (size_t)M * kp, hplane = (size_t)kp * N;
+  auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
+  size_t bytes = al(sizeof(int) * M) + al(sizeof(int) * N) + al(2 * s * aplane) + al(2 * s * hplane);
+  char* ws = (char*)split_workspace(bytes);
+  if (!ws) return 1002;
+  int* ea = (int*)ws;
+  int* eh = (int*)(ws + al(sizeof(int) * M));
+  unsigned short* as = (unsigned short*)(ws + al(sizeof(int) * M) + al(sizeof(int) * N));
+  unsigned short* hs = (unsigned short*)((char*)as + al(2 * s * aplane));
+  hipLaunchKernelGGL(fill_int_kernel, dim3((M + 255) / 256), dim3(256), 0, st, ea, M, kNoExp);
+  hipLaunchKernelGGL(fill_int_kernel, dim3((N + 255) / 256), dim3(256), 0, st, eh, N, kNoExp);
+  const int kchunk = 256;
+  hipLaunchKernelGGL(row_exp_kernel, dim3((M + 255) / 256, (K + kchunk - 1) / kchunk), dim3(256),
+                     0, st, M, K, A, lda, kchunk, ea);
+  hipLaunchKernelGGL(col_exp_kernel<KIND>, dim3((N + 255) / 256, (K + kchunk - 1) / kchunk),
+                     dim3(256), 0, st, K, N, g, r0, c0, kchunk, eh);
+  hipLaunchKernelGGL(split_a_bf16_kernel, dim3((M + 63) / 64, kp / 64), dim3(256), 0, st, M, K, A,
+                     lda, ea, s, as, kp, aplane);
+  hipLaunchKernelGGL(split_h_bf16_kernel<KIND>, dim3((N + 15) / 16, kp / 256), dim3(256), 0, st, K,
+                     N, g, r0, c0, eh, s, hs, kp, hplane);
+  int nb = ((M + 63) / 64) * ((N + 63) / 64);
+#define PSMMA_BF16_CASE(SS)                                                                   \
+  case SS:                                                                                    \
+    hipLaunchKernelGGL(gemm_bf16_slices_kernel<SS>, dim3(nb), dim3(256), 0, st, M, N, kp, as, \
+                       hs, aplane, hplane, ea, eh, alpha, beta, C, ldc);                     \
+    break;
+  switch (s) {
+    PSMMA_BF16_CASE(1)
+    PSMMA_BF16_CASE(2)
+    PSMMA_BF16_CASE(3)
+    PSMMA_BF16_CASE(4)
+    PSMMA_BF16_CASE(5)
+    PSMMA_BF16_CASE(6)
+    PSMMA_BF16_CASE(7)
+    default: return -16;
+  }
+#undef PSMMA_BF16_CASE
+  hipError_t e = hipGetLastError();
+  return e == hipSuccess ? 0 : 1000 + (int)e;
+}
+
+static int launch_right_split_bf16(int kind, int s, int M, int N, int K, double alpha,
+                                   const double* A, int lda, const GenPtrs& g, int r0, int c0,
+                                   double beta, double* C, int ldc, hipStream_t st) {
+  if (s < 1 || s > 7) return -16;
+  switch (kind) {
+    case KIND_CAUCHY: return run_split_bf16<KIND_CAUCHY>(s, M, N, K, alpha, A, lda, g, r0, c0, beta, C, ldc, st);
+    case KIND_TOEPLITZ: return run_split_bf16<KIND_TOEPLITZ>(s, M, N, K, alpha, A, lda, g, r0, c0, beta, C, ldc, st);
+    case KIND_HANKEL: return run_split_bf16<KIND_HANKEL>(s, M, N, K, alpha, A, lda, g, r0, c0, beta, C, ldc, st);
+    case KIND_CAUCHYEIG: return run_split_bf16<KIND_CAUCHYEIG>(s, M, N, K, alpha, A, lda, g, r0, c0, beta, C, ldc, st);
+  }
+  return -1;
+}
+
+// Split-precision dispatcher.  slices = number of int8 moduli (PSMMA_PREC_SPLIT_I8)
+// or of bf16 slices per operand (PSMMA_PREC_SPLIT_BF16).  An empty contraction
+// (K == 0) has no products to split: it is the beta * C epilogue of the FP64 kernel.
+int launch_right_split(int kind, int prec, int slices, int M, int N, int K, double alpha,
+                       const double* A, int lda, const GenPtrs& g, int r0, int c0, double beta,
+                       double* C, int ldc, hipStream_t s) {
+  if (K == 0) {
+    hipError_t e = launch_right_f64(kind, -1, M, N, K, alpha, A, lda, g, r0, c0, beta, C, ldc, s);
+    return e == hipSuccess ? 0 : 1000 + (int)e;
+  }
+  if (prec == 1) return launch_right_split_i8(kind, slices, M, N, K, alpha, A, lda, g, r0, c0, beta, C, ldc, s);
+  return launch_right_split_bf16(kind, slices, M, N, K, alpha, A, lda, g, r0, c0, beta, C, ldc, s);
+}
+
+}  // namespace psmma

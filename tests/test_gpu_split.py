@@ -1,0 +1,111 @@
+"""Split-precision (int8 modular) products against the FP64 oracle (MI355X only).
+
+Bar: relative Frobenius error <= 1e-12 for t >= 13 moduli at the sizes below;
+the error shrinks as t grows (each modulus adds ~8 bits to the exact integer
+range, i.e. ~4 bits per operand).
+"""
+import numpy as np
+import pytest
+
+from psmma_git_amd import api
+from test_gpu_parity import ORA_KIND, dev_colmajor, host, make_gens, run_dev
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("kind", ["cauchy", "toeplitz", "hankel", "cauchyeig"])
+@pytest.mark.parametrize("m,n,k", [(1, 1, 1), (100, 130, 77), (513, 257, 1000), (1024, 1024, 1024)])
+def test_split_i8_parity(gpu, ora, kind, m, n, k):
+    import torch
+
+    print(f"case split kind={kind} m={m} n={n} k={k}", flush=True)
+    rng = np.random.default_rng(hash(("s", kind, m, n, k)) % (1 << 31))
+    nglob = max(k, n) + 5
+    gens, dev = make_gens(kind, nglob, rng)
+    a = rng.uniform(-1, 1, (m, k))
+    c = rng.standard_normal((m, n))
+    want = ora.structured_gemm_np(ORA_KIND[kind], 0.75, a, gens, 2, 1, n, beta=0.5, c=c)
+    errs = {}
+    for t in (10, 13, 14):
+        da = dev_colmajor(a, ld=m + 1)
+        dc = dev_colmajor(c, ld=m + 2)
+        run_dev(kind, 0.75, da, dev, gens, n, dc, 0.5, 2, 1, prec=api.PREC_SPLIT_I8, slices=t)
+        torch.cuda.synchronize()
+        errs[t] = ora.rel_fro(host(dc), want)
+    print(f"errs={errs}", flush=True)
+    assert errs[13] <= 1e-12 and errs[14] <= 1e-12
+    assert errs[14] <= errs[10]
+
+
+def test_split_i8_zero_rows_and_columns(gpu, ora):
+    import torch
+
+    rng = np.random.default_rng(3)
+    n = 200
+    g = rng.standard_normal(2 * n - 1)
+    g[: n // 2] = 0.0  # makes the last columns of H... partly zero
+    a = rng.uniform(-1, 1, (n, n))
+    a[5, :] = 0.0
+    a[17, :] = 0.0
+    da = dev_colmajor(a)
+    dc = dev_colmajor(np.zeros((n, n)))
+    api.toeplitz_dev(1.0, da, torch.from_numpy(g).cuda(), n, n, dc, prec=api.PREC_SPLIT_I8, slices=14)
+    torch.cuda.synchronize()
+    want = ora.structured_gemm_np(1, 1.0, a, dict(g=g, korder=n), 0, 0, n)
+    x = host(dc)
+    assert np.all(x[5] == 0.0) and np.all(x[17] == 0.0)
+    assert ora.rel_fro(x, want) <= 1e-12
+
+
+def test_split_i8_wide_dynamic_range_cauchy(gpu, ora):
+    # Cauchy entries near the poles reach ~1/(min gap): per-column scaling keeps accuracy
+    import torch
+
+    from oracle import inputs
+
+    n = 2048
+    b, a = inputs.config0(n, seed=1)
+    d, f, u, v = (torch.from_numpy(np.ascontiguousarray(b[:, i])).cuda() for i in range(4))
+    da = dev_colmajor(a)
+    dc = dev_colmajor(np.zeros((n, n)))
+    api.cauchy_dev(1.0, da, d, f, u, v, n, dc, prec=api.PREC_SPLIT_I8, slices=14)
+    torch.cuda.synchronize()
+    gens = dict(d=b[:, 0], f=b[:, 1], u=b[:, 2], v=b[:, 3])
+    err = ora.rel_fro(host(dc), ora.structured_gemm_np(0, 1.0, a, gens, 0, 0, n))
+    print(f"config0 split err {err:.3e}")
+    assert err <= 1e-12
+
+
+def test_split_rejects_bad_slices(gpu):
+    import torch
+
+    a = dev_colmajor(np.ones((8, 8)))
+    c = dev_colmajor(np.zeros((8, 8)))
+    g = torch.ones(15, dtype=torch.float64, device="cuda")
+    with pytest.raises(api.PsmmaError):
+        api.toeplitz_dev(1.0, a, g, 8, 8, c, prec=api.PREC_SPLIT_I8, slices=1)
+
+
+@pytest.mark.parametrize("kind", ["cauchy", "toeplitz", "hankel", "cauchyeig"])
+@pytest.mark.parametrize("m,n,k", [(1, 1, 1), (100, 130, 77), (600, 300, 700)])
+def test_split_bf16_parity(gpu, ora, kind, m, n, k):
+    # s bf16 slices per operand keep ~8s bits: 3 slices ~1e-7, 6 slices <= 1e-12
+    import torch
+
+    print(f"case bf16 kind={kind} m={m} n={n} k={k}", flush=True)
+    rng = np.random.default_rng(hash(("b", kind, m, n, k)) % (1 << 31))
+    nglob = max(k, n) + 5
+    gens, dev = make_gens(kind, nglob, rng)
+    a = rng.uniform(-1, 1, (m, k))
+    c = rng.standard_normal((m, n))
+    want = ora.structured_gemm_np(ORA_KIND[kind], -1.5, a, gens, 1, 3, n, beta=2.0, c=c)
+    errs = {}
+    for s in (3, 6):
+        da = dev_colmajor(a)
+        dc = dev_colmajor(c)
+        run_dev(kind, -1.5, da, dev, gens, n, dc, 2.0, 1, 3, prec=api.PREC_SPLIT_BF16, slices=s)
+        torch.cuda.synchronize()
+        errs[s] = ora.rel_fro(host(dc), want)
+    print(f"errs={errs}", flush=True)
+    assert errs[3] <= 1e-6
+    assert errs[6] <= 1e-12
