@@ -95,4 +95,27 @@ __device__ __forceinline__ int xcd_remap(int bid, int nblocks) {
   return (bid % NXCD) * per + bid / NXCD;
 }
 
+// Patch rasterisation: workgroup b runs on XCD b % 8 as that XCD's (b / 8)-th
+// workgroup.  Consecutive local indices of one XCD are laid out over a PR x PC
+// patch of output tiles (patches dealt round-robin to the XCDs), so the
+// workgroups an XCD runs at the same time share PR row blocks of the first
+// operand and PC column blocks of the second in its L2.  Returns (bm, bn).
+template <int PR, int PC>
+__device__ __forceinline__ void patch_remap(int bid, int nbm, int nbn, int& bm, int& bn) {
+  constexpr int NXCD = 8, P = PR * PC;
+  const int npr = nbm / PR, npc = nbn / PC;
+  const int npatch = npr * npc;
+  if (nbm % PR == 0 && nbn % PC == 0 && npatch % NXCD == 0) {
+    int x = bid % NXCD, li = bid / NXCD;
+    int q = (li / P) * NXCD + x;  // global patch id
+    int p = li % P;
+    bm = (q / npc) * PR + p / PC;
+    bn = (q % npc) * PC + p % PC;
+  } else {
+    int t = xcd_remap(bid, nbm * nbn);
+    bm = t / nbn;
+    bn = t % nbn;
+  }
+}
+
 }  // namespace psmma

@@ -251,40 +251,105 @@ __global__ __launch_bounds__(256) void split_h_kernel(int K, int N, GenPtrs g, i
   }
 }
 
-// int8 GEMM per modulus (blockIdx.y = modulus index):
-//   R_l = (Ares_l * Hres_l) mod m_l, stored as uint8, column-major M x N.
-// Ares_l: M x kp row-major, Hres_l: kp x N column-major; kp % BK == 0.
+// ---------------------------------------------------------------------------
+// Window form for Toeplitz/Hankel.  With one power-of-two scale for the whole
+// block of H, H' is built from an integer generator, so its residues are
+// residues of that generator: R_l[u], u = k + (N-1-j) (Toeplitz, generator
+// read backwards) or u = k + j (Hankel), u in [0, K+N-1).  The GEMM then only
+// streams the residues of A; each workgroup stages the K+BN-1 bytes of its
+// window in LDS once and assembles every B fragment from five dword reads and
+// four v_alignbyte_b32 (the 16 bytes a lane supplies are contiguous in u).
+
+// max exponent of |g| over [lo, hi] -> *e
+__global__ void gen_range_exp_kernel(const double* __restrict__ g, long long lo, long long hi,
+                                     int* __restrict__ e) {
+  long long i = lo + (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  double v = (i <= hi) ? fabs(g[i]) : 0.0;
+  // wave-level max, then one atomic per wave
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o));
+  if ((threadIdx.x & 63) == 0 && v > 0.0) atomicMax(e, exponent_of(v));
+}
+
+__global__ void fill_from_kernel(int* __restrict__ dst, int n, const int* __restrict__ src) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = *src;
+}
+
+// Residue planes of the scaled generator: R_l[u] = sym(rint(g[src(u)] 2^sh)) mod m_l,
+// src(u) = hi - u when reversed (Toeplitz) else lo + u; zero for u >= L.
+__global__ void gen_residue_kernel(const double* __restrict__ g, long long lo, long long hi,
+                                   int reversed, const int* __restrict__ e, int hbits, CrtParams cp,
+                                   signed char* __restrict__ res, size_t gplane, int lpad) {
+  int u = blockIdx.x * blockDim.x + threadIdx.x;
+  if (u >= lpad) return;
+  long long L = hi - lo + 1;
+  double v = 0.0;
+  if (u < L) v = rint(ldexp(g[reversed ? hi - u : lo + u], hbits - *e));
+  for (int l = 0; l < cp.t; ++l) res[l * gplane + u] = (signed char)sym_residue(v, cp.m[l], cp.rm[l]);
+}
+
+// ---------------------------------------------------------------------------
+// int8 modular GEMM, one modulus per blockIdx.y:
+//   R_l = (Ares_l * B_l) mod m_l, stored as uint8, column-major M x N.
+// Ares_l: M x kp row-major int8.  B_l is either streamed (BMODE_DENSE: kp x N
+// column-major residues) or assembled from a generator window (BMODE_TOE /
+// BMODE_HANK: residues R_l[u] of the scaled generator, u = k + (N-1-j) or
+// u = k + j, staged once per workgroup in LDS; every B fragment is five dword
+// reads and four v_alignbyte_b32).
+//
+// Pipeline: global loads run two stages ahead in registers (the loads for
+// stage s+2 are issued when stage s starts and reach LDS at the end of stage
+// s+1); LDS fragments run one MFMA k-step ahead; one barrier per stage.
 // v_mfma_i32_16x16x64_i8 lane layout (measured): A lane l supplies
 // A[l%16][16*(l/16) .. +15], B lane l supplies B[16*(l/16) .. +15][l%16],
 // D lane l element r holds D[4*(l/16)+r][l%16].
-template <int BM, int BN, int BK, int WGM, int WGN>
-__global__ __launch_bounds__(WGM* WGN * 64, 2) void gemm_i8_mod_kernel(
-    int M, int N, int kp, const signed char* __restrict__ Ares, const signed char* __restrict__ Hres,
-    size_t aplane, size_t hplane, CrtParams cp, unsigned char* __restrict__ out, size_t oplane) {
+enum { BMODE_DENSE = 0, BMODE_TOE = 1, BMODE_HANK = 2 };
+
+template <int BMODE, int BM, int BN, int BK, int WGM, int WGN, int MINB, int KMAX>
+__global__ __launch_bounds__(WGM* WGN * 64, MINB) void gemm_i8_kernel(
+    int M, int N, int kp, const signed char* __restrict__ Ares, const signed char* __restrict__ Bres,
+    size_t aplane, size_t bplane, int lpad, CrtParams cp, unsigned char* __restrict__ out,
+    size_t oplane) {
+  constexpr bool WIN = BMODE != BMODE_DENSE;
   constexpr int NT = WGM * WGN * 64;
   constexpr int WM = BM / WGM, WN = BN / WGN;
   constexpr int TM = WM / 16, TN = WN / 16;
-  constexpr int LDB = BK + 16;  // bytes per staged row (conflict-free 16-byte reads)
-  constexpr int CH = BK / 16;   // 16-byte chunks per row
-  constexpr int AQ = BM * CH / NT, BQ = BN * CH / NT;
-  static_assert(BM * CH % NT == 0 && BN * CH % NT == 0, "tile/threads mismatch");
+  constexpr int LDB = BK + 16;  // staged row pitch: conflict-free 16-byte fragment reads
+  constexpr int CH = BK / 16;
+  constexpr int AQ = BM * CH / NT;
+  constexpr int BQ = WIN ? 1 : BN * CH / NT;
+  constexpr int KS = BK / 64;
+  constexpr int WBYTES = WIN ? KMAX + BN + 32 : 16;
+  static_assert(BM * CH % NT == 0 && (WIN || BN * CH % NT == 0), "tile/threads mismatch");
   static_assert(BK % 64 == 0, "BK must be a multiple of the MFMA depth");
 
   __shared__ __align__(16) signed char As[2][BM * LDB];
-  __shared__ __align__(16) signed char Bs[2][BN * LDB];
+  __shared__ __align__(16) signed char Bs[2][WIN ? 16 : BN * LDB];
+  __shared__ __align__(16) unsigned int Win[WBYTES / 4];
 
   const int l = blockIdx.y;
   const int m = cp.m[l];
   const signed char* Al = Ares + l * aplane;
-  const signed char* Hl = Hres + l * hplane;
+  const signed char* Bl = Bres + l * bplane;
   unsigned char* Ol = out + l * oplane;
 
   const int nbn = (N + BN - 1) / BN, nbm = (M + BM - 1) / BM;
-  const int bid = xcd_remap(blockIdx.x, nbn * nbm);
-  const int m0 = (bid / nbn) * BM, n0 = (bid % nbn) * BN;
+  int bmi, bni;
+  patch_remap<4, 8>(blockIdx.x, nbm, nbn, bmi, bni);
+  const int m0 = bmi * BM, n0 = bni * BN;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm0 = (wave / WGN) * WM, wn0 = (wave % WGN) * WN;
   const int lr = lane & 15, lg = lane >> 4;
+
+  if constexpr (WIN) {
+    // window u in [ubase, ubase + kp + BN + 16), zero outside [0, lpad)
+    const int ubase = (BMODE == BMODE_TOE) ? (N - n0 - BN) : n0;
+    unsigned char* wb = (unsigned char*)Win;
+    for (int x = tid; x < kp + BN + 16; x += NT) {
+      int u = ubase + x;
+      wb[x] = (u >= 0 && u < lpad) ? (unsigned char)Bl[u] : 0;
+    }
+  }
 
   v4i acc[TM][TN];
 #pragma unroll
@@ -292,61 +357,135 @@ __global__ __launch_bounds__(WGM* WGN * 64, 2) void gemm_i8_mod_kernel(
 #pragma unroll
     for (int j = 0; j < TN; ++j) acc[i][j] = (v4i){0, 0, 0, 0};
 
-  uint4 ar[AQ], br[BQ];
-  auto load = [&](int k0) {
+  int cj[TN];
+#pragma unroll
+  for (int j = 0; j < TN; ++j)
+    cj[j] = (BMODE == BMODE_TOE) ? (BN - 1 - wn0 - 16 * j - lr) : (wn0 + 16 * j + lr);
+
+  // Branch-free staging loads: out-of-range rows/columns read a clamped valid
+  // address and are replaced by zero with a select.
+  int arow[AQ], apart[AQ], brow[BQ], bpart[BQ];
+  bool aok[AQ], bok[BQ];
+#pragma unroll
+  for (int q = 0; q < AQ; ++q) {
+    int c = tid + q * NT;
+    int row = c / CH;
+    apart[q] = c % CH;
+    aok[q] = (m0 + row) < M;
+    arow[q] = min(m0 + row, M - 1);
+  }
+#pragma unroll
+  for (int q = 0; q < BQ; ++q) {
+    int c = tid + q * NT;
+    int col = c / CH;
+    bpart[q] = c % CH;
+    bok[q] = (n0 + col) < N;
+    brow[q] = min(n0 + col, N - 1);
+  }
+  auto gload = [&](int k0, uint4* ra, uint4* rb) {
 #pragma unroll
     for (int q = 0; q < AQ; ++q) {
-      int c = tid + q * NT, row = c / CH, part = c % CH;
-      int gr = m0 + row;
-      ar[q] = (gr < M) ? *(const uint4*)(Al + (size_t)gr * kp + k0 + part * 16)
-                       : make_uint4(0, 0, 0, 0);
+      uint4 v = *(const uint4*)(Al + (size_t)arow[q] * kp + k0 + apart[q] * 16);
+      ra[q] = aok[q] ? v : make_uint4(0, 0, 0, 0);
     }
+    if constexpr (!WIN) {
 #pragma unroll
-    for (int q = 0; q < BQ; ++q) {
-      int c = tid + q * NT, col = c / CH, part = c % CH;
-      int gc = n0 + col;
-      br[q] = (gc < N) ? *(const uint4*)(Hl + (size_t)gc * kp + k0 + part * 16)
-                       : make_uint4(0, 0, 0, 0);
+      for (int q = 0; q < BQ; ++q) {
+        uint4 v = *(const uint4*)(Bl + (size_t)brow[q] * kp + k0 + bpart[q] * 16);
+        rb[q] = bok[q] ? v : make_uint4(0, 0, 0, 0);
+      }
     }
   };
-  auto store = [&](int buf) {
+  auto lstore = [&](int buf, const uint4* ra, const uint4* rb) {
 #pragma unroll
     for (int q = 0; q < AQ; ++q) {
-      int c = tid + q * NT, row = c / CH, part = c % CH;
-      *(uint4*)(&As[buf][row * LDB + part * 16]) = ar[q];
+      int row = (tid + q * NT) / CH;
+      *(uint4*)(&As[buf][row * LDB + apart[q] * 16]) = ra[q];
     }
+    if constexpr (!WIN) {
 #pragma unroll
-    for (int q = 0; q < BQ; ++q) {
-      int c = tid + q * NT, col = c / CH, part = c % CH;
-      *(uint4*)(&Bs[buf][col * LDB + part * 16]) = br[q];
+      for (int q = 0; q < BQ; ++q) {
+        int col = (tid + q * NT) / CH;
+        *(uint4*)(&Bs[buf][col * LDB + bpart[q] * 16]) = rb[q];
+      }
+    }
+  };
+
+  v2l af[2][TM];
+  v2l bfd[2][WIN ? 1 : TN];           // streamed B fragments
+  unsigned bw[2][WIN ? TN : 1][5];    // window dwords
+  auto frags = [&](int buf, int kglob, int ks, int slot) {
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+      af[slot][i] = *(const v2l*)(&As[buf][(wm0 + i * 16 + lr) * LDB + ks * 64 + lg * 16]);
+    if constexpr (WIN) {
+#pragma unroll
+      for (int j = 0; j < TN; ++j) {
+        int w0 = (kglob + ks * 64 + lg * 16 + cj[j]) >> 2;
+#pragma unroll
+        for (int d = 0; d < 5; ++d) bw[slot][j][d] = Win[w0 + d];
+      }
+    } else {
+#pragma unroll
+      for (int j = 0; j < TN; ++j)
+        bfd[slot][j] = *(const v2l*)(&Bs[buf][(wn0 + j * 16 + lr) * LDB + ks * 64 + lg * 16]);
+    }
+  };
+  auto bfrag = [&](int slot, int j, int kglob, int ks) -> v2l {
+    if constexpr (WIN) {
+      int sh = (kglob + ks * 64 + lg * 16 + cj[j]) & 3;
+      unsigned o0 = __builtin_amdgcn_alignbyte(bw[slot][j][1], bw[slot][j][0], sh);
+      unsigned o1 = __builtin_amdgcn_alignbyte(bw[slot][j][2], bw[slot][j][1], sh);
+      unsigned o2 = __builtin_amdgcn_alignbyte(bw[slot][j][3], bw[slot][j][2], sh);
+      unsigned o3 = __builtin_amdgcn_alignbyte(bw[slot][j][4], bw[slot][j][3], sh);
+      v2l r;
+      r[0] = (long)(((unsigned long)o1 << 32) | o0);
+      r[1] = (long)(((unsigned long)o3 << 32) | o2);
+      return r;
+    } else {
+      return bfd[slot][j];
     }
   };
 
   const int nkt = kp / BK;
-  load(0);
-  store(0);
-  __syncthreads();
-  for (int kt = 0; kt < nkt; ++kt) {
+  uint4 ra0[AQ], rb0[BQ], ra1[AQ], rb1[BQ];
+
+  // One stage: hold = registers with stage kt+1, into = registers for stage kt+2.
+  auto stage = [&](int kt, int slot0, uint4* hold_a, uint4* hold_b, uint4* into_a,
+                   uint4* into_b) {
     const int cur = kt & 1;
-    const bool more = kt + 1 < nkt;
-    if (more) load((kt + 1) * BK);
+    const int kglob = kt * BK;
+    if (kt + 2 < nkt) gload((kt + 2) * BK, into_a, into_b);
 #pragma unroll
-    for (int ks = 0; ks < BK; ks += 64) {
-      v2l af[TM], bf[TN];
+    for (int ks = 0; ks < KS; ++ks) {
+      const int slot = (slot0 + ks) & 1;
+      if (ks + 1 < KS) {
+        frags(cur, kglob, ks + 1, slot ^ 1);
+      } else if (kt + 1 < nkt) {
+        lstore(cur ^ 1, hold_a, hold_b);
+        __syncthreads();
+        frags(cur ^ 1, kglob + BK, 0, slot ^ 1);
+      }
+      v2l bf[TN];
 #pragma unroll
-      for (int i = 0; i < TM; ++i)
-        af[i] = *(const v2l*)(&As[cur][(wm0 + i * 16 + lr) * LDB + ks + lg * 16]);
-#pragma unroll
-      for (int j = 0; j < TN; ++j)
-        bf[j] = *(const v2l*)(&Bs[cur][(wn0 + j * 16 + lr) * LDB + ks + lg * 16]);
+      for (int j = 0; j < TN; ++j) bf[j] = bfrag(slot, j, kglob, ks);
 #pragma unroll
       for (int i = 0; i < TM; ++i)
 #pragma unroll
         for (int j = 0; j < TN; ++j)
-          acc[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(af[i], bf[j], acc[i][j], 0, 0, 0);
+          acc[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(af[slot][i], bf[j], acc[i][j], 0, 0, 0);
     }
-    if (more) store(cur ^ 1);
-    __syncthreads();
+  };
+
+  gload(0, ra0, rb0);
+  lstore(0, ra0, rb0);
+  if (nkt > 1) gload(BK, ra1, rb1);
+  __syncthreads();
+  frags(0, 0, 0, 0);
+  constexpr int SLOT1 = KS & 1;  // slot in which an odd stage starts
+  for (int kt = 0; kt < nkt; kt += 2) {
+    stage(kt, 0, ra1, rb1, ra0, rb0);
+    if (kt + 1 < nkt) stage(kt + 1, SLOT1, ra0, rb0, ra1, rb1);
   }
 
 #pragma unroll
@@ -372,28 +511,31 @@ __global__ __launch_bounds__(WGM* WGN * 64, 2) void gemm_i8_mod_kernel(
     }
 }
 
+
+// The error-free transformations below rely on every rounding being the one
+// written: FMA contraction (hipcc's default) would fuse a product into an
+// addition and break them, so it is off in these functions.
 __device__ __forceinline__ void two_sum(double a, double b, double& s, double& e) {
+#pragma clang fp contract(off)
   s = a + b;
   double bb = s - a;
   e = (a - (s - bb)) + (b - bb);
 }
 
 // CRT recombination + scaling: C(i,j) = alpha * X * 2^-(sa_i + sh_j) + beta * C(i,j).
-// X is rebuilt from its residues by Garner's mixed-radix conversion with signed
-// digits d_l in [-m_l/2, m_l/2) (exact small-integer arithmetic, moduli known at
-// compile time), X = d_0 + m_0 (d_1 + m_1 (d_2 + ...)), evaluated by Horner in
-// double-double from the most significant digit: a small |X| has leading zero
-// digits, so its relative precision does not depend on the size of M.
+//
+// Fast form: X/M = frac(sum_l n_l / m_l), n_l = c_l y_l mod m_l, y_l = (M/m_l)^-1
+// mod m_l, summed in double-double.  The fraction is known to ~2^-100 absolute,
+// so it gives X to full FP64 precision whenever |X/M| >= 2^-40.
+// Exact form (taken only for the rare |X/M| < 2^-40): Garner's mixed-radix
+// conversion with signed digits d_l in [-m_l/2, m_l/2) (exact small-integer
+// arithmetic, moduli known at compile time), X = d_0 + m_0 (d_1 + m_1 (...)),
+// evaluated by Horner in double-double from the most significant digit, so a
+// small |X| loses nothing to the size of M.
 template <int T>
-__global__ __launch_bounds__(256) void crt_kernel(int M, int N, const unsigned char* __restrict__ res,
-                                                  size_t oplane, CrtParams cp,
-                                                  const int* __restrict__ ea,
-                                                  const int* __restrict__ eh, int abits, int hbits,
-                                                  double alpha, double beta, double* __restrict__ C,
-                                                  int ldc) {
-  size_t off = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (off >= (size_t)M * N) return;
-  int i = (int)(off % M), j = (int)(off / M);
+__device__ __forceinline__ double crt_garner(const unsigned char* __restrict__ res, size_t oplane,
+                                          size_t off, const CrtParams& cp) {
+#pragma clang fp contract(off)
   int d[T];
 #pragma unroll
   for (int l = 0; l < T; ++l) {
@@ -415,7 +557,44 @@ __global__ __launch_bounds__(256) void crt_kernel(int M, int N, const unsigned c
     yh = sh + sl;
     yl = sl - (yh - sh);
   }
-  double x = yh + yl;
+  return yh + yl;
+}
+
+template <int T>
+__global__ __launch_bounds__(256) void crt_kernel(int M, int N, const unsigned char* __restrict__ res,
+                                                  size_t oplane, CrtParams cp,
+                                                  const int* __restrict__ ea,
+                                                  const int* __restrict__ eh, int abits, int hbits,
+                                                  double alpha, double beta, double* __restrict__ C,
+                                                  int ldc) {
+#pragma clang fp contract(off)
+  size_t off = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (off >= (size_t)M * N) return;
+  int i = (int)(off % M), j = (int)(off / M);
+  double shi = 0.0, slo = 0.0;
+#pragma unroll
+  for (int l = 0; l < T; ++l) {
+    const int m = kModuli[l];
+    int nl = ((int)res[l * oplane + off] * cp.y[l]) % m;
+    double p = (double)nl * cp.inv_hi[l];
+    double pe = fma((double)nl, cp.inv_hi[l], -p) + (double)nl * cp.inv_lo[l];
+    double sh, se;
+    two_sum(shi, p, sh, se);
+    shi = sh;
+    slo += se + pe;
+  }
+  double fh = shi - rint(shi);
+  double f, fl;
+  two_sum(fh, slo, f, fl);
+  if (f > 0.5) f -= 1.0;
+  else if (f < -0.5) f += 1.0;
+  double x;
+  if (fabs(f) >= 0x1p-40) {
+    double xh = f * cp.M_hi;
+    x = xh + (fma(f, cp.M_hi, -xh) + f * cp.M_lo + fl * cp.M_hi);
+  } else {
+    x = crt_garner<T>(res, oplane, off, cp);
+  }
   int s_a = abits - ea[i];
   int s_h = hbits - eh[j];
   double v = alpha * ldexp(x, -(s_a + s_h));
@@ -451,43 +630,40 @@ static void* split_workspace(size_t bytes) {
   return w.p;
 }
 
-template <int KIND>
-static int run_split_i8(int t, int M, int N, int K, double alpha, const double* A, int lda,
-                        const GenPtrs& g, int r0, int c0, double beta, double* C, int ldc,
-                        hipStream_t s) {
-  const CrtParams cp = make_crt(t);
-  const int abits = choose_abits(t, K);
-  if (abits < 8) return -16;
-  const int kp = ((K + 127) / 128) * 128;
-  const size_t aplane = (size_t)M * kp, hplane = (size_t)kp * N, oplane = (size_t)M * N;
-  auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
-  size_t bytes = al(sizeof(int) * M) + al(sizeof(int) * N) + al(t * aplane) + al(t * hplane) +
-                 al(t * oplane);
-  char* ws = (char*)split_workspace(bytes);
-  if (!ws) return 1002;
-  int* ea = (int*)ws;
-  int* eh = (int*)(ws + al(sizeof(int) * M));
-  signed char* ares = (signed char*)(ws + al(sizeof(int) * M) + al(sizeof(int) * N));
-  signed char* hres = ares + al(t * aplane);
-  unsigned char* ores = (unsigned char*)(hres + al(t * hplane));
-
-  hipLaunchKernelGGL(fill_int_kernel, dim3((M + 255) / 256), dim3(256), 0, s, ea, M, kNoExp);
-  hipLaunchKernelGGL(fill_int_kernel, dim3((N + 255) / 256), dim3(256), 0, s, eh, N, kNoExp);
-  const int kchunk = 256;
-  if (K > 0) {
-    hipLaunchKernelGGL(row_exp_kernel, dim3((M + 255) / 256, (K + kchunk - 1) / kchunk), dim3(256),
-                       0, s, M, K, A, lda, kchunk, ea);
-    hipLaunchKernelGGL(col_exp_kernel<KIND>, dim3((N + 255) / 256, (K + kchunk - 1) / kchunk),
-                       dim3(256), 0, s, K, N, g, r0, c0, kchunk, eh);
+// Launches the int8 modular GEMM for all t moduli.  cfg: PSMMA_I8_CFG or by size.
+template <int BMODE>
+static void launch_gemm_i8(int t, int M, int N, int kp, const signed char* ares,
+                           const signed char* bres, size_t aplane, size_t bplane, int lpad,
+                           const CrtParams& cp, unsigned char* ores, size_t oplane, hipStream_t s) {
+  constexpr int KMAX = 16384;
+  static int forced = [] {
+    const char* e = getenv("PSMMA_I8_CFG");
+    return e ? atoi(e) : -1;
+  }();
+  int cfg = forced;
+  if (cfg < 0) cfg = 0;
+  auto nblk = [&](int bm, int bn) { return ((M + bm - 1) / bm) * ((N + bn - 1) / bn); };
+  switch (cfg) {
+    case 1:
+      hipLaunchKernelGGL((gemm_i8_kernel<BMODE, 128, 256, 128, 2, 2, 1, KMAX>),
+                         dim3(nblk(128, 256), t), dim3(256), 0, s, M, N, kp, ares, bres, aplane,
+                         bplane, lpad, cp, ores, oplane);
+      break;
+    case 2:
+      hipLaunchKernelGGL((gemm_i8_kernel<BMODE, 256, 128, 128, 2, 2, 1, KMAX>),
+                         dim3(nblk(256, 128), t), dim3(256), 0, s, M, N, kp, ares, bres, aplane,
+                         bplane, lpad, cp, ores, oplane);
+      break;
+    default:
+      hipLaunchKernelGGL((gemm_i8_kernel<BMODE, 128, 128, 128, 2, 2, 2, KMAX>),
+                         dim3(nblk(128, 128), t), dim3(256), 0, s, M, N, kp, ares, bres, aplane,
+                         bplane, lpad, cp, ores, oplane);
   }
-  hipLaunchKernelGGL(split_a_kernel, dim3((M + 63) / 64, kp / 64), dim3(256), 0, s, M, K, A, lda,
-                     ea, abits, cp, ares, kp, aplane);
-  hipLaunchKernelGGL(split_h_kernel<KIND>, dim3((N + 15) / 16, (kp + 255) / 256), dim3(256), 0, s, K, N, g,
-                     r0, c0, eh, abits, cp, hres, kp, hplane);
-  constexpr int BM = 128, BN = 128, BK = 128;
-  int nb = ((M + BM - 1) / BM) * ((N + BN - 1) / BN);
-  hipLaunchKernelGGL((gemm_i8_mod_kernel<BM, BN, BK, 2, 2>), dim3(nb, t), dim3(256), 0, s, M, N, kp,
-                     ares, hres, aplane, hplane, cp, ores, oplane);
+}
+
+static int finish_crt(int t, int M, int N, const unsigned char* ores, size_t oplane,
+                      const CrtParams& cp, const int* ea, const int* eh, int abits, double alpha,
+                      double beta, double* C, int ldc, hipStream_t s) {
   size_t nel = (size_t)M * N;
   dim3 cg((unsigned)((nel + 255) / 256));
 #define PSMMA_CRT_CASE(TT)                                                                     \
@@ -504,6 +680,64 @@ static int run_split_i8(int t, int M, int N, int K, double alpha, const double* 
 #undef PSMMA_CRT_CASE
   hipError_t e = hipGetLastError();
   return e == hipSuccess ? 0 : 1000 + (int)e;
+}
+
+template <int KIND>
+static int run_split_i8(int t, int M, int N, int K, double alpha, const double* A, int lda,
+                        const GenPtrs& g, int r0, int c0, double beta, double* C, int ldc,
+                        hipStream_t s) {
+  const CrtParams cp = make_crt(t);
+  const int abits = choose_abits(t, K);
+  if (abits < 8) return -16;
+  const int kp = ((K + 255) / 256) * 256;
+  const size_t aplane = (size_t)M * kp, hplane = (size_t)kp * N, oplane = (size_t)M * N;
+  auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
+  size_t bytes = al(sizeof(int) * M) + al(sizeof(int) * N) + al(t * aplane) + al(t * hplane) +
+                 al(t * oplane);
+  char* ws = (char*)split_workspace(bytes);
+  if (!ws) return 1002;
+  int* ea = (int*)ws;
+  int* eh = (int*)(ws + al(sizeof(int) * M));
+  signed char* ares = (signed char*)(ws + al(sizeof(int) * M) + al(sizeof(int) * N));
+  signed char* hres = ares + al(t * aplane);
+  unsigned char* ores = (unsigned char*)(hres + al(t * hplane));
+
+  hipLaunchKernelGGL(fill_int_kernel, dim3((M + 255) / 256), dim3(256), 0, s, ea, M, kNoExp);
+  hipLaunchKernelGGL(fill_int_kernel, dim3((N + 255) / 256), dim3(256), 0, s, eh, N, kNoExp);
+  const int kchunk = 256;
+  hipLaunchKernelGGL(row_exp_kernel, dim3((M + 255) / 256, (K + kchunk - 1) / kchunk), dim3(256), 0,
+                     s, M, K, A, lda, kchunk, ea);
+  hipLaunchKernelGGL(split_a_kernel, dim3((M + 63) / 64, kp / 64), dim3(256), 0, s, M, K, A, lda,
+                     ea, abits, cp, ares, kp, aplane);
+  constexpr int KMAX = 16384;  // longest contraction the generator window form stages in LDS
+  if constexpr (KIND == KIND_TOEPLITZ || KIND == KIND_HANKEL) {
+    if (kp <= KMAX) {
+      // one scale for the block of H; residues of the generator window
+      constexpr bool TOE = (KIND == KIND_TOEPLITZ);
+      long long glo = TOE ? (long long)g.korder - 1 + c0 - (r0 + K - 1) : (long long)r0 + c0;
+      long long ghi = TOE ? (long long)g.korder - 1 + (c0 + N - 1) - r0
+                          : (long long)r0 + K - 1 + c0 + N - 1;
+      const int lpad = (int)(((ghi - glo + 1) + 15) / 16 * 16);
+      int* e1 = (int*)ores;  // scratch int, consumed before the GEMM writes ores
+      signed char* gres = hres;  // t planes of lpad bytes (fit in the H residue area)
+      hipLaunchKernelGGL(fill_int_kernel, dim3(1), dim3(64), 0, s, e1, 1, kNoExp);
+      long long L = ghi - glo + 1;
+      hipLaunchKernelGGL(gen_range_exp_kernel, dim3((unsigned)((L + 255) / 256)), dim3(256), 0, s,
+                         g.p0, glo, ghi, e1);
+      hipLaunchKernelGGL(fill_from_kernel, dim3((N + 255) / 256), dim3(256), 0, s, eh, N, e1);
+      hipLaunchKernelGGL(gen_residue_kernel, dim3((lpad + 255) / 256), dim3(256), 0, s, g.p0, glo,
+                         ghi, TOE ? 1 : 0, e1, abits, cp, gres, (size_t)lpad, lpad);
+      launch_gemm_i8<TOE ? BMODE_TOE : BMODE_HANK>(t, M, N, kp, ares, gres, aplane, (size_t)lpad,
+                                                     lpad, cp, ores, oplane, s);
+      return finish_crt(t, M, N, ores, oplane, cp, ea, eh, abits, alpha, beta, C, ldc, s);
+    }
+  }
+  hipLaunchKernelGGL(col_exp_kernel<KIND>, dim3((N + 255) / 256, (K + kchunk - 1) / kchunk),
+                     dim3(256), 0, s, K, N, g, r0, c0, kchunk, eh);
+  hipLaunchKernelGGL(split_h_kernel<KIND>, dim3((N + 15) / 16, (kp + 255) / 256), dim3(256), 0, s, K, N, g,
+                     r0, c0, eh, abits, cp, hres, kp, hplane);
+  launch_gemm_i8<BMODE_DENSE>(t, M, N, kp, ares, hres, aplane, hplane, 0, cp, ores, oplane, s);
+  return finish_crt(t, M, N, ores, oplane, cp, ea, eh, abits, alpha, beta, C, ldc, s);
 }
 
 hipError_t launch_right_f64(int kind, int cfg, int M, int N, int K, double alpha, const double* A,
