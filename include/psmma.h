@@ -171,6 +171,48 @@ int psmma_cauchyeig_dev(int m, int n, int k, double alpha, const double* a, int 
 int pzmdft_dev(int m, int n, int k, int nd, double alpha, const double* b, int ldb, int r0, int c0,
                double beta, double* c, int ldc, int prec, void* stream);
 
+/* ===========================================================================
+ * 3. Batched uneven products (device pointers).
+ *
+ * One entry per problem: C(m x n) = alpha * A(m x k) * H(r0:r0+k, c0:c0+n) + beta * C.
+ * Generators per kind, as in the single-problem calls above:
+ *   CAUCHY:    g0 = d, g1 = f, g2 = u, g3 = v
+ *   TOEPLITZ:  g0 = g, lg = its length, korder = K
+ *   HANKEL:    g0 = g, lg = its length
+ *   CAUCHYEIG: g0 = fpole, g1 = u, g2 = v, g3 = difl, g4 = difr, lg = global order
+ * ======================================================================== */
+typedef struct psmma_problem {
+  int kind;
+  int m, n, k;
+  double alpha, beta;
+  const double* a;
+  int lda;
+  const double *g0, *g1, *g2, *g3, *g4;
+  int lg;
+  int korder;
+  int r0, c0;
+  double* c;
+  int ldc;
+} psmma_problem;
+
+/* All problems in one go on `stream`.  PSMMA_PREC_FP64: one persistent grouped
+ * launch over the tiles of every problem.  Split forms: the per-problem
+ * pipelines in sequence; slices = 0 picks the number of moduli per problem for
+ * a relative error <= 1e-12 (PSMMA_PREC_SPLIT_I8).  The problem table is copied
+ * to the device before the call returns. */
+int psmma_batched_dev(int nprob, const psmma_problem* probs, int prec, int slices, void* stream);
+
+/* A reusable plan: validates the problems, uploads the table, picks the tile
+ * shape (FP64: both grouped shapes are timed on the batch when every beta is 0),
+ * and captures the whole batch in a hipGraph.  Launching the plan replays the
+ * graph on `stream`.  No allocation happens at launch time. */
+int psmma_batched_plan_create(int nprob, const psmma_problem* probs, int prec, int slices,
+                              void** plan);
+int psmma_batched_plan_launch(void* plan, void* stream);
+/* Tile configuration chosen (0: 64x64, 1: 128x128) and number of graph kernel nodes. */
+int psmma_batched_plan_info(void* plan, int* tile_cfg, int* nodes);
+int psmma_batched_plan_destroy(void* plan);
+
 /* Library version string. */
 const char* psmma_version(void);
 

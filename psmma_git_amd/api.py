@@ -65,6 +65,11 @@ _SIGS = {
     "psmma_cauchyeig_dev": ([_I, _I, _I, _D, _P, _I, _P, _P, _P, _P, _P, _I, _I, _I, _D, _P, _I,
                              _I, _I, _P], _I),
     "pzmdft_dev": ([_I, _I, _I, _I, _D, _P, _I, _I, _I, _D, _P, _I, _I, _P], _I),
+    "psmma_batched_dev": ([_I, _P, _I, _I, _P], _I),
+    "psmma_batched_plan_create": ([_I, _P, _I, _I, ctypes.POINTER(ctypes.c_void_p)], _I),
+    "psmma_batched_plan_launch": ([_P, _P], _I),
+    "psmma_batched_plan_info": ([_P, _PI, _PI], _I),
+    "psmma_batched_plan_destroy": ([_P], _I),
 }
 
 
@@ -288,3 +293,66 @@ def dft_dev(alpha, b, nd, m, c, beta=0.0, r0=0, c0=0, prec=PREC_FP64, stream=Non
     pc, mc, nc, ldc = _colmajor(c)
     _check(lib().pzmdft_dev(m, n, k, nd, alpha, pb, ldb, r0, c0, beta, pc, ldc, prec,
                             _stream_ptr(stream)), "pzmdft_dev")
+
+
+# ---------------------------------------------------------------------------
+# Batched uneven products (include/psmma.h section 3).
+
+class Problem(ctypes.Structure):
+    """psmma_problem: C = alpha * A * H(r0:r0+k, c0:c0+n) + beta * C on device pointers."""
+    _fields_ = [
+        ("kind", ctypes.c_int), ("m", ctypes.c_int), ("n", ctypes.c_int), ("k", ctypes.c_int),
+        ("alpha", ctypes.c_double), ("beta", ctypes.c_double),
+        ("a", ctypes.c_void_p), ("lda", ctypes.c_int),
+        ("g0", ctypes.c_void_p), ("g1", ctypes.c_void_p), ("g2", ctypes.c_void_p),
+        ("g3", ctypes.c_void_p), ("g4", ctypes.c_void_p),
+        ("lg", ctypes.c_int), ("korder", ctypes.c_int), ("r0", ctypes.c_int), ("c0", ctypes.c_int),
+        ("c", ctypes.c_void_p), ("ldc", ctypes.c_int),
+    ]
+
+
+def make_problem(kind, alpha, a, gens, n, c, beta=0.0, r0=0, c0=0, lg=0, korder=0):
+    """Problem from torch tensors: a, c column-major; gens: list of up to five 1-D tensors."""
+    pa, m, k, lda = _colmajor(a)
+    pc, _, _, ldc = _colmajor(c)
+    g = [t.data_ptr() for t in gens] + [0] * (5 - len(gens))
+    return Problem(kind, m, n, k, float(alpha), float(beta), pa, lda, *g, int(lg), int(korder),
+                   int(r0), int(c0), pc, ldc)
+
+
+def batched_dev(problems, prec=PREC_FP64, slices=0, stream=None):
+    arr = (Problem * len(problems))(*problems)
+    _check(lib().psmma_batched_dev(len(problems), ctypes.cast(arr, ctypes.c_void_p), prec, slices,
+                                   _stream_ptr(stream)), "psmma_batched_dev")
+
+
+class BatchedPlan:
+    """hipGraph-captured batch; launch() replays it on a stream."""
+
+    def __init__(self, problems, prec=PREC_FP64, slices=0):
+        arr = (Problem * len(problems))(*problems)
+        h = ctypes.c_void_p()
+        _check(lib().psmma_batched_plan_create(len(problems), ctypes.cast(arr, ctypes.c_void_p),
+                                               prec, slices, ctypes.byref(h)),
+               "psmma_batched_plan_create")
+        self._h = h
+
+    def launch(self, stream=None):
+        _check(lib().psmma_batched_plan_launch(self._h, _stream_ptr(stream)),
+               "psmma_batched_plan_launch")
+
+    def info(self):
+        cfg, nodes = ctypes.c_int(), ctypes.c_int()
+        lib().psmma_batched_plan_info(self._h, ctypes.byref(cfg), ctypes.byref(nodes))
+        return cfg.value, nodes.value
+
+    def close(self):
+        if self._h:
+            lib().psmma_batched_plan_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

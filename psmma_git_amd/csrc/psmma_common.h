@@ -119,3 +119,21 @@ __device__ __forceinline__ void patch_remap(int bid, int nbm, int nbn, int& bm, 
 }
 
 }  // namespace psmma
+
+namespace psmma {
+
+// One problem of a batched launch (device-side table entry).
+struct BatchProblem {
+  int kind, m, n, k;
+  double alpha, beta;
+  const double* A;
+  int lda;
+  GenPtrs g;
+  int r0, c0;
+  double* C;
+  int ldc;
+  int tile_begin;  // first index of this problem's tiles in the batch tile list
+  int ntm, ntn;    // tiles along m and n
+};
+
+}  // namespace psmma

@@ -30,15 +30,28 @@ namespace psmma {
 // (measured on gfx950; the bf16/int8 16x16 forms instead hold D[4*(l/16)+r][l%16]).
 // (The k-slot assignment only has to agree between A and B.)
 
+// LDS footprint (doubles) of one right-structured tile configuration.
+template <int KIND, int BM, int BN, int BK>
+struct RightLds {
+  static constexpr bool WIN = (KIND == KIND_TOEPLITZ || KIND == KIND_HANKEL);
+  static constexpr int ALD = BM + 16, HLD = BN + 16;
+  static constexpr int ASZ = BK * ALD;
+  static constexpr int HSZ = WIN ? BK + BN : BK * HLD;
+};
+
+// One BM x BN output tile at (m0, n0) of C = alpha * A * H(r0:, c0:) + beta * C.
+// As/Hs: LDS areas of 2 * RightLds::ASZ and 2 * RightLds::HSZ doubles.  Ends
+// with all waves past a barrier, so the caller may reuse the LDS at once.
 template <int KIND, int BM, int BN, int BK, int WGM, int WGN>
-__global__ __launch_bounds__(WGM* WGN * 64, 2) void sgemm_f64_right_kernel(
-    int M, int N, int K, double alpha, const double* __restrict__ A, int lda, GenPtrs g, int r0,
-    int c0, double beta, double* __restrict__ C, int ldc) {
+__device__ __forceinline__ void right_tile(int M, int N, int K, double alpha,
+                                           const double* __restrict__ A, int lda, const GenPtrs& g,
+                                           int r0, int c0, double beta, double* __restrict__ C,
+                                           int ldc, int m0, int n0, double* __restrict__ As_,
+                                           double* __restrict__ Hs_) {
   constexpr int NT = WGM * WGN * 64;
   constexpr int WM = BM / WGM, WN = BN / WGN;
   constexpr int TM = WM / 16, TN = WN / 16;
-  constexpr int APAD = 16, HPAD = 16;
-  constexpr int ALD = BM + APAD, HLD = BN + HPAD;
+  constexpr int ALD = BM + 16, HLD = BN + 16;
   constexpr int AREG = BM * BK / NT;  // dense elements staged per thread per slab
   // Toeplitz and Hankel tiles are functions of (col - row) / (row + col): one
   // slab needs only a window of BK+BN-1 generator values.
@@ -46,19 +59,11 @@ __global__ __launch_bounds__(WGM* WGN * 64, 2) void sgemm_f64_right_kernel(
   constexpr int WLEN = BK + BN - 1;
   constexpr int WREG = (WLEN + NT - 1) / NT;
   constexpr int HREG = WIN ? 1 : BK * BN / NT;  // generated entries per thread per slab
-  constexpr int HSZ = WIN ? WLEN + 1 : BK * HLD;
+  constexpr int ASZ = RightLds<KIND, BM, BN, BK>::ASZ;
+  constexpr int HSZ = RightLds<KIND, BM, BN, BK>::HSZ;
   static_assert(BM * BK % NT == 0 && BK * BN % NT == 0, "tile/threads mismatch");
   static_assert(WIN || NT % BN == 0, "generated tiles assume a fixed column per thread");
   static_assert(WM % 16 == 0 && WN % 16 == 0 && BK % 4 == 0, "bad wave tile");
-
-  __shared__ double As[2][BK * ALD];
-  __shared__ double Hs[2][HSZ];
-
-  const int nbn = (N + BN - 1) / BN;
-  const int nbm = (M + BM - 1) / BM;
-  const int bid = xcd_remap(blockIdx.x, nbn * nbm);
-  const int bm = bid / nbn, bn = bid % nbn;
-  const int m0 = bm * BM, n0 = bn * BN;
 
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
@@ -177,19 +182,19 @@ __global__ __launch_bounds__(WGM* WGN * 64, 2) void sgemm_f64_right_kernel(
 #pragma unroll
     for (int q = 0; q < AREG; ++q) {
       int idx = tid + q * NT;
-      As[buf][(idx / BM) * ALD + idx % BM] = areg[q];
+      As_[buf * ASZ + (idx / BM) * ALD + idx % BM] = areg[q];
     }
     if constexpr (WIN) {
 #pragma unroll
       for (int q = 0; q < WREG; ++q) {
         int t = tid + q * NT;
-        if (t < WLEN) Hs[buf][t] = wreg[q];
+        if (t < WLEN) Hs_[buf * HSZ + t] = wreg[q];
       }
     } else {
 #pragma unroll
       for (int q = 0; q < HREG; ++q) {
         int row = tid / BN + q * (NT / BN);
-        Hs[buf][row * HLD + hcol] = hreg[q];
+        Hs_[buf * HSZ + row * HLD + hcol] = hreg[q];
       }
     }
   };
@@ -210,8 +215,8 @@ __global__ __launch_bounds__(WGM* WGN * 64, 2) void sgemm_f64_right_kernel(
       load_a((kt + 1) * BK);
       fetch_h((kt + 1) * BK);
     }
-    const double* as = As[cur];
-    const double* hs = Hs[cur];
+    const double* as = As_ + cur * ASZ;
+    const double* hs = Hs_ + cur * HSZ;
 #pragma unroll
     for (int kk = 0; kk < BK; kk += 4) {
       double af[TM], bf[TN];
@@ -259,6 +264,20 @@ __global__ __launch_bounds__(WGM* WGN * 64, 2) void sgemm_f64_right_kernel(
       }
     }
   }
+}
+
+template <int KIND, int BM, int BN, int BK, int WGM, int WGN>
+__global__ __launch_bounds__(WGM* WGN * 64, 2) void sgemm_f64_right_kernel(
+    int M, int N, int K, double alpha, const double* __restrict__ A, int lda, GenPtrs g, int r0,
+    int c0, double beta, double* __restrict__ C, int ldc) {
+  using L = RightLds<KIND, BM, BN, BK>;
+  __shared__ double As[2 * L::ASZ];
+  __shared__ double Hs[2 * L::HSZ];
+  const int nbn = (N + BN - 1) / BN;
+  const int nbm = (M + BM - 1) / BM;
+  const int bid = xcd_remap(blockIdx.x, nbn * nbm);
+  right_tile<KIND, BM, BN, BK, WGM, WGN>(M, N, K, alpha, A, lda, g, r0, c0, beta, C, ldc,
+                                         (bid / nbn) * BM, (bid % nbn) * BN, As, Hs);
 }
 
 // ---------------------------------------------------------------------------
@@ -431,6 +450,74 @@ __global__ __launch_bounds__(WGM* WGN * 64, 2) void dft_f64_left_kernel(
       }
     }
   }
+}
+
+// ---------------------------------------------------------------------------
+// Persistent grouped launch for a batch of uneven structured products of any
+// kinds.  All problems' output tiles form one list (problems sorted by
+// decreasing K, so the longest tiles are dealt first); workgroup w takes tiles
+// w, w + G, w + 2G, ... of that list, finds its problem by binary search over
+// the tile prefix sums and runs the same tile body as the single-problem
+// kernel.  The LDS areas are sized for the largest kind.
+template <int BM, int BN, int BK, int WGM, int WGN, int MINB>
+__global__ __launch_bounds__(WGM* WGN * 64, MINB) void sgemm_f64_grouped_kernel(
+    const BatchProblem* __restrict__ probs, int nprob, int ntiles) {
+  using LC = RightLds<KIND_CAUCHY, BM, BN, BK>;  // the largest footprint
+  __shared__ double As[2 * LC::ASZ];
+  __shared__ double Hs[2 * LC::HSZ];
+  for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    int lo = 0, hi = nprob - 1;
+    while (lo < hi) {  // last problem with tile_begin <= t
+      int mid = (lo + hi + 1) >> 1;
+      if (probs[mid].tile_begin <= t) lo = mid; else hi = mid - 1;
+    }
+    const BatchProblem& p = probs[lo];
+    int local = t - p.tile_begin;
+    int m0 = (local / p.ntn) * BM, n0 = (local % p.ntn) * BN;
+    switch (p.kind) {
+      case KIND_CAUCHY:
+        right_tile<KIND_CAUCHY, BM, BN, BK, WGM, WGN>(p.m, p.n, p.k, p.alpha, p.A, p.lda, p.g, p.r0,
+                                                      p.c0, p.beta, p.C, p.ldc, m0, n0, As, Hs);
+        break;
+      case KIND_TOEPLITZ:
+        right_tile<KIND_TOEPLITZ, BM, BN, BK, WGM, WGN>(p.m, p.n, p.k, p.alpha, p.A, p.lda, p.g,
+                                                        p.r0, p.c0, p.beta, p.C, p.ldc, m0, n0, As,
+                                                        Hs);
+        break;
+      case KIND_HANKEL:
+        right_tile<KIND_HANKEL, BM, BN, BK, WGM, WGN>(p.m, p.n, p.k, p.alpha, p.A, p.lda, p.g, p.r0,
+                                                      p.c0, p.beta, p.C, p.ldc, m0, n0, As, Hs);
+        break;
+      default:
+        right_tile<KIND_CAUCHYEIG, BM, BN, BK, WGM, WGN>(p.m, p.n, p.k, p.alpha, p.A, p.lda, p.g,
+                                                         p.r0, p.c0, p.beta, p.C, p.ldc, m0, n0,
+                                                         As, Hs);
+    }
+  }
+}
+
+// Tile shape of a grouped configuration.
+void grouped_tile_shape(int cfg, int* bm, int* bn) {
+  if (cfg == 1) {
+    *bm = 128;
+    *bn = 128;
+  } else {
+    *bm = 64;
+    *bn = 64;
+  }
+}
+
+hipError_t launch_grouped_f64(int cfg, const BatchProblem* dprobs, int nprob, int ntiles,
+                              int nworkgroups, hipStream_t s) {
+  if (ntiles <= 0) return hipSuccess;
+  int g = ntiles < nworkgroups ? ntiles : nworkgroups;
+  if (cfg == 1)
+    hipLaunchKernelGGL((sgemm_f64_grouped_kernel<128, 128, 16, 2, 2, 1>), dim3(g), dim3(256), 0, s,
+                       dprobs, nprob, ntiles);
+  else
+    hipLaunchKernelGGL((sgemm_f64_grouped_kernel<64, 64, 16, 2, 2, 2>), dim3(g), dim3(256), 0, s,
+                       dprobs, nprob, ntiles);
+  return hipGetLastError();
 }
 
 // ---------------------------------------------------------------------------

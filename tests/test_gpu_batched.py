@@ -1,0 +1,118 @@
+"""Batched uneven products (grouped launch and hipGraph plans) against the oracle."""
+import numpy as np
+import pytest
+
+from oracle import inputs
+from psmma_git_amd import api
+from test_gpu_parity import ORA_KIND, dev_colmajor, host
+
+pytestmark = pytest.mark.gpu
+
+KINDS = ["cauchy", "toeplitz", "hankel", "cauchyeig"]
+
+
+def make_batch(rng, nprob, nmax, beta_zero=True):
+    import torch
+
+    probs, keep = [], []
+    for i in range(nprob):
+        kind = KINDS[i % 4]
+        n = int(rng.integers(1, nmax + 1))
+        m = int(rng.integers(1, nmax + 1))
+        nglob = n + 3
+        if kind == "cauchy":
+            b = inputs.cauchy_generators(nglob, rng)
+            gens = dict(d=b[:, 0].copy(), f=b[:, 1].copy(), u=b[:, 2].copy(), v=b[:, 3].copy())
+            order = ["d", "f", "u", "v"]
+        elif kind in ("toeplitz", "hankel"):
+            gens = dict(g=rng.standard_normal(2 * nglob - 1), korder=nglob)
+            order = ["g"]
+        else:
+            e = inputs.config3(nglob, seed=int(rng.integers(1 << 30)), with_q=False)
+            gens = dict(f=e["f"], u=e["u"], v=e["v"], difl=e["difl"], difr=e["difr"])
+            order = ["f", "u", "v", "difl", "difr"]
+        dgens = [torch.from_numpy(np.ascontiguousarray(gens[k])).cuda() for k in order]
+        a = rng.uniform(-1, 1, (m, n))
+        c = rng.standard_normal((m, n))
+        beta = 0.0 if beta_zero else 0.5
+        da = dev_colmajor(a, ld=m + 1)
+        dc = dev_colmajor(c, ld=m + 2)
+        r0, c0 = (2, 1) if kind != "toeplitz" else (1, 2)
+        lg = len(gens["g"]) if "g" in gens else (nglob if kind == "cauchyeig" else 0)
+        p = api.make_problem(ORA_KIND[kind], 1.5, da, dgens, n, dc, beta=beta, r0=r0, c0=c0,
+                             lg=lg, korder=nglob if kind == "toeplitz" else 0)
+        want = ora_want(kind, a, gens, r0, c0, n, beta, c)
+        probs.append(p)
+        keep.append((dc, want, dgens, da))
+    return probs, keep
+
+
+def ora_want(kind, a, gens, r0, c0, n, beta, c):
+    from oracle import oracle as ora
+
+    return ora.structured_gemm_np(ORA_KIND[kind], 1.5, a, gens, r0, c0, n, beta=beta, c=c)
+
+
+def errors(keep):
+    from oracle import oracle as ora
+
+    return [ora.rel_fro(host(dc), want) for dc, want, _, _ in keep]
+
+
+def test_batched_fp64_one_shot(gpu):
+    import torch
+
+    rng = np.random.default_rng(21)
+    probs, keep = make_batch(rng, 24, 300, beta_zero=False)
+    api.batched_dev(probs)
+    torch.cuda.synchronize()
+    errs = errors(keep)
+    print("max err", max(errs))
+    assert max(errs) < 1e-13
+
+
+def test_batched_fp64_plan_replays(gpu):
+    # graph replays with eager work and synchronisation between them; every
+    # replay must equal the eager result
+    import torch
+
+    rng = np.random.default_rng(22)
+    probs, keep = make_batch(rng, 40, 400)
+    plan = api.BatchedPlan(probs)
+    cfg, nodes = plan.info()
+    print("plan cfg", cfg, "nodes", nodes)
+    assert nodes == 1
+    for rep in range(3):
+        for dc, _, _, _ in keep:
+            dc.fill_(float("nan"))  # eager work on the same stream between replays
+        torch.cuda.synchronize()
+        plan.launch()
+        torch.cuda.synchronize()
+        assert max(errors(keep)) < 1e-13, rep
+    plan.close()
+
+
+def test_batched_split_plan(gpu):
+    import torch
+
+    rng = np.random.default_rng(23)
+    probs, keep = make_batch(rng, 16, 300)
+    plan = api.BatchedPlan(probs, prec=api.PREC_SPLIT_I8, slices=0)
+    for rep in range(2):
+        plan.launch()
+        torch.cuda.synchronize()
+        errs = errors(keep)
+        print("split errs max", max(errs))
+        assert max(errs) <= 1e-12
+    plan.close()
+
+
+def test_batched_rejects_invalid_problem(gpu):
+    import torch
+
+    a = dev_colmajor(np.ones((4, 4)))
+    c = dev_colmajor(np.zeros((4, 4)))
+    g = torch.ones(3, dtype=torch.float64, device="cuda")  # too short for K = 4
+    p = api.make_problem(api.KIND_TOEPLITZ, 1.0, a, [g], 4, c, lg=3, korder=4)
+    with pytest.raises(api.PsmmaError):
+        api.batched_dev([p])
