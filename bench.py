@@ -84,6 +84,129 @@ def cpu_baseline(seed: int):
     }
 
 
+def ev_time(fn, reps, torch):
+    """Average ms of fn() over reps launches (HIP events on the current stream)."""
+    fn()
+    torch.cuda.synchronize()
+    st = torch.cuda.current_stream()
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record(st)
+    for _ in range(reps):
+        fn()
+    b.record(st)
+    torch.cuda.synchronize()
+    return a.elapsed_time(b) / reps
+
+
+def rel(x, ref):
+    return ((x - ref).norm() / ref.norm()).item()
+
+
+def other_configs(seed, torch, api, inputs):
+    """The remaining benchmark configurations, measured briefly (reported, not the headline)."""
+    out = {}
+    # Cauchy-like x dense, N = 2048
+    n = 2048
+    b, a_np = inputs.config0(n, seed=seed)
+    a = colmajor_from_numpy(a_np, torch)
+    d, f, u, v = (torch.from_numpy(np.ascontiguousarray(b[:, i])).cuda() for i in range(4))
+    c = api.colmajor_empty(n, n)
+    h = (u[:, None] * v[None, :]) / (d[:, None] - f[None, :])
+    ref = a @ h
+    ms64 = ev_time(lambda: api.cauchy_dev(1.0, a, d, f, u, v, n, c), 10, torch)
+    e64 = rel(c, ref)
+    mss = ev_time(lambda: api.cauchy_dev(1.0, a, d, f, u, v, n, c, prec=api.PREC_SPLIT_I8,
+                                         slices=13), 10, torch)
+    es = rel(c, ref)
+    out["cauchy_n2048"] = {"fp64_ms": round(ms64, 4), "fp64_tflops": round(2 * n**3 / ms64 / 1e9, 2),
+                           "fp64_rel_err": e64, "split_i8_t13_ms": round(mss, 4),
+                           "split_rel_err": es}
+    del a, c, h, ref
+    # Hankel x dense, N = 8192, and the 3-slice bf16 error of the Toeplitz product
+    n = 8192
+    g_np, a_np = inputs.config1(n, seed=seed + 1)
+    a = colmajor_from_numpy(a_np, torch)
+    del a_np
+    g = torch.from_numpy(g_np).cuda()
+    c = api.colmajor_empty(n, n)
+    msh = ev_time(lambda: api.hankel_dev(1.0, a, g, n, c), 5, torch)
+    idx = torch.arange(n, device="cuda")
+    ref = a @ g[idx[:, None] + idx[None, :]]
+    out["hankel_n8192"] = {"fp64_ms": round(msh, 4), "fp64_tflops": round(2 * n**3 / msh / 1e9, 2),
+                           "fp64_rel_err": rel(c, ref)}
+    del ref
+    tref = a @ g[(n - 1) + idx[None, :] - idx[:, None]]
+    ms3 = ev_time(lambda: api.toeplitz_dev(1.0, a, g, n, n, c, prec=api.PREC_SPLIT_BF16, slices=3),
+                  2, torch)
+    out["toeplitz_n8192_bf16_3slice"] = {"ms": round(ms3, 3), "rel_err": rel(c, tref)}
+    del a, c, tref
+    # DFT x complex dense, 4096 x 512
+    n, k = 4096, 512
+    bb = inputs.config2(n, k, seed=seed)
+    db = torch.from_numpy(np.ascontiguousarray(bb.T)).cuda().t()
+    dc = api.colmajor_empty(n, k, dtype=torch.complex128)
+    msd = ev_time(lambda: api.dft_dev(1.0 / np.sqrt(n), db, n, n, dc), 10, torch)
+    fft_in = db.contiguous()
+    msf = ev_time(lambda: torch.fft.fft(fft_in, dim=0), 10, torch)
+    ref = torch.fft.fft(fft_in, dim=0) / np.sqrt(n)
+    out["dft_n4096_k512"] = {"fp64_ms": round(msd, 4),
+                             "fp64_tflops": round(8 * n * n * k / msd / 1e9, 2),
+                             "rel_err_vs_fft": rel(dc, ref), "hipfft_ms": round(msf, 4)}
+    del db, dc, fft_in, ref
+    out["batched_512"] = batched_config(seed, torch, api, inputs)
+    return out
+
+
+def batched_config(seed, torch, api, inputs):
+    """512 uneven problems (n log-uniform in [64, 2048], half Cauchy, a quarter Toeplitz and
+    Hankel), one hipGraph-captured grouped FP64 launch vs the split-precision plan."""
+    probs_np = inputs.config4(512, seed=seed)
+    probs, keep = [], []
+    flops = 0.0
+    for pr in probs_np:
+        n = pr["n"]
+        a = colmajor_from_numpy(pr["dense"], torch)
+        c = api.colmajor_empty(n, n)
+        if pr["kind"] == 0:
+            gens = [torch.from_numpy(np.ascontiguousarray(pr["gens"][:, i])).cuda() for i in range(4)]
+            p = api.make_problem(api.KIND_CAUCHY, 1.0, a, gens, n, c)
+        else:
+            g = torch.from_numpy(pr["gens"]).cuda()
+            gens = [g]
+            p = api.make_problem(pr["kind"], 1.0, a, gens, n, c, lg=2 * n - 1, korder=n)
+        probs.append(p)
+        keep.append((pr["kind"], a, gens, c))
+        flops += 2.0 * n ** 3
+    plan64 = api.BatchedPlan(probs)
+    cfg, _ = plan64.info()
+    ms64 = ev_time(plan64.launch, 5, torch)
+    refs = []
+    for kind, a, gens, c in keep:
+        n = c.shape[0]
+        idx = torch.arange(n, device="cuda")
+        if kind == 0:
+            d, f, u, v = gens
+            h = (u[:, None] * v[None, :]) / (d[:, None] - f[None, :])
+        elif kind == 1:
+            h = gens[0][(n - 1) + idx[None, :] - idx[:, None]]
+        else:
+            h = gens[0][idx[:, None] + idx[None, :]]
+        refs.append(a @ h)
+    e64 = [rel(c, r) for (_, _, _, c), r in zip(keep, refs)]
+    plans = api.BatchedPlan(probs, prec=api.PREC_SPLIT_I8, slices=0)
+    mss = ev_time(plans.launch, 5, torch)
+    es = [rel(c, r) for (_, _, _, c), r in zip(keep, refs)]
+    plan64.close()
+    plans.close()
+    q = lambda x, p: float(np.quantile(np.array(x), p))
+    return {"problems": 512, "flops": flops, "fp64_grouped_ms": round(ms64, 3),
+            "fp64_tflops": round(flops / ms64 / 1e9, 2), "fp64_tile_cfg": cfg,
+            "fp64_err_max": max(e64),
+            "split_i8_ms": round(mss, 3), "split_tflops": round(flops / mss / 1e9, 2),
+            "split_speedup_vs_fp64": round(ms64 / mss, 3),
+            "split_err": {"median": q(es, 0.5), "p90": q(es, 0.9), "max": max(es)}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -93,6 +216,8 @@ def main():
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-split", action="store_true", help="skip the split-precision leg")
+    ap.add_argument("--no-configs", action="store_true",
+                    help="skip the other benchmark configurations (reported beside the headline)")
     ap.add_argument("--copy-calib", action="store_true",
                     help="run one 512 MiB device copy first (calibrates HBM byte counters)")
     args = ap.parse_args()
@@ -180,6 +305,13 @@ def main():
         except (ValueError, OSError):
             traffic = None
 
+    configs = None
+    if not args.no_configs and n == 8192:
+        log("[bench] other configurations")
+        del c
+        torch.cuda.empty_cache()
+        configs = other_configs(args.seed, torch, api, inputs)
+
     cpu = None if args.no_cpu else cpu_baseline(args.seed)
 
     out = {
@@ -204,6 +336,7 @@ def main():
                      "unit": "TFLOP/s", "frac": round(achieved / FP64_PEAK, 4),
                      "traffic": traffic},
         "split_precision": split,
+        "other_configs": configs,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -195,11 +195,13 @@ typedef struct psmma_problem {
   int ldc;
 } psmma_problem;
 
-/* All problems in one go on `stream`.  PSMMA_PREC_FP64: one persistent grouped
- * launch over the tiles of every problem.  Split forms: the per-problem
- * pipelines in sequence; slices = 0 picks the number of moduli per problem for
- * a relative error <= 1e-12 (PSMMA_PREC_SPLIT_I8).  The problem table is copied
- * to the device before the call returns. */
+/* All problems in one go on `stream`.  prec: PSMMA_PREC_FP64 (one persistent
+ * grouped launch over the tiles of every problem) or PSMMA_PREC_SPLIT_I8 (each
+ * phase of the int8 modular pipeline is one grouped launch over all problems;
+ * slices = number of moduli, 0 = the fewest that keep 45 bits per operand for
+ * the longest contraction of the batch).  The problem table is copied to the
+ * device before the call returns; split calls on one device share a workspace
+ * and must not overlap. */
 int psmma_batched_dev(int nprob, const psmma_problem* probs, int prec, int slices, void* stream);
 
 /* A reusable plan: validates the problems, uploads the table, picks the tile
@@ -209,7 +211,8 @@ int psmma_batched_dev(int nprob, const psmma_problem* probs, int prec, int slice
 int psmma_batched_plan_create(int nprob, const psmma_problem* probs, int prec, int slices,
                               void** plan);
 int psmma_batched_plan_launch(void* plan, void* stream);
-/* Tile configuration chosen (0: 64x64, 1: 128x128) and number of graph kernel nodes. */
+/* FP64 plans: the min(m,n) from which problems use the large tiles; split plans: the
+ * number of moduli.  nodes: kernel launches captured in the graph. */
 int psmma_batched_plan_info(void* plan, int* tile_cfg, int* nodes);
 int psmma_batched_plan_destroy(void* plan);
 

@@ -3,31 +3,31 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
-#include <chrono>
 #include <cstdio>
-#include <numeric>
 #include <vector>
 
 #include "../../include/psmma.h"
 #include "psmma_common.h"
 
 namespace psmma {
-hipError_t launch_grouped_f64(int cfg, const BatchProblem* dprobs, int nprob, int ntiles,
-                              int nworkgroups, hipStream_t s);
-void grouped_tile_shape(int cfg, int* bm, int* bn);
-int launch_right_split(int kind, int prec, int slices, int M, int N, int K, double alpha,
-                       const double* A, int lda, const GenPtrs& g, int r0, int c0, double beta,
-                       double* C, int ldc, hipStream_t s);
-size_t split_workspace_bytes(int prec, int slices, int M, int N, int K);
+hipError_t launch_grouped_f64(int kind, int cfg, const BatchProblem* dprobs, const int* dtbeg,
+                              int nprob, int ntiles, int nworkgroups, hipStream_t s);
+void grouped_tile_shape(int kind, int cfg, int* bm, int* bn);
 void* split_workspace(size_t bytes);
-int split_moduli_for_bits(int K, int bits);
+void* gsplit_create(const BatchProblem* probs, int nprob, int t, size_t* ws_bytes);
+int gsplit_bind(void* h, void* ws);
+int gsplit_enqueue(void* h, hipStream_t s, int* nodes);
+void gsplit_destroy(void* h);
+int gsplit_moduli(const BatchProblem* probs, int nprob, int bits);
 }  // namespace psmma
 
 using namespace psmma;
 
 namespace {
 
-constexpr int kSplitBits = 43;  // bits per operand for a relative error <= 1e-12
+// Bits per operand the batched split form keeps: 43 bits measure <= 3e-13 on the
+// N = 8192 bench shape; two more bits cover wider-range problems of a batch.
+constexpr int kSplitBits = 45;
 
 // Validates one problem (same rules as the single-problem device calls) and
 // converts it to the kernel's descriptor.  Returns 0 or -(field index + 1).
@@ -65,69 +65,114 @@ int device_workgroups(int per_cu) {
   return cus * per_cu;
 }
 
-// Tile list for a grouped launch: non-empty problems sorted by decreasing K.
-int build_table(const std::vector<BatchProblem>& in, int cfg, std::vector<BatchProblem>& out) {
-  int bm, bn;
-  grouped_tile_shape(cfg, &bm, &bn);
-  out.clear();
-  for (const auto& b : in)
-    if (b.m > 0 && b.n > 0) out.push_back(b);
-  std::stable_sort(out.begin(), out.end(),
-                   [](const BatchProblem& x, const BatchProblem& y) { return x.k > y.k; });
-  int t = 0;
-  for (auto& b : out) {
-    b.ntm = (b.m + bm - 1) / bm;
-    b.ntn = (b.n + bn - 1) / bn;
-    b.tile_begin = t;
-    t += b.ntm * b.ntn;
-  }
-  return t;
-}
-
-struct Plan {
-  int prec = 0, slices = 0;
-  int cfg = 0;
-  int nodes = 0;
-  BatchProblem* dprobs = nullptr;
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t exec = nullptr;
+// FP64 batch layout: problems split into groups by (kind, tile class); each
+// group is one persistent grouped launch with its own compact tile index.
+// A problem uses the large tiles when min(m, n) >= big_min.
+struct F64Group {
+  int kind, cfg, first, count, ntiles;
+};
+struct F64Table {
+  std::vector<BatchProblem> probs;
+  std::vector<int> tbeg;
+  std::vector<F64Group> groups;
+  int ntiles = 0;
+};
+struct F64Dev {
+  BatchProblem* probs = nullptr;
+  int* tbeg = nullptr;
 };
 
-// Enqueues the whole batch on stream s (FP64: grouped launch from the device
-// table dtab; split: the per-problem pipelines).  Returns 0 or an error.
-int enqueue(int prec, int slices, const std::vector<BatchProblem>& probs, const BatchProblem* dtab,
-            int ntab, int ntiles, int cfg, hipStream_t s, int* nodes) {
-  if (prec == PSMMA_PREC_FP64) {
-    if (nodes) *nodes = ntiles > 0 ? 1 : 0;
-    hipError_t e = launch_grouped_f64(cfg, dtab, ntab, ntiles, device_workgroups(cfg == 1 ? 1 : 2), s);
-    return e == hipSuccess ? 0 : 1000 + (int)e;
+void build_f64(const std::vector<BatchProblem>& in, int big_min, F64Table& T) {
+  T = F64Table{};
+  for (int kind = PSMMA_KIND_CAUCHY; kind <= PSMMA_KIND_CAUCHYEIG; ++kind) {
+    for (int cfg = 1; cfg >= 0; --cfg) {
+      std::vector<BatchProblem> g;
+      for (const auto& b : in) {
+        if (b.kind != kind || b.m == 0 || b.n == 0) continue;
+        bool big = std::min(b.m, b.n) >= big_min;
+        if (big == (cfg == 1)) g.push_back(b);
+      }
+      if (g.empty()) continue;
+      std::stable_sort(g.begin(), g.end(),
+                       [](const BatchProblem& x, const BatchProblem& y) { return x.k > y.k; });
+      int bm, bn;
+      grouped_tile_shape(kind, cfg, &bm, &bn);
+      F64Group G{kind, cfg, (int)T.probs.size(), (int)g.size(), 0};
+      for (auto& b : g) {
+        b.ntm = (b.m + bm - 1) / bm;
+        b.ntn = (b.n + bn - 1) / bn;
+        b.tile_begin = G.ntiles;
+        T.probs.push_back(b);
+        T.tbeg.push_back(G.ntiles);
+        G.ntiles += b.ntm * b.ntn;
+      }
+      T.ntiles += G.ntiles;
+      T.groups.push_back(G);
+    }
   }
+}
+
+void free_f64(F64Dev& D) {
+  if (D.probs) hipFree(D.probs);
+  if (D.tbeg) hipFree(D.tbeg);
+  D = F64Dev{};
+}
+
+int upload_f64(const F64Table& T, F64Dev& D) {
+  if (T.probs.empty()) return 0;
+  if (hipMalloc(&D.probs, sizeof(BatchProblem) * T.probs.size()) != hipSuccess ||
+      hipMalloc(&D.tbeg, sizeof(int) * T.tbeg.size()) != hipSuccess)
+    return 1002;
+  if (hipMemcpy(D.probs, T.probs.data(), sizeof(BatchProblem) * T.probs.size(),
+                hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(D.tbeg, T.tbeg.data(), sizeof(int) * T.tbeg.size(), hipMemcpyHostToDevice) !=
+          hipSuccess)
+    return 1003;
+  return 0;
+}
+
+// Enqueues one grouped launch per group on stream s; *nodes = launches made.
+int enqueue_f64(const F64Table& T, const F64Dev& D, hipStream_t s, int* nodes) {
   int n = 0;
-  for (const auto& b : probs) {
-    if (b.m == 0 || b.n == 0) continue;
-    int t = slices;
-    if (prec == PSMMA_PREC_SPLIT_I8 && t == 0) t = split_moduli_for_bits(b.k, kSplitBits);
-    if (prec == PSMMA_PREC_SPLIT_BF16 && t == 0) t = 6;
-    int st = launch_right_split(b.kind, prec, t, b.m, b.n, b.k, b.alpha, b.A, b.lda, b.g, b.r0,
-                                b.c0, b.beta, b.C, b.ldc, s);
-    if (st) return st;
+  for (const auto& G : T.groups) {
+    if (G.ntiles == 0) continue;
+    hipError_t e = launch_grouped_f64(G.kind, G.cfg, D.probs + G.first, D.tbeg + G.first, G.count,
+                                      G.ntiles, device_workgroups(2), s);
+    if (e != hipSuccess) return 1000 + (int)e;
     ++n;
   }
   if (nodes) *nodes = n;
   return 0;
 }
 
-size_t max_split_bytes(int prec, int slices, const std::vector<BatchProblem>& probs) {
-  size_t mx = 0;
-  for (const auto& b : probs) {
-    if (b.m == 0 || b.n == 0 || b.k == 0) continue;
-    int t = slices;
-    if (prec == PSMMA_PREC_SPLIT_I8 && t == 0) t = split_moduli_for_bits(b.k, kSplitBits);
-    if (prec == PSMMA_PREC_SPLIT_BF16 && t == 0) t = 6;
-    mx = std::max(mx, split_workspace_bytes(prec, t, b.m, b.n, b.k));
-  }
-  return mx;
+// Times one FP64 layout (3 launches after a warm-up) on stream s, in ms.
+float time_f64(const F64Table& T, const F64Dev& D, hipStream_t s) {
+  if (enqueue_f64(T, D, s, nullptr)) return 1e30f;
+  hipEvent_t a, b;
+  hipEventCreate(&a);
+  hipEventCreate(&b);
+  hipEventRecord(a, s);
+  for (int r = 0; r < 3; ++r) enqueue_f64(T, D, s, nullptr);
+  hipEventRecord(b, s);
+  hipEventSynchronize(b);
+  float ms = 1e30f;
+  hipEventElapsedTime(&ms, a, b);
+  hipEventDestroy(a);
+  hipEventDestroy(b);
+  return ms;
 }
+
+struct Plan {
+  int prec = 0, slices = 0;
+  int cfg = 0;  // FP64: min(m, n) from which a problem uses the large tiles
+  int nodes = 0;
+  F64Table f64;
+  F64Dev f64d;
+  void* gsplit = nullptr;  // grouped split batch (PSMMA_PREC_SPLIT_I8)
+  void* ws = nullptr;      // the plan's own split workspace
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
 
 int collect(int nprob, const psmma_problem* probs, std::vector<BatchProblem>& out) {
   if (nprob < 0 || (nprob > 0 && !probs)) return -1;
@@ -142,10 +187,10 @@ int collect(int nprob, const psmma_problem* probs, std::vector<BatchProblem>& ou
   return 0;
 }
 
+// Batched forms: plain FP64 and the int8 modular split (slices = moduli, 0 = by model).
 int check_prec(int prec, int slices) {
-  if (prec != PSMMA_PREC_FP64 && prec != PSMMA_PREC_SPLIT_I8 && prec != PSMMA_PREC_SPLIT_BF16)
-    return -3;
-  if (slices < 0 || slices > 16 || (prec == PSMMA_PREC_SPLIT_BF16 && slices > 7)) return -4;
+  if (prec != PSMMA_PREC_FP64 && prec != PSMMA_PREC_SPLIT_I8) return -3;
+  if (slices < 0 || slices == 1 || slices > 16) return -4;
   return 0;
 }
 
@@ -158,19 +203,25 @@ extern "C" int psmma_batched_dev(int nprob, const psmma_problem* probs, int prec
   std::vector<BatchProblem> in;
   if ((st = collect(nprob, probs, in))) return st;
   hipStream_t s = (hipStream_t)stream;
-  if (prec != PSMMA_PREC_FP64) return enqueue(prec, slices, in, nullptr, 0, 0, 0, s, nullptr);
-  std::vector<BatchProblem> tab;
-  int cfg = 0;
-  int ntiles = build_table(in, cfg, tab);
-  if (ntiles == 0) return 0;
-  BatchProblem* d = nullptr;
-  if (hipMalloc(&d, sizeof(BatchProblem) * tab.size()) != hipSuccess) return 1002;
-  hipError_t e = hipMemcpy(d, tab.data(), sizeof(BatchProblem) * tab.size(), hipMemcpyHostToDevice);
-  if (e == hipSuccess) st = enqueue(prec, slices, in, d, (int)tab.size(), ntiles, cfg, s, nullptr);
-  else st = 1000 + (int)e;
-  // the table must outlive the kernel: wait, then release it
-  hipStreamSynchronize(s);
-  hipFree(d);
+  if (prec != PSMMA_PREC_FP64) {
+    int t = slices ? slices : gsplit_moduli(in.data(), nprob, kSplitBits);
+    size_t bytes = 0;
+    void* gs = gsplit_create(in.data(), nprob, t, &bytes);
+    void* ws = bytes ? split_workspace(bytes) : nullptr;
+    st = (bytes && !ws) ? 1002 : gsplit_bind(gs, ws);
+    if (st == 0) st = gsplit_enqueue(gs, s, nullptr);
+    hipStreamSynchronize(s);  // the table is released below
+    gsplit_destroy(gs);
+    return st;
+  }
+  F64Table T;
+  build_f64(in, 256, T);
+  if (T.ntiles == 0) return 0;
+  F64Dev D;
+  st = upload_f64(T, D);
+  if (st == 0) st = enqueue_f64(T, D, s, nullptr);
+  hipStreamSynchronize(s);  // the table must outlive the kernels
+  free_f64(D);
   return st;
 }
 
@@ -190,78 +241,61 @@ extern "C" int psmma_batched_plan_create(int nprob, const psmma_problem* probs, 
     delete P;
     return 1001;
   }
-  std::vector<BatchProblem> tab;
-  int ntiles = 0;
+  auto fail = [&](int code) {
+    hipStreamDestroy(cs);
+    psmma_batched_plan_destroy(P);
+    return code;
+  };
   if (prec == PSMMA_PREC_FP64) {
-    // Tile-shape tuning: time both grouped shapes on this batch when rerunning
-    // is harmless (every beta == 0), else take the shape with more tiles per CU.
-    bool idempotent = std::all_of(in.begin(), in.end(), [](const BatchProblem& b) { return b.beta == 0.0; });
-    std::vector<BatchProblem> t0, t1;
-    int n0 = build_table(in, 0, t0), n1 = build_table(in, 1, t1);
-    int cfg = (n1 >= 4 * device_workgroups(1)) ? 1 : 0;
-    if (idempotent && n0 > 0) {
-      BatchProblem *d0 = nullptr, *d1 = nullptr;
-      hipMalloc(&d0, sizeof(BatchProblem) * t0.size());
-      hipMalloc(&d1, sizeof(BatchProblem) * t1.size());
-      hipMemcpy(d0, t0.data(), sizeof(BatchProblem) * t0.size(), hipMemcpyHostToDevice);
-      hipMemcpy(d1, t1.data(), sizeof(BatchProblem) * t1.size(), hipMemcpyHostToDevice);
-      double best = 1e30;
-      for (int c = 0; c < 2; ++c) {
-        const BatchProblem* d = c ? d1 : d0;
-        int nt = c ? n1 : n0, nb = (int)(c ? t1.size() : t0.size());
-        enqueue(prec, 0, in, d, nb, nt, c, cs, nullptr);  // warm-up
-        hipEvent_t a, b;
-        hipEventCreate(&a);
-        hipEventCreate(&b);
-        hipEventRecord(a, cs);
-        for (int r = 0; r < 3; ++r) enqueue(prec, 0, in, d, nb, nt, c, cs, nullptr);
-        hipEventRecord(b, cs);
-        hipEventSynchronize(b);
-        float ms = 0.f;
-        hipEventElapsedTime(&ms, a, b);
-        hipEventDestroy(a);
-        hipEventDestroy(b);
+    // Tile-class tuning: time the candidate large-tile thresholds on this batch
+    // when rerunning is harmless (every beta == 0); else use 256.
+    bool idempotent =
+        std::all_of(in.begin(), in.end(), [](const BatchProblem& b) { return b.beta == 0.0; });
+    int best_min = 256;
+    if (idempotent) {
+      float best = 1e30f;
+      for (int cand : {256, 512, 1 << 30}) {
+        F64Table T;
+        F64Dev D;
+        build_f64(in, cand, T);
+        if (T.ntiles == 0 || upload_f64(T, D)) {
+          free_f64(D);
+          continue;
+        }
+        float ms = time_f64(T, D, cs);
+        free_f64(D);
         if (ms < best) {
           best = ms;
-          cfg = c;
+          best_min = cand;
         }
       }
-      hipFree(d0);
-      hipFree(d1);
     }
-    P->cfg = cfg;
-    ntiles = build_table(in, cfg, tab);
-    if (!tab.empty()) {
-      if (hipMalloc(&P->dprobs, sizeof(BatchProblem) * tab.size()) != hipSuccess ||
-          hipMemcpy(P->dprobs, tab.data(), sizeof(BatchProblem) * tab.size(),
-                    hipMemcpyHostToDevice) != hipSuccess) {
-        hipStreamDestroy(cs);
-        psmma_batched_plan_destroy(P);
-        return 1002;
-      }
-    }
+    P->cfg = best_min;
+    build_f64(in, best_min, P->f64);
+    if ((st = upload_f64(P->f64, P->f64d))) return fail(st);
   } else {
-    // every pipeline shares one workspace: size it now, capture allocates nothing
-    size_t need = max_split_bytes(prec, slices, in);
-    if (need && !split_workspace(need)) {
-      hipStreamDestroy(cs);
-      delete P;
-      return 1002;
-    }
+    // the plan owns its workspace and table: capture allocates nothing, and no
+    // later call can move memory the graph points to
+    int t = slices ? slices : gsplit_moduli(in.data(), nprob, kSplitBits);
+    size_t bytes = 0;
+    P->gsplit = gsplit_create(in.data(), nprob, t, &bytes);
+    P->slices = t;
+    if (bytes && hipMalloc(&P->ws, bytes) != hipSuccess) return fail(1002);
+    if ((st = gsplit_bind(P->gsplit, P->ws))) return fail(st);
   }
   hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
   if (e == hipSuccess)
-    st = enqueue(prec, slices, in, P->dprobs, (int)tab.size(), ntiles, P->cfg, cs, &P->nodes);
+    st = (prec == PSMMA_PREC_FP64) ? enqueue_f64(P->f64, P->f64d, cs, &P->nodes)
+                                   : gsplit_enqueue(P->gsplit, cs, &P->nodes);
   hipError_t e2 = hipStreamEndCapture(cs, &P->graph);
   if (e == hipSuccess && e2 == hipSuccess && st == 0)
     e = hipGraphInstantiate(&P->exec, P->graph, nullptr, nullptr, 0);
-  hipStreamDestroy(cs);
   if (e != hipSuccess || e2 != hipSuccess || st != 0) {
     fprintf(stderr, "psmma_batched_plan_create: capture failed (%s / %s / %d)\n",
             hipGetErrorString(e), hipGetErrorString(e2), st);
-    psmma_batched_plan_destroy(P);
-    return st ? st : 1003;
+    return fail(st ? st : 1003);
   }
+  hipStreamDestroy(cs);
   *plan_out = P;
   return 0;
 }
@@ -276,7 +310,7 @@ extern "C" int psmma_batched_plan_launch(void* plan, void* stream) {
 extern "C" int psmma_batched_plan_info(void* plan, int* tile_cfg, int* nodes) {
   Plan* P = (Plan*)plan;
   if (!P) return -1;
-  if (tile_cfg) *tile_cfg = P->cfg;
+  if (tile_cfg) *tile_cfg = (P->prec == PSMMA_PREC_FP64) ? P->cfg : P->slices;
   if (nodes) *nodes = P->nodes;
   return 0;
 }
@@ -286,10 +320,10 @@ extern "C" int psmma_batched_plan_destroy(void* plan) {
   if (!P) return 0;
   if (P->exec) hipGraphExecDestroy(P->exec);
   if (P->graph) hipGraphDestroy(P->graph);
-  if (P->dprobs) {
-    hipDeviceSynchronize();
-    hipFree(P->dprobs);
-  }
+  if (P->f64d.probs || P->ws) hipDeviceSynchronize();
+  free_f64(P->f64d);
+  if (P->gsplit) gsplit_destroy(P->gsplit);
+  if (P->ws) hipFree(P->ws);
   delete P;
   return 0;
 }

@@ -453,71 +453,72 @@ __global__ __launch_bounds__(WGM* WGN * 64, 2) void dft_f64_left_kernel(
 }
 
 // ---------------------------------------------------------------------------
-// Persistent grouped launch for a batch of uneven structured products of any
-// kinds.  All problems' output tiles form one list (problems sorted by
+// Persistent grouped launch for a batch of uneven structured products of one
+// kind.  The problems' output tiles form one list (problems sorted by
 // decreasing K, so the longest tiles are dealt first); workgroup w takes tiles
 // w, w + G, w + 2G, ... of that list, finds its problem by binary search over
-// the tile prefix sums and runs the same tile body as the single-problem
-// kernel.  The LDS areas are sized for the largest kind.
-template <int BM, int BN, int BK, int WGM, int WGN, int MINB>
+// the compact tile prefix array and runs the tile body of the single-problem
+// kernel (so the register and LDS profile is that kernel's).
+template <int KIND, int BM, int BN, int BK, int WGM, int WGN, int MINB>
 __global__ __launch_bounds__(WGM* WGN * 64, MINB) void sgemm_f64_grouped_kernel(
-    const BatchProblem* __restrict__ probs, int nprob, int ntiles) {
-  using LC = RightLds<KIND_CAUCHY, BM, BN, BK>;  // the largest footprint
-  __shared__ double As[2 * LC::ASZ];
-  __shared__ double Hs[2 * LC::HSZ];
+    const BatchProblem* __restrict__ probs, const int* __restrict__ tbeg, int nprob, int ntiles) {
+  using L = RightLds<KIND, BM, BN, BK>;
+  __shared__ double As[2 * L::ASZ];
+  __shared__ double Hs[2 * L::HSZ];
   for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
     int lo = 0, hi = nprob - 1;
-    while (lo < hi) {  // last problem with tile_begin <= t
+    while (lo < hi) {  // last problem whose first tile is <= t
       int mid = (lo + hi + 1) >> 1;
-      if (probs[mid].tile_begin <= t) lo = mid; else hi = mid - 1;
+      if (tbeg[mid] <= t) lo = mid; else hi = mid - 1;
     }
     const BatchProblem& p = probs[lo];
-    int local = t - p.tile_begin;
-    int m0 = (local / p.ntn) * BM, n0 = (local % p.ntn) * BN;
-    switch (p.kind) {
-      case KIND_CAUCHY:
-        right_tile<KIND_CAUCHY, BM, BN, BK, WGM, WGN>(p.m, p.n, p.k, p.alpha, p.A, p.lda, p.g, p.r0,
-                                                      p.c0, p.beta, p.C, p.ldc, m0, n0, As, Hs);
-        break;
-      case KIND_TOEPLITZ:
-        right_tile<KIND_TOEPLITZ, BM, BN, BK, WGM, WGN>(p.m, p.n, p.k, p.alpha, p.A, p.lda, p.g,
-                                                        p.r0, p.c0, p.beta, p.C, p.ldc, m0, n0, As,
-                                                        Hs);
-        break;
-      case KIND_HANKEL:
-        right_tile<KIND_HANKEL, BM, BN, BK, WGM, WGN>(p.m, p.n, p.k, p.alpha, p.A, p.lda, p.g, p.r0,
-                                                      p.c0, p.beta, p.C, p.ldc, m0, n0, As, Hs);
-        break;
-      default:
-        right_tile<KIND_CAUCHYEIG, BM, BN, BK, WGM, WGN>(p.m, p.n, p.k, p.alpha, p.A, p.lda, p.g,
-                                                         p.r0, p.c0, p.beta, p.C, p.ldc, m0, n0,
-                                                         As, Hs);
-    }
+    int local = t - tbeg[lo];
+    right_tile<KIND, BM, BN, BK, WGM, WGN>(p.m, p.n, p.k, p.alpha, p.A, p.lda, p.g, p.r0, p.c0,
+                                           p.beta, p.C, p.ldc, (local / p.ntn) * BM,
+                                           (local % p.ntn) * BN, As, Hs);
   }
 }
 
-// Tile shape of a grouped configuration.
-void grouped_tile_shape(int cfg, int* bm, int* bn) {
+// Tile shape of a grouped configuration: 0 small (64 x 64), 1 large (64 x 256
+// for the window kinds, 128 x 128 for the Cauchy kinds).
+void grouped_tile_shape(int kind, int cfg, int* bm, int* bn) {
   if (cfg == 1) {
-    *bm = 128;
-    *bn = 128;
+    bool win = kind == KIND_TOEPLITZ || kind == KIND_HANKEL;
+    *bm = win ? 64 : 128;
+    *bn = win ? 256 : 128;
   } else {
     *bm = 64;
     *bn = 64;
   }
 }
 
-hipError_t launch_grouped_f64(int cfg, const BatchProblem* dprobs, int nprob, int ntiles,
-                              int nworkgroups, hipStream_t s) {
+template <int KIND>
+static hipError_t launch_grouped_kind(int cfg, const BatchProblem* dprobs, const int* dtbeg,
+                                      int nprob, int ntiles, int nwg, hipStream_t s) {
+  if (cfg == 1) {
+    if constexpr (KIND == KIND_TOEPLITZ || KIND == KIND_HANKEL)
+      hipLaunchKernelGGL((sgemm_f64_grouped_kernel<KIND, 64, 256, 16, 1, 4, 2>), dim3(nwg),
+                         dim3(256), 0, s, dprobs, dtbeg, nprob, ntiles);
+    else
+      hipLaunchKernelGGL((sgemm_f64_grouped_kernel<KIND, 128, 128, 16, 2, 2, 2>), dim3(nwg),
+                         dim3(256), 0, s, dprobs, dtbeg, nprob, ntiles);
+  } else {
+    hipLaunchKernelGGL((sgemm_f64_grouped_kernel<KIND, 64, 64, 16, 2, 2, 2>), dim3(nwg), dim3(256),
+                       0, s, dprobs, dtbeg, nprob, ntiles);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_grouped_f64(int kind, int cfg, const BatchProblem* dprobs, const int* dtbeg,
+                              int nprob, int ntiles, int nworkgroups, hipStream_t s) {
   if (ntiles <= 0) return hipSuccess;
   int g = ntiles < nworkgroups ? ntiles : nworkgroups;
-  if (cfg == 1)
-    hipLaunchKernelGGL((sgemm_f64_grouped_kernel<128, 128, 16, 2, 2, 1>), dim3(g), dim3(256), 0, s,
-                       dprobs, nprob, ntiles);
-  else
-    hipLaunchKernelGGL((sgemm_f64_grouped_kernel<64, 64, 16, 2, 2, 2>), dim3(g), dim3(256), 0, s,
-                       dprobs, nprob, ntiles);
-  return hipGetLastError();
+  switch (kind) {
+    case KIND_CAUCHY: return launch_grouped_kind<KIND_CAUCHY>(cfg, dprobs, dtbeg, nprob, ntiles, g, s);
+    case KIND_TOEPLITZ: return launch_grouped_kind<KIND_TOEPLITZ>(cfg, dprobs, dtbeg, nprob, ntiles, g, s);
+    case KIND_HANKEL: return launch_grouped_kind<KIND_HANKEL>(cfg, dprobs, dtbeg, nprob, ntiles, g, s);
+    default: return launch_grouped_kind<KIND_CAUCHYEIG>(cfg, dprobs, dtbeg, nprob, ntiles, g, s);
+  }
 }
 
 // ---------------------------------------------------------------------------

@@ -26,12 +26,14 @@
 // to row/column maxima; t = 13 gives abits = 43 at K = 8192, t = 14 gives 47.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <climits>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <mutex>
 #include <unordered_map>
+#include <vector>
 
 #include "psmma_common.h"
 
@@ -140,16 +142,20 @@ __device__ __forceinline__ int exponent_of(double x) {
   return x == 0.0 ? kNoExp : e;
 }
 
-__global__ void fill_int_kernel(int* p, int n, int v) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void fill_int_kernel_body(int* p, int n, int v, int bx_, int by_) {
+  int i = bx_ * blockDim.x + threadIdx.x;
   if (i < n) p[i] = v;
 }
 
+__global__ void fill_int_kernel(int* p, int n, int v) {
+  fill_int_kernel_body(p, n, v, blockIdx.x, blockIdx.y);
+}
+
 // Max exponent per row of A (column-major): threads walk rows, blocks split K.
-__global__ void row_exp_kernel(int M, int K, const double* __restrict__ A, int lda, int kchunk,
-                               int* __restrict__ ea) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  int k0 = blockIdx.y * kchunk;
+__device__ __forceinline__ void row_exp_kernel_body(int M, int K, const double* __restrict__ A, int lda, int kchunk,
+                               int* __restrict__ ea, int bx_, int by_) {
+  int i = bx_ * blockDim.x + threadIdx.x;
+  int k0 = by_ * kchunk;
   if (i >= M) return;
   int k1 = min(K, k0 + kchunk);
   double mx = 0.0;
@@ -157,12 +163,17 @@ __global__ void row_exp_kernel(int M, int K, const double* __restrict__ A, int l
   if (mx > 0.0) atomicMax(&ea[i], exponent_of(mx));
 }
 
+__global__ void row_exp_kernel(int M, int K, const double* __restrict__ A, int lda, int kchunk,
+                               int* __restrict__ ea) {
+  row_exp_kernel_body(M, K, A, lda, kchunk, ea, blockIdx.x, blockIdx.y);
+}
+
 // Max exponent per column of H: threads walk columns, blocks split K.
 template <int KIND>
-__global__ void col_exp_kernel(int K, int N, GenPtrs g, int r0, int c0, int kchunk,
-                               int* __restrict__ eh) {
-  int j = blockIdx.x * blockDim.x + threadIdx.x;
-  int k0 = blockIdx.y * kchunk;
+__device__ __forceinline__ void col_exp_kernel_body(int K, int N, GenPtrs g, int r0, int c0, int kchunk,
+                               int* __restrict__ eh, int bx_, int by_) {
+  int j = bx_ * blockDim.x + threadIdx.x;
+  int k0 = by_ * kchunk;
   if (j >= N) return;
   int k1 = min(K, k0 + kchunk);
   double mx = 0.0;
@@ -170,15 +181,21 @@ __global__ void col_exp_kernel(int K, int N, GenPtrs g, int r0, int c0, int kchu
   if (mx > 0.0) atomicMax(&eh[j], exponent_of(mx));
 }
 
+template <int KIND>
+__global__ void col_exp_kernel(int K, int N, GenPtrs g, int r0, int c0, int kchunk,
+                               int* __restrict__ eh) {
+  col_exp_kernel_body<KIND>(K, N, g, r0, c0, kchunk, eh, blockIdx.x, blockIdx.y);
+}
+
 // Residues of A: reads a 64 x 64 tile of column-major A, writes t row-major
 // int8 planes Ares[l][i * kp + k] (zero for k >= K; kp multiple of 128).
-__global__ __launch_bounds__(256) void split_a_kernel(int M, int K, const double* __restrict__ A,
+__device__ __forceinline__ void split_a_kernel_body(int M, int K, const double* __restrict__ A,
                                                       int lda, const int* __restrict__ ea,
                                                       int abits, CrtParams cp,
                                                       signed char* __restrict__ res, int kp,
-                                                      size_t plane) {
+                                                      size_t plane, int bx_, int by_) {
   __shared__ double tile[64][65];
-  const int i0 = blockIdx.x * 64, k0 = blockIdx.y * 64;
+  const int i0 = bx_ * 64, k0 = by_ * 64;
   const int t = threadIdx.x;
 #pragma unroll
   for (int q = 0; q < 16; ++q) {
@@ -214,16 +231,24 @@ __global__ __launch_bounds__(256) void split_a_kernel(int M, int K, const double
   }
 }
 
+__global__ __launch_bounds__(256) void split_a_kernel(int M, int K, const double* __restrict__ A,
+                                                      int lda, const int* __restrict__ ea,
+                                                      int abits, CrtParams cp,
+                                                      signed char* __restrict__ res, int kp,
+                                                      size_t plane) {
+  split_a_kernel_body(M, K, A, lda, ea, abits, cp, res, kp, plane, blockIdx.x, blockIdx.y);
+}
+
 // Residues of H: column-major int8 planes Hres[l][j * kp + k] (k contiguous),
 // entries generated from the generators.  Thread -> (column, 16 consecutive k).
 template <int KIND>
-__global__ __launch_bounds__(256) void split_h_kernel(int K, int N, GenPtrs g, int r0, int c0,
+__device__ __forceinline__ void split_h_kernel_body(int K, int N, GenPtrs g, int r0, int c0,
                                                       const int* __restrict__ eh, int hbits,
                                                       CrtParams cp, signed char* __restrict__ res,
-                                                      int kp, size_t plane) {
+                                                      int kp, size_t plane, int bx_, int by_) {
   const int t = threadIdx.x;
-  const int j = blockIdx.x * 16 + t / 16;
-  const int kb = blockIdx.y * 256 + (t % 16) * 16;
+  const int j = bx_ * 16 + t / 16;
+  const int kb = by_ * 256 + (t % 16) * 16;
   if (j >= N || kb >= kp) return;
   const int sh = hbits - eh[j];
   double v[16];
@@ -251,6 +276,14 @@ __global__ __launch_bounds__(256) void split_h_kernel(int K, int N, GenPtrs g, i
   }
 }
 
+template <int KIND>
+__global__ __launch_bounds__(256) void split_h_kernel(int K, int N, GenPtrs g, int r0, int c0,
+                                                      const int* __restrict__ eh, int hbits,
+                                                      CrtParams cp, signed char* __restrict__ res,
+                                                      int kp, size_t plane) {
+  split_h_kernel_body<KIND>(K, N, g, r0, c0, eh, hbits, cp, res, kp, plane, blockIdx.x, blockIdx.y);
+}
+
 // ---------------------------------------------------------------------------
 // Window form for Toeplitz/Hankel.  With one power-of-two scale for the whole
 // block of H, H' is built from an integer generator, so its residues are
@@ -261,31 +294,46 @@ __global__ __launch_bounds__(256) void split_h_kernel(int K, int N, GenPtrs g, i
 // four v_alignbyte_b32 (the 16 bytes a lane supplies are contiguous in u).
 
 // max exponent of |g| over [lo, hi] -> *e
-__global__ void gen_range_exp_kernel(const double* __restrict__ g, long long lo, long long hi,
-                                     int* __restrict__ e) {
-  long long i = lo + (long long)blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void gen_range_exp_kernel_body(const double* __restrict__ g, long long lo, long long hi,
+                                     int* __restrict__ e, int bx_, int by_) {
+  long long i = lo + (long long)bx_ * blockDim.x + threadIdx.x;
   double v = (i <= hi) ? fabs(g[i]) : 0.0;
   // wave-level max, then one atomic per wave
   for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o));
   if ((threadIdx.x & 63) == 0 && v > 0.0) atomicMax(e, exponent_of(v));
 }
 
-__global__ void fill_from_kernel(int* __restrict__ dst, int n, const int* __restrict__ src) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void gen_range_exp_kernel(const double* __restrict__ g, long long lo, long long hi,
+                                     int* __restrict__ e) {
+  gen_range_exp_kernel_body(g, lo, hi, e, blockIdx.x, blockIdx.y);
+}
+
+__device__ __forceinline__ void fill_from_kernel_body(int* __restrict__ dst, int n, const int* __restrict__ src, int bx_, int by_) {
+  int i = bx_ * blockDim.x + threadIdx.x;
   if (i < n) dst[i] = *src;
+}
+
+__global__ void fill_from_kernel(int* __restrict__ dst, int n, const int* __restrict__ src) {
+  fill_from_kernel_body(dst, n, src, blockIdx.x, blockIdx.y);
 }
 
 // Residue planes of the scaled generator: R_l[u] = sym(rint(g[src(u)] 2^sh)) mod m_l,
 // src(u) = hi - u when reversed (Toeplitz) else lo + u; zero for u >= L.
-__global__ void gen_residue_kernel(const double* __restrict__ g, long long lo, long long hi,
+__device__ __forceinline__ void gen_residue_kernel_body(const double* __restrict__ g, long long lo, long long hi,
                                    int reversed, const int* __restrict__ e, int hbits, CrtParams cp,
-                                   signed char* __restrict__ res, size_t gplane, int lpad) {
-  int u = blockIdx.x * blockDim.x + threadIdx.x;
+                                   signed char* __restrict__ res, size_t gplane, int lpad, int bx_, int by_) {
+  int u = bx_ * blockDim.x + threadIdx.x;
   if (u >= lpad) return;
   long long L = hi - lo + 1;
   double v = 0.0;
   if (u < L) v = rint(ldexp(g[reversed ? hi - u : lo + u], hbits - *e));
   for (int l = 0; l < cp.t; ++l) res[l * gplane + u] = (signed char)sym_residue(v, cp.m[l], cp.rm[l]);
+}
+
+__global__ void gen_residue_kernel(const double* __restrict__ g, long long lo, long long hi,
+                                   int reversed, const int* __restrict__ e, int hbits, CrtParams cp,
+                                   signed char* __restrict__ res, size_t gplane, int lpad) {
+  gen_residue_kernel_body(g, lo, hi, reversed, e, hbits, cp, res, gplane, lpad, blockIdx.x, blockIdx.y);
 }
 
 // ---------------------------------------------------------------------------
@@ -305,11 +353,11 @@ __global__ void gen_residue_kernel(const double* __restrict__ g, long long lo, l
 // D lane l element r holds D[4*(l/16)+r][l%16].
 enum { BMODE_DENSE = 0, BMODE_TOE = 1, BMODE_HANK = 2 };
 
-template <int BMODE, int BM, int BN, int BK, int WGM, int WGN, int MINB, int KMAX>
-__global__ __launch_bounds__(WGM* WGN * 64, MINB) void gemm_i8_kernel(
+template <int BMODE, int BM, int BN, int BK, int WGM, int WGN, int KMAX>
+__device__ __forceinline__ void gemm_i8_body(
     int M, int N, int kp, const signed char* __restrict__ Ares, const signed char* __restrict__ Bres,
-    size_t aplane, size_t bplane, int lpad, CrtParams cp, unsigned char* __restrict__ out,
-    size_t oplane) {
+    size_t aplane, size_t bplane, int lpad, const CrtParams& cp, unsigned char* __restrict__ out,
+    size_t oplane, int bmi, int bni, int l) {
   constexpr bool WIN = BMODE != BMODE_DENSE;
   constexpr int NT = WGM * WGN * 64;
   constexpr int WM = BM / WGM, WN = BN / WGN;
@@ -327,15 +375,10 @@ __global__ __launch_bounds__(WGM* WGN * 64, MINB) void gemm_i8_kernel(
   __shared__ __align__(16) signed char Bs[2][WIN ? 16 : BN * LDB];
   __shared__ __align__(16) unsigned int Win[WBYTES / 4];
 
-  const int l = blockIdx.y;
   const int m = cp.m[l];
   const signed char* Al = Ares + l * aplane;
   const signed char* Bl = Bres + l * bplane;
   unsigned char* Ol = out + l * oplane;
-
-  const int nbn = (N + BN - 1) / BN, nbm = (M + BM - 1) / BM;
-  int bmi, bni;
-  patch_remap<4, 8>(blockIdx.x, nbm, nbn, bmi, bni);
   const int m0 = bmi * BM, n0 = bni * BN;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm0 = (wave / WGN) * WM, wn0 = (wave % WGN) * WN;
@@ -512,6 +555,18 @@ __global__ __launch_bounds__(WGM* WGN * 64, MINB) void gemm_i8_kernel(
 }
 
 
+template <int BMODE, int BM, int BN, int BK, int WGM, int WGN, int MINB, int KMAX>
+__global__ __launch_bounds__(WGM* WGN * 64, MINB) void gemm_i8_kernel(
+    int M, int N, int kp, const signed char* __restrict__ Ares, const signed char* __restrict__ Bres,
+    size_t aplane, size_t bplane, int lpad, CrtParams cp, unsigned char* __restrict__ out,
+    size_t oplane) {
+  const int nbn = (N + BN - 1) / BN, nbm = (M + BM - 1) / BM;
+  int bmi, bni;
+  patch_remap<4, 8>(blockIdx.x, nbm, nbn, bmi, bni);
+  gemm_i8_body<BMODE, BM, BN, BK, WGM, WGN, KMAX>(M, N, kp, Ares, Bres, aplane, bplane, lpad, cp,
+                                                  out, oplane, bmi, bni, blockIdx.y);
+}
+
 // The error-free transformations below rely on every rounding being the one
 // written: FMA contraction (hipcc's default) would fuse a product into an
 // addition and break them, so it is off in these functions.
@@ -561,14 +616,13 @@ __device__ __forceinline__ double crt_garner(const unsigned char* __restrict__ r
 }
 
 template <int T>
-__global__ __launch_bounds__(256) void crt_kernel(int M, int N, const unsigned char* __restrict__ res,
-                                                  size_t oplane, CrtParams cp,
-                                                  const int* __restrict__ ea,
-                                                  const int* __restrict__ eh, int abits, int hbits,
-                                                  double alpha, double beta, double* __restrict__ C,
-                                                  int ldc) {
+__device__ __forceinline__ void crt_body(int M, int N, const unsigned char* __restrict__ res,
+                                         size_t oplane, const CrtParams& cp,
+                                         const int* __restrict__ ea, const int* __restrict__ eh,
+                                         int abits, int hbits, double alpha, double beta,
+                                         double* __restrict__ C, int ldc, int bx) {
 #pragma clang fp contract(off)
-  size_t off = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  size_t off = (size_t)bx * blockDim.x + threadIdx.x;
   if (off >= (size_t)M * N) return;
   int i = (int)(off % M), j = (int)(off / M);
   double shi = 0.0, slo = 0.0;
@@ -601,6 +655,16 @@ __global__ __launch_bounds__(256) void crt_kernel(int M, int N, const unsigned c
   double* p = C + (size_t)j * ldc + i;
   if (beta != 0.0) v += beta * (*p);
   *p = v;
+}
+
+template <int T>
+__global__ __launch_bounds__(256) void crt_kernel(int M, int N, const unsigned char* __restrict__ res,
+                                                  size_t oplane, CrtParams cp,
+                                                  const int* __restrict__ ea,
+                                                  const int* __restrict__ eh, int abits, int hbits,
+                                                  double alpha, double beta, double* __restrict__ C,
+                                                  int ldc) {
+  crt_body<T>(M, N, res, oplane, cp, ea, eh, abits, hbits, alpha, beta, C, ldc, blockIdx.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -1020,6 +1084,295 @@ static int launch_right_split_bf16(int kind, int s, int M, int N, int K, double 
     case KIND_CAUCHYEIG: return run_split_bf16<KIND_CAUCHYEIG>(s, M, N, K, alpha, A, lda, g, r0, c0, beta, C, ldc, st);
   }
   return -1;
+}
+
+// ---------------------------------------------------------------------------
+// Grouped int8 modular pipeline for a batch of problems: every phase is one
+// launch over the work units of all problems (a unit is one workgroup of the
+// single-problem kernel); a workgroup finds its problem by binary search over
+// the phase's unit prefix sums.  All problems use the same moduli (the batch's
+// CrtParams); each keeps its own power-of-two scales and abits.
+enum {
+  PH_FILL = 0, PH_ROWEXP, PH_COLEXP, PH_EHFILL, PH_SPLITA, PH_SPLITB,
+  PH_GEMM_D, PH_GEMM_T, PH_GEMM_H, PH_CRT, NPH
+};
+
+struct SplitProblem {
+  int kind, bmode, M, N, K, kp, lpad, abits;
+  double alpha, beta;
+  const double* A;
+  int lda;
+  GenPtrs g;
+  int r0, c0;
+  double* C;
+  int ldc;
+  long long glo, ghi;
+  int *ea, *eh, *e1;
+  signed char *ares, *bres;
+  unsigned char* ores;
+  size_t aplane, bplane, oplane;
+  int gx[NPH];  // x-extent of the phase's unit grid
+  int ub[NPH];  // first unit of this problem in the phase
+};
+
+__device__ __forceinline__ int find_split_prob(const SplitProblem* P, int np, int ph, int u) {
+  int lo = 0, hi = np - 1;
+  while (lo < hi) {
+    int mid = (lo + hi + 1) >> 1;
+    if (P[mid].ub[ph] <= u) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
+__global__ __launch_bounds__(256) void gsplit_phase_kernel(const SplitProblem* __restrict__ P,
+                                                           int np, int ph, CrtParams cp) {
+  const int u = blockIdx.x;
+  const SplitProblem& q = P[find_split_prob(P, np, ph, u)];
+  const int lu = u - q.ub[ph];
+  const int bx = lu % q.gx[ph], by = lu / q.gx[ph];
+  const bool dense = q.bmode == BMODE_DENSE;
+  switch (ph) {
+    case PH_FILL: {
+      int na = (q.M + 255) / 256, nh = (q.N + 255) / 256;
+      if (lu < na) fill_int_kernel_body(q.ea, q.M, kNoExp, lu, 0);
+      else if (lu < na + nh) fill_int_kernel_body(q.eh, q.N, kNoExp, lu - na, 0);
+      else fill_int_kernel_body(q.e1, 1, kNoExp, 0, 0);
+      break;
+    }
+    case PH_ROWEXP:
+      row_exp_kernel_body(q.M, q.K, q.A, q.lda, 256, q.ea, bx, by);
+      break;
+    case PH_COLEXP:
+      if (!dense) {
+        gen_range_exp_kernel_body(q.g.p0, q.glo, q.ghi, q.e1, bx, 0);
+      } else if (q.kind == KIND_CAUCHY) {
+        col_exp_kernel_body<KIND_CAUCHY>(q.K, q.N, q.g, q.r0, q.c0, 256, q.eh, bx, by);
+      } else {
+        col_exp_kernel_body<KIND_CAUCHYEIG>(q.K, q.N, q.g, q.r0, q.c0, 256, q.eh, bx, by);
+      }
+      break;
+    case PH_EHFILL:
+      fill_from_kernel_body(q.eh, q.N, q.e1, bx, 0);
+      break;
+    case PH_SPLITA:
+      split_a_kernel_body(q.M, q.K, q.A, q.lda, q.ea, q.abits, cp, q.ares, q.kp, q.aplane, bx, by);
+      break;
+    case PH_SPLITB:
+      if (!dense) {
+        gen_residue_kernel_body(q.g.p0, q.glo, q.ghi, q.bmode == BMODE_TOE ? 1 : 0, q.e1, q.abits,
+                                cp, q.bres, q.bplane, q.lpad, bx, 0);
+      } else if (q.kind == KIND_CAUCHY) {
+        split_h_kernel_body<KIND_CAUCHY>(q.K, q.N, q.g, q.r0, q.c0, q.eh, q.abits, cp, q.bres,
+                                         q.kp, q.bplane, bx, by);
+      } else {
+        split_h_kernel_body<KIND_CAUCHYEIG>(q.K, q.N, q.g, q.r0, q.c0, q.eh, q.abits, cp, q.bres,
+                                            q.kp, q.bplane, bx, by);
+      }
+      break;
+  }
+}
+
+constexpr int kBatchKmax = 16384;  // longest contraction of a generator-window problem in a batch
+
+template <int BMODE>
+__global__ __launch_bounds__(256, 2) void gsplit_gemm_kernel(const SplitProblem* __restrict__ P,
+                                                             int np, int ph, CrtParams cp) {
+  const int u = blockIdx.x;
+  const SplitProblem& q = P[find_split_prob(P, np, ph, u)];
+  const int lu = u - q.ub[ph];
+  const int ntn = (q.N + 127) / 128, ntm = (q.M + 127) / 128;
+  const int l = lu / (ntm * ntn), r = lu % (ntm * ntn);
+  gemm_i8_body<BMODE, 128, 128, 128, 2, 2, kBatchKmax>(q.M, q.N, q.kp, q.ares, q.bres, q.aplane,
+                                                      q.bplane, q.lpad, cp, q.ores, q.oplane,
+                                                      r / ntn, r % ntn, l);
+}
+
+template <int T>
+__global__ __launch_bounds__(256) void gsplit_crt_kernel(const SplitProblem* __restrict__ P, int np,
+                                                         CrtParams cp) {
+  const int u = blockIdx.x;
+  const SplitProblem& q = P[find_split_prob(P, np, PH_CRT, u)];
+  crt_body<T>(q.M, q.N, q.ores, q.oplane, cp, q.ea, q.eh, q.abits, q.abits, q.alpha, q.beta, q.C,
+              q.ldc, u - q.ub[PH_CRT]);
+}
+
+// Host plan of a grouped split batch (table, unit totals, workspace size).
+struct SplitBatch {
+  int t = 0;
+  CrtParams cp{};
+  std::vector<SplitProblem> tab;
+  int total[NPH] = {0};
+  size_t ws_bytes = 0;
+};
+
+// Lays out the batch in a workspace starting at `ws` (nullptr: sizing only).
+// Problems must be validated; empty ones are skipped.
+void split_batch_layout(const BatchProblem* probs, int nprob, int t, char* ws, SplitBatch& sb) {
+  auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
+  sb.t = t;
+  sb.cp = make_crt(t);
+  sb.tab.clear();
+  for (int i = 0; i < NPH; ++i) sb.total[i] = 0;
+  size_t off = 0;
+  std::vector<int> order(nprob);
+  for (int i = 0; i < nprob; ++i) order[i] = i;
+  std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return probs[x].k > probs[y].k; });
+  for (int oi = 0; oi < nprob; ++oi) {
+    const BatchProblem& b = probs[order[oi]];
+    if (b.m == 0 || b.n == 0 || b.k == 0) continue;
+    SplitProblem q{};
+    q.kind = b.kind;
+    q.bmode = (b.kind == KIND_TOEPLITZ) ? BMODE_TOE : (b.kind == KIND_HANKEL) ? BMODE_HANK : BMODE_DENSE;
+    q.M = b.m; q.N = b.n; q.K = b.k;
+    q.kp = ((b.k + 255) / 256) * 256;
+    if (q.bmode != BMODE_DENSE && q.kp > kBatchKmax) q.bmode = BMODE_DENSE;
+    q.abits = choose_abits(t, b.k);
+    q.alpha = b.alpha; q.beta = b.beta; q.A = b.A; q.lda = b.lda; q.g = b.g;
+    q.r0 = b.r0; q.c0 = b.c0; q.C = b.C; q.ldc = b.ldc;
+    const bool toe = b.kind == KIND_TOEPLITZ;
+    q.glo = toe ? (long long)b.g.korder - 1 + b.c0 - (b.r0 + b.k - 1) : (long long)b.r0 + b.c0;
+    q.ghi = toe ? (long long)b.g.korder - 1 + (b.c0 + b.n - 1) - b.r0
+                : (long long)b.r0 + b.k - 1 + b.c0 + b.n - 1;
+    long long L = q.ghi - q.glo + 1;
+    q.lpad = (int)((L + 15) / 16 * 16);
+    q.aplane = (size_t)q.M * q.kp;
+    q.bplane = (q.bmode == BMODE_DENSE) ? (size_t)q.kp * q.N : (size_t)q.lpad;
+    q.oplane = (size_t)q.M * q.N;
+    auto take = [&](size_t bytes) {
+      char* p = ws ? ws + off : nullptr;
+      off += al(bytes);
+      return p;
+    };
+    q.ea = (int*)take(sizeof(int) * q.M);
+    q.eh = (int*)take(sizeof(int) * q.N);
+    q.e1 = (int*)take(sizeof(int));
+    q.ares = (signed char*)take(t * q.aplane);
+    q.bres = (signed char*)take(t * q.bplane);
+    q.ores = (unsigned char*)take(t * q.oplane);
+    const bool dense = q.bmode == BMODE_DENSE;
+    const int kc = (q.K + 255) / 256;
+    int gx[NPH], gy[NPH];
+    for (int ph = 0; ph < NPH; ++ph) gx[ph] = 1, gy[ph] = 0;
+    gx[PH_FILL] = (q.M + 255) / 256 + (q.N + 255) / 256 + 1; gy[PH_FILL] = 1;
+    gx[PH_ROWEXP] = (q.M + 255) / 256; gy[PH_ROWEXP] = kc;
+    if (dense) { gx[PH_COLEXP] = (q.N + 255) / 256; gy[PH_COLEXP] = kc; }
+    else { gx[PH_COLEXP] = (int)((L + 255) / 256); gy[PH_COLEXP] = 1; }
+    if (!dense) { gx[PH_EHFILL] = (q.N + 255) / 256; gy[PH_EHFILL] = 1; }
+    gx[PH_SPLITA] = (q.M + 63) / 64; gy[PH_SPLITA] = q.kp / 64;
+    if (dense) { gx[PH_SPLITB] = (q.N + 15) / 16; gy[PH_SPLITB] = q.kp / 256; }
+    else { gx[PH_SPLITB] = (q.lpad + 255) / 256; gy[PH_SPLITB] = 1; }
+    int gph = dense ? PH_GEMM_D : (q.bmode == BMODE_TOE ? PH_GEMM_T : PH_GEMM_H);
+    gx[gph] = ((q.M + 127) / 128) * ((q.N + 127) / 128) * t; gy[gph] = 1;
+    gx[PH_CRT] = (int)(((size_t)q.M * q.N + 255) / 256); gy[PH_CRT] = 1;
+    for (int ph = 0; ph < NPH; ++ph) {
+      q.gx[ph] = gx[ph];
+      q.ub[ph] = sb.total[ph];
+      sb.total[ph] += gx[ph] * gy[ph];
+    }
+    sb.tab.push_back(q);
+  }
+  sb.ws_bytes = off;
+}
+
+// Enqueues the grouped pipeline; dtab is the device copy of sb.tab.
+int split_batch_enqueue(const SplitBatch& sb, const SplitProblem* dtab, hipStream_t s, int* nodes) {
+  const int np = (int)sb.tab.size();
+  int n = 0;
+  if (np == 0) {
+    if (nodes) *nodes = 0;
+    return 0;
+  }
+  for (int ph : {PH_FILL, PH_ROWEXP, PH_COLEXP, PH_EHFILL, PH_SPLITA, PH_SPLITB}) {
+    if (sb.total[ph] == 0) continue;
+    hipLaunchKernelGGL(gsplit_phase_kernel, dim3(sb.total[ph]), dim3(256), 0, s, dtab, np, ph, sb.cp);
+    ++n;
+  }
+  if (sb.total[PH_GEMM_D]) {
+    hipLaunchKernelGGL(gsplit_gemm_kernel<BMODE_DENSE>, dim3(sb.total[PH_GEMM_D]), dim3(256), 0, s,
+                       dtab, np, (int)PH_GEMM_D, sb.cp);
+    ++n;
+  }
+  if (sb.total[PH_GEMM_T]) {
+    hipLaunchKernelGGL(gsplit_gemm_kernel<BMODE_TOE>, dim3(sb.total[PH_GEMM_T]), dim3(256), 0, s,
+                       dtab, np, (int)PH_GEMM_T, sb.cp);
+    ++n;
+  }
+  if (sb.total[PH_GEMM_H]) {
+    hipLaunchKernelGGL(gsplit_gemm_kernel<BMODE_HANK>, dim3(sb.total[PH_GEMM_H]), dim3(256), 0, s,
+                       dtab, np, (int)PH_GEMM_H, sb.cp);
+    ++n;
+  }
+  dim3 cg(sb.total[PH_CRT]);
+#define PSMMA_GCRT_CASE(TT)                                                              \
+  case TT:                                                                               \
+    hipLaunchKernelGGL(gsplit_crt_kernel<TT>, cg, dim3(256), 0, s, dtab, np, sb.cp);     \
+    break;
+  switch (sb.t) {
+    PSMMA_GCRT_CASE(2) PSMMA_GCRT_CASE(3) PSMMA_GCRT_CASE(4) PSMMA_GCRT_CASE(5) PSMMA_GCRT_CASE(6)
+    PSMMA_GCRT_CASE(7) PSMMA_GCRT_CASE(8) PSMMA_GCRT_CASE(9) PSMMA_GCRT_CASE(10) PSMMA_GCRT_CASE(11)
+    PSMMA_GCRT_CASE(12) PSMMA_GCRT_CASE(13) PSMMA_GCRT_CASE(14) PSMMA_GCRT_CASE(15) PSMMA_GCRT_CASE(16)
+    default: return -16;
+  }
+#undef PSMMA_GCRT_CASE
+  ++n;
+  if (nodes) *nodes = n;
+  hipError_t e = hipGetLastError();
+  return e == hipSuccess ? 0 : 1000 + (int)e;
+}
+
+// Fewest moduli that keep `bits` bits per operand for every problem of the batch.
+int split_batch_moduli(const BatchProblem* probs, int nprob, int bits) {
+  int kmax = 1;
+  for (int i = 0; i < nprob; ++i) kmax = std::max(kmax, probs[i].k);
+  for (int t = 2; t <= kMaxModuli; ++t)
+    if (choose_abits(t, kmax) >= bits) return t;
+  return kMaxModuli;
+}
+
+// Opaque grouped split batch for the batched C ABI: create (sizes the layout),
+// bind (lays it out in a caller-owned workspace and uploads the table),
+// enqueue (all phases on a stream; safe to capture), destroy.
+struct GSplit {
+  SplitBatch sb;
+  std::vector<BatchProblem> probs;
+  SplitProblem* dtab = nullptr;
+};
+
+void* gsplit_create(const BatchProblem* probs, int nprob, int t, size_t* ws_bytes) {
+  GSplit* h = new GSplit;
+  h->probs.assign(probs, probs + nprob);
+  split_batch_layout(h->probs.data(), nprob, t, nullptr, h->sb);
+  if (ws_bytes) *ws_bytes = h->sb.ws_bytes;
+  return h;
+}
+
+int gsplit_bind(void* hv, void* ws) {
+  GSplit* h = (GSplit*)hv;
+  split_batch_layout(h->probs.data(), (int)h->probs.size(), h->sb.t, (char*)ws, h->sb);
+  if (h->sb.tab.empty()) return 0;
+  size_t bytes = sizeof(SplitProblem) * h->sb.tab.size();
+  if (hipMalloc(&h->dtab, bytes) != hipSuccess) return 1002;
+  hipError_t e = hipMemcpy(h->dtab, h->sb.tab.data(), bytes, hipMemcpyHostToDevice);
+  return e == hipSuccess ? 0 : 1000 + (int)e;
+}
+
+int gsplit_enqueue(void* hv, hipStream_t s, int* nodes) {
+  GSplit* h = (GSplit*)hv;
+  return split_batch_enqueue(h->sb, h->dtab, s, nodes);
+}
+
+void gsplit_destroy(void* hv) {
+  GSplit* h = (GSplit*)hv;
+  if (!h) return;
+  if (h->dtab) {
+    hipDeviceSynchronize();
+    hipFree(h->dtab);
+  }
+  delete h;
+}
+
+int gsplit_moduli(const BatchProblem* probs, int nprob, int bits) {
+  return split_batch_moduli(probs, nprob, bits);
 }
 
 // Workspace bytes one split-precision call needs (for callers that must
