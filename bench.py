@@ -154,6 +154,68 @@ def other_configs(seed, torch, api, inputs):
                              "rel_err_vs_fft": rel(dc, ref), "hipfft_ms": round(msf, 4)}
     del db, dc, fft_in, ref
     out["batched_512"] = batched_config(seed, torch, api, inputs)
+    torch.cuda.empty_cache()
+    out["eig_merge_n16384"] = eig_merge_config(seed, torch, api, inputs)
+    return out
+
+
+def eig_merge_config(seed, torch, api, inputs, n=16384):
+    """Divide-and-conquer eigenvector merge: X = Q * C with C(i,j) = zhat_i v_j / (d_i - lam_j),
+    generators from the GPU secular solver (roots, Loewner z, column normalisers), Q from the QR
+    of a Gaussian matrix; plain FP64 and split forms, error against the FP64 oracle (explicit C
+    and rocBLAS DGEMM) and orthogonality ||I - X^T X||."""
+    d_np, z_np, _ = inputs.config3_dz(n, seed)
+    td = torch.from_numpy(d_np).cuda()
+    tz = torch.from_numpy(z_np).cuda()
+    gens = api.secular_dev(td, tz, 1.0, loewner=True)
+    torch.cuda.synchronize()
+    ms_sec = ev_time(lambda: api.secular_dev(td, tz, 1.0, loewner=True), 2, torch)
+    difl, difr, u, v = gens
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(seed)
+    q, _ = torch.linalg.qr(torch.randn((n, n), dtype=torch.float64, device="cuda", generator=gen))
+    tq = q if q.stride(0) == 1 else q.t()  # a column-major view of an orthogonal matrix
+    del q
+    torch.cuda.empty_cache()
+    x = api.colmajor_empty(n, n)
+    flops = 2.0 * n ** 3
+    out = {"n": n, "secular_ms": round(ms_sec, 3)}
+
+    def ortho(xm):
+        g = xm.t() @ xm
+        g.diagonal().sub_(1.0)
+        r = (g.abs().max().item(), g.norm().item())
+        del g
+        return r
+
+    ms64 = ev_time(lambda: api.cauchyeig_dev(1.0, tq, td, u, v, difl, difr, n, x), 3, torch)
+    idx = torch.arange(n, device="cuda")
+    r, c = idx[:, None], idx[None, :]
+    cp1 = torch.clamp(c + 1, max=n - 1)
+    den = torch.where(r <= c, (td[r] - td[c]) - difl[c], (td[r] - td[cp1]) - difr[c])
+    cm = (u[r] * v[c]) / den
+    del den, r, c, cp1
+    ref = tq @ cm
+    del cm
+    torch.cuda.empty_cache()
+    e64 = rel(x, ref)
+    om, of = ortho(x)
+    out.update({"fp64_ms": round(ms64, 3), "fp64_tflops": round(flops / ms64 / 1e9, 2),
+                "fp64_rel_err": e64, "fp64_orth_max": om, "fp64_orth_fro": of})
+    best = None
+    for t in (13, 14):
+        ms = ev_time(lambda: api.cauchyeig_dev(1.0, tq, td, u, v, difl, difr, n, x,
+                                               prec=api.PREC_SPLIT_I8, slices=t), 3, torch)
+        err = rel(x, ref)
+        rec = {"slices": t, "ms": round(ms, 3), "rel_err": err}
+        if err <= 1e-12 and (best is None or ms < best["ms"]):
+            best = rec
+            best["orth_max"], best["orth_fro"] = ortho(x)
+    if best is not None:
+        best["speedup_vs_fp64"] = round(ms64 / best["ms"], 3)
+    out["split_i8"] = best
+    del x, ref, tq
+    torch.cuda.empty_cache()
     return out
 
 

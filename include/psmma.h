@@ -216,6 +216,23 @@ int psmma_batched_plan_launch(void* plan, void* stream);
 int psmma_batched_plan_info(void* plan, int* tile_cfg, int* nodes);
 int psmma_batched_plan_destroy(void* plan);
 
+/* ===========================================================================
+ * 4. Generators of the eigen-Cauchy operand (device pointers).
+ *
+ * Replaces the merge step that produces psmma_cauchyeig's inputs
+ * (SRC/BasicMM.f90:538-590: a DLASD4 call per root, the Loewner recomputation
+ * of Z, DNRM2 column scaling).  d (n, strictly increasing), z (n, no zero
+ * entry), rho > 0 describe D + rho z z^T.  Outputs (device, length n):
+ *   difl[j] = lambda_j - d_j > 0, difr[j] = lambda_j - d_{j+1} < 0 (last root:
+ *   d_{n+1} = d_n + rho z^T z), found relative to the nearer pole;
+ *   zhat (may be NULL): Loewner's z, for which the lambda_j are exact
+ *   eigenvalues (sign of z); v (may be NULL): column normalisers
+ *   v_j = 1 / || u ./ (d - lambda_j) || with u = zhat if given, else z.
+ * Then psmma_cauchyeig_dev(..., fpole = d, u, v, difl, difr, ...) multiplies
+ * by the orthonormal eigenvector matrix.  Returns 0 or -(argument number). */
+int psmma_secular_dev(int n, const double* d, const double* z, double rho, double* difl,
+                      double* difr, double* zhat, double* v, void* stream);
+
 /* Library version string. */
 const char* psmma_version(void);
 

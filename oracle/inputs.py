@@ -92,26 +92,64 @@ def secular_roots(d: np.ndarray, z: np.ndarray, rho: float = 1.0, chunk: int = 1
     return difl, difr
 
 
+def _lam_minus_d(d, difl, difr, rows, cols):
+    """lambda_c - d_r for index arrays rows (column vector) and cols (row vector), from the
+    shifted roots: (d_c - d_r) + DIFL_c for r <= c, (d_{c+1} - d_r) + DIFR_c otherwise."""
+    n = len(d)
+    cp1 = np.minimum(cols + 1, n - 1)
+    return np.where(rows <= cols, (d[cols] - d[rows]) + difl[cols],
+                    (d[cp1] - d[rows]) + difr[cols])
+
+
+def loewner_z(d, z, difl, difr, rho: float = 1.0, chunk: int = 512):
+    """Loewner's z for computed roots (the z that makes lambda exact eigenvalues of
+    D + rho z z^T): zhat_i^2 = prod_j (lam_j - d_i) / prod_{j != i} (d_j - d_i) / rho, with
+    the sign of z_i; factors paired as ratios (lam_j - d_i) / (d_j - d_i)."""
+    d = np.asarray(d, np.float64)
+    n = len(d)
+    out = np.empty(n)
+    cols = np.arange(n)[None, :]
+    for i0 in range(0, n, chunk):
+        rows = np.arange(i0, min(n, i0 + chunk))[:, None]
+        num = _lam_minus_d(d, difl, difr, rows, cols)
+        den = d[cols] - d[rows]
+        ratio = np.where(rows == cols, num, num / np.where(rows == cols, 1.0, den))
+        out[i0:i0 + rows.shape[0]] = np.prod(ratio, axis=1)
+    return np.copysign(np.sqrt(np.abs(out) / rho), z)
+
+
+def column_normalisers(d, u, difl, difr, chunk: int = 1024):
+    """v_c = 1 / || u ./ (d - lam_c) || with the differences the tile generator forms."""
+    d = np.asarray(d, np.float64)
+    n = len(d)
+    v = np.empty(n)
+    rows = np.arange(n)[:, None]
+    for j0 in range(0, n, chunk):
+        cols = np.arange(j0, min(n, j0 + chunk))[None, :]
+        den = -_lam_minus_d(d, difl, difr, rows, cols)
+        v[j0:j0 + cols.shape[1]] = 1.0 / np.linalg.norm(u[:, None] / den, axis=0)
+    return v
+
+
+def config3_dz(n: int = 16384, seed: int = 0):
+    """The first draws of config3 (same seed, same order): sorted poles d and unit z."""
+    rng = np.random.default_rng(seed)
+    d = np.sort(rng.random(n))
+    z = rng.standard_normal(n)
+    z /= np.linalg.norm(z)
+    return d, z, rng
+
+
 def config3(n: int = 16384, seed: int = 0, with_q: bool = True):
     """Eigenvector merge: d sorted U[0,1), z ~ N(0,1) normalised, rho = 1.
 
     Returns dict with f (= d, the poles), u (= z), v (column normalisers),
     difl, difr and optionally the orthogonal q (QR of an n x n Gaussian)."""
-    rng = np.random.default_rng(seed)
-    d = np.sort(rng.random(n))
-    z = rng.standard_normal(n)
-    z /= np.linalg.norm(z)
+    d, z, rng = config3_dz(n, seed)
     difl, difr = secular_roots(d, z, 1.0)
     # column normaliser: v_j = 1 / || z / (d - lam_j) ||, with d_i - lam_j formed
     # exactly as the tile generator does
-    v = np.empty(n)
-    for j0 in range(0, n, 1024):
-        js = np.arange(j0, min(n, j0 + 1024))
-        r = np.arange(n)[:, None]
-        c = js[None, :]
-        cp1 = np.minimum(c + 1, n - 1)
-        den = np.where(r <= c, (d[r] - d[c]) - difl[c], (d[r] - d[cp1]) - difr[c])
-        v[js] = 1.0 / np.linalg.norm(z[:, None] / den, axis=0)
+    v = column_normalisers(d, z, difl, difr)
     out = dict(f=d, u=z, v=v, difl=difl, difr=difr)
     if with_q:
         g = rng.standard_normal(n * n).reshape((n, n), order="F")

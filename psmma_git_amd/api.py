@@ -65,6 +65,7 @@ _SIGS = {
     "psmma_cauchyeig_dev": ([_I, _I, _I, _D, _P, _I, _P, _P, _P, _P, _P, _I, _I, _I, _D, _P, _I,
                              _I, _I, _P], _I),
     "pzmdft_dev": ([_I, _I, _I, _I, _D, _P, _I, _I, _I, _D, _P, _I, _I, _P], _I),
+    "psmma_secular_dev": ([_I, _P, _P, _D, _P, _P, _P, _P, _P], _I),
     "psmma_batched_dev": ([_I, _P, _I, _I, _P], _I),
     "psmma_batched_plan_create": ([_I, _P, _I, _I, ctypes.POINTER(ctypes.c_void_p)], _I),
     "psmma_batched_plan_launch": ([_P, _P], _I),
@@ -293,6 +294,22 @@ def dft_dev(alpha, b, nd, m, c, beta=0.0, r0=0, c0=0, prec=PREC_FP64, stream=Non
     pc, mc, nc, ldc = _colmajor(c)
     _check(lib().pzmdft_dev(m, n, k, nd, alpha, pb, ldb, r0, c0, beta, pc, ldc, prec,
                             _stream_ptr(stream)), "pzmdft_dev")
+
+
+def secular_dev(d, z, rho=1.0, loewner=True, stream=None):
+    """Generators of the eigen-Cauchy operand of D + rho z z^T (d strictly increasing,
+    float64 device tensors).  Returns (difl, difr, u, v): u is Loewner's z when
+    loewner is true, else z itself; v the column normalisers for that u."""
+    import torch
+    n = d.numel()
+    difl = torch.empty_like(d)
+    difr = torch.empty_like(d)
+    zhat = torch.empty_like(d) if loewner else None
+    v = torch.empty_like(d)
+    _check(lib().psmma_secular_dev(n, d.data_ptr(), z.data_ptr(), float(rho), difl.data_ptr(),
+                                   difr.data_ptr(), zhat.data_ptr() if loewner else None,
+                                   v.data_ptr(), _stream_ptr(stream)), "psmma_secular_dev")
+    return difl, difr, (zhat if loewner else z), v
 
 
 # ---------------------------------------------------------------------------

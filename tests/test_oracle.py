@@ -220,3 +220,31 @@ def test_cauchyeig_columns_are_orthonormal_eigenvectors(ora):
     lam = e["f"] + e["difl"]
     assert np.max(np.abs(m @ c - c * lam[None, :])) < 1e-12
     assert np.max(np.abs(c.T @ c - np.eye(n))) < 1e-10
+
+
+def test_loewner_z_makes_roots_exact_and_vectors_orthonormal(ora):
+    """Loewner's theorem: D + zhat zhat^T has exactly the computed roots as eigenvalues,
+    and the eigen-Cauchy columns built from zhat are orthonormal to rounding level."""
+    n = 160
+    e = inputs.config3(n, seed=6, with_q=False)
+    d, z, difl, difr = e["f"], e["u"], e["difl"], e["difr"]
+    zh = inputs.loewner_z(d, z, difl, difr)
+    assert np.all(np.sign(zh) == np.sign(z))
+    # the computed roots are accurate, so zhat is close to z
+    assert np.max(np.abs(zh - z)) < 1e-12
+    lam = d + difl
+    want = np.linalg.eigvalsh(np.diag(d) + np.outer(zh, zh))
+    assert np.max(np.abs(np.sort(lam) - want)) < 1e-14
+    v = inputs.column_normalisers(d, zh, difl, difr)
+    c = ora.cauchyeig_matrix(d, zh, v, difl, difr, 0, n, 0, n)
+    assert np.max(np.abs(c.T @ c - np.eye(n))) < 1e-13
+    # a planted sign error in one factor would break the product's magnitude
+    bad = inputs.loewner_z(d, z, difl, -difr)
+    assert not np.allclose(bad, zh, rtol=1e-6)
+
+
+def test_column_normalisers_match_explicit_norms(ora):
+    n = 90
+    e = inputs.config3(n, seed=7, with_q=False)
+    c = ora.cauchyeig_matrix(e["f"], e["u"], np.ones(n), e["difl"], e["difr"], 0, n, 0, n)
+    assert np.max(np.abs(e["v"] * np.linalg.norm(c, axis=0) - 1.0)) < 1e-14
