@@ -79,6 +79,8 @@ def _declare(L):
     L.ora_structured_gemm.restype = I
     L.ora_mdgeqp3.argtypes = [I, I, P, I, P, P, D, I, P]
     L.ora_mdgeqp3.restype = I
+    L.ora_compress_rrqr.argtypes = [I, I, P, I, P, D]
+    L.ora_compress_rrqr.restype = I
 
 
 def _f64(x):
@@ -225,6 +227,42 @@ def dgemm_c(alpha, a, b, beta=0.0, c=None) -> np.ndarray:
     lib().ora_dgemm(m, n, k, float(alpha), _ptr(a), max(1, m), _ptr(b), max(1, k), float(beta),
                     _ptr(out), max(1, m))
     return out
+
+
+# ---------------------------------------------------------------------------
+# Truncated pivoted QR (C restatement of MDGEQP3 / Compress_RRQR).
+
+RRQR_TOL = float(np.float32(1e-13))  # Compress_RRQR's TOL: the default-real literal 1.0E-13
+
+
+def mdgeqp3(a, tol: float = 0.0, nb: int = 32, jpvt=None):
+    """SRC/mdgeqp3.f:153-370.  Returns (factored A, jpvt (1-based), tau, rk_bound)."""
+    a = np.array(a, dtype=np.float64, order="F", copy=True)
+    m, n = a.shape
+    jp = np.zeros(n, np.int32) if jpvt is None else np.array(jpvt, np.int32)
+    tau = np.zeros(max(1, min(m, n)))
+    rk = ctypes.c_int(0)
+    st = lib().ora_mdgeqp3(m, n, _ptr(a), max(1, m), _ptr(jp), _ptr(tau), tol, nb,
+                           ctypes.byref(rk))
+    if st != 0:
+        raise RuntimeError(f"ora_mdgeqp3 returned {st}")
+    return a, jp, tau[:min(m, n)], rk.value
+
+
+def compress_rrqr(a, tol: float = RRQR_TOL):
+    """SRC/Auxil_2p5D.f90:1060-1189.  Returns (rk, Q (m x rk), R P^T (rk x n)); Q and
+    R P^T are None at full rank (the reference leaves H unwritten there)."""
+    a = np.array(a, dtype=np.float64, order="F", copy=True)
+    m, n = a.shape
+    h = np.zeros(max(1, 2 * m * n))
+    rk = lib().ora_compress_rrqr(m, n, _ptr(a), max(1, m), _ptr(h), tol)
+    if rk < 0:
+        raise RuntimeError("ora_compress_rrqr failed")
+    if rk == min(m, n):
+        return rk, None, None
+    q = h[:m * rk].reshape((m, rk), order="F").copy()
+    r = h[m * rk:m * rk + rk * n].reshape((rk, n), order="F").copy()
+    return rk, q, r
 
 
 def rel_fro(x, ref) -> float:

@@ -222,7 +222,8 @@ static void* rank_body(void* arg) {
       for (int i = 0; i < mp; ++i)
         aloc[(size_t)kk * mp + i] = p->a[(size_t)(ridx[kk] - 1) * p->lda + rows[i] - 1];
     int lowrank = ora_is_intersect(lc, cidx, lr, ridx);
-    if (lowrank && p->mode != ORA_MODE_DENSE) {
+    /* PSHANKEL_COMPUTE has no low-rank branch (SRC/pshankel_compute.f90) */
+    if (lowrank && p->mode != ORA_MODE_DENSE && p->kind != ORA_HANKEL) {
       int rk = -1;
       int st = ora_lowrank_tile(p, kbp, nq, ridx, cidx, hbuf, hcap, &rk);
       if (st < 0) {

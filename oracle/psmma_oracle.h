@@ -52,4 +52,6 @@ int ora_lowrank_tile(const ora_problem* p, int kbp, int nq, const int* ridx, con
 /* Truncated pivoted QR (SRC/mdgeqp3.f): see psmma_oracle_lowrank.c. */
 int ora_mdgeqp3(int m, int n, double* a, int lda, int* jpvt, double* tau, double tol, int nb,
                 int* rk);
+/* Compress_RRQR (SRC/Auxil_2p5D.f90:1060-1189): returns the rank; h = [Q | R P^T]. */
+int ora_compress_rrqr(int m, int n, double* a, int lda, double* h, double tol);
 #endif
