@@ -233,6 +233,25 @@ int psmma_batched_plan_destroy(void* plan);
 int psmma_secular_dev(int n, const double* d, const double* z, double rho, double* difl,
                       double* difr, double* zhat, double* v, void* stream);
 
+/* ===========================================================================
+ * 5. Batched truncated RRQR (device pointers).
+ *
+ * Compress_RRQR (SRC/Auxil_2p5D.f90:1060-1189, over MDGEQP3, SRC/mdgeqp3.f:153-370)
+ * for nblk column-major m x n blocks, m, n <= 256: block b is a + b*stride_a
+ * (leading dimension lda).  Column pivoting on the largest partial norm;
+ * rank[b] = number of leading |R(i,i)| > tol |R(0,0)|, with the reference's
+ * conventions: rank = min(m,n) at full rank (outputs left unwritten), and here
+ * also rank = -1 when kmax (1..32) or more columns stay above the tolerance
+ * (outputs left unwritten).  For 0 < rank < min(m,n):
+ *   q   + b*stride_q (ldq):  Q, m x rank, orthonormal columns;
+ *   rpt + b*stride_r (ldr):  R P^T, rank x n, so that block ~ Q * (R P^T).
+ * jpvt[b*n + j] = 1-based source column of pivot position j.  One workgroup
+ * per block; asynchronous on `stream`.  Returns 0 or -(argument number). */
+int psmma_rrqr_batched_dev(int nblk, int m, int n, const double* a, int lda, long long stride_a,
+                           double tol, int kmax, int* rank, int* jpvt, double* q, int ldq,
+                           long long stride_q, double* rpt, int ldr, long long stride_r,
+                           void* stream);
+
 /* Library version string. */
 const char* psmma_version(void);
 

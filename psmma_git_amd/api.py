@@ -66,6 +66,8 @@ _SIGS = {
                              _I, _I, _P], _I),
     "pzmdft_dev": ([_I, _I, _I, _I, _D, _P, _I, _I, _I, _D, _P, _I, _I, _P], _I),
     "psmma_secular_dev": ([_I, _P, _P, _D, _P, _P, _P, _P, _P], _I),
+    "psmma_rrqr_batched_dev": ([_I, _I, _I, _P, _I, ctypes.c_longlong, _D, _I, _P, _P, _P, _I,
+                                ctypes.c_longlong, _P, _I, ctypes.c_longlong, _P], _I),
     "psmma_batched_dev": ([_I, _P, _I, _I, _P], _I),
     "psmma_batched_plan_create": ([_I, _P, _I, _I, ctypes.POINTER(ctypes.c_void_p)], _I),
     "psmma_batched_plan_launch": ([_P, _P], _I),
@@ -310,6 +312,32 @@ def secular_dev(d, z, rho=1.0, loewner=True, stream=None):
                                    difr.data_ptr(), zhat.data_ptr() if loewner else None,
                                    v.data_ptr(), _stream_ptr(stream)), "psmma_secular_dev")
     return difl, difr, (zhat if loewner else z), v
+
+
+RRQR_KMAX = 32
+
+
+def rrqr_batched_dev(blocks, tol, kmax=RRQR_KMAX, stream=None):
+    """Batched truncated RRQR of the blocks of a (nblk, n, m) float64 device tensor (block b
+    is the column-major m x n matrix blocks[b].T).  Returns (rank, jpvt, q, rpt): rank
+    (nblk,) int32; jpvt (nblk, n) 1-based; q (nblk, kq, m) and rpt (nblk, n, kq) hold, as
+    column-major views q[b].T and rpt[b].T, Q (m x rank) and R P^T (rank x n) in their
+    leading columns / rows, kq = min(kmax, m, n)."""
+    import torch
+    nblk, n, m = blocks.shape
+    if not blocks.is_contiguous():
+        raise ValueError("blocks must be contiguous (nblk, n, m)")
+    kq = max(1, min(kmax, m, n))
+    dev = blocks.device
+    rank = torch.empty(nblk, dtype=torch.int32, device=dev)
+    jpvt = torch.empty((nblk, n), dtype=torch.int32, device=dev)
+    q = torch.zeros((nblk, kq, m), dtype=torch.float64, device=dev)
+    rpt = torch.zeros((nblk, n, kq), dtype=torch.float64, device=dev)
+    _check(lib().psmma_rrqr_batched_dev(nblk, m, n, blocks.data_ptr(), max(1, m), max(1, m) * n,
+                                        float(tol), kmax, rank.data_ptr(), jpvt.data_ptr(),
+                                        q.data_ptr(), max(1, m), kq * m, rpt.data_ptr(), kq,
+                                        kq * n, _stream_ptr(stream)), "psmma_rrqr_batched_dev")
+    return rank, jpvt, q, rpt
 
 
 # ---------------------------------------------------------------------------
