@@ -214,6 +214,19 @@ def eig_merge_config(seed, torch, api, inputs, n=16384):
     if best is not None:
         best["speedup_vs_fp64"] = round(ms64 / best["ms"], 3)
     out["split_i8"] = best
+    # compressed form: off-diagonal 256 x 256 blocks by the batched RRQR at tol 1e-12
+    stats = {}
+
+    def blr():
+        stats.update(api.cauchyeig_blr_dev(1.0, tq, td, u, v, difl, difr, x, leaf=256, tol=1e-12))
+
+    msb = ev_time(blr, 3, torch)
+    eb = rel(x, ref)
+    bm, bf = ortho(x)
+    out["blr_rrqr_leaf256_tol1e-12"] = {
+        "ms": round(msb, 3), "eff_tflops_dense_equiv": round(flops / msb / 1e9, 2),
+        "speedup_vs_fp64": round(ms64 / msb, 3), "rel_err": eb, "orth_max": bm, "orth_fro": bf,
+        "stats": stats}
     del x, ref, tq
     torch.cuda.empty_cache()
     return out

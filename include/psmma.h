@@ -252,6 +252,26 @@ int psmma_rrqr_batched_dev(int nblk, int m, int n, const double* a, int lda, lon
                            long long stride_q, double* rpt, int ldr, long long stride_r,
                            void* stream);
 
+/* ===========================================================================
+ * 6. Compressed eigenvector-merge product (device pointers).
+ *
+ * X(m x n) = alpha * A(m x n) * C + beta * X with C the n x n eigen-Cauchy
+ * matrix of psmma_cauchyeig_dev (generators fpole, u, v, difl, difr, order n):
+ * C is split into leaf x leaf blocks (leaf <= 256, n % leaf == 0); every
+ * off-diagonal block is compressed by the batched truncated RRQR at relative
+ * tolerance tol (psmma_rrqr_batched_dev semantics, rank cap 32) from the
+ * generators, and multiplied in compressed form; diagonal and incompressible
+ * blocks use the structured FP64 kernel.  The reference's off-diagonal ring
+ * step (SRC/pstoeplitz_compute.f90:263-288: Compress_RRQR per tile, then two
+ * DGEMMs) on a grid of leaf tiles.  Error: each block within ~tol of its largest
+ * R entry.  stats (host, may be NULL, 5 entries): blocks, low-rank blocks,
+ * incompressible blocks, zero blocks, sum of ranks.  Synchronises `stream` once
+ * (the ranks decide the GEMM shapes).  Returns 0 or -(argument number). */
+int psmma_cauchyeig_blr_dev(int m, int n, double alpha, const double* a, int lda,
+                            const double* fpole, const double* u, const double* v,
+                            const double* difl, const double* difr, int leaf, double tol,
+                            double beta, double* c, int ldc, long long* stats, void* stream);
+
 /* Library version string. */
 const char* psmma_version(void);
 

@@ -68,6 +68,8 @@ _SIGS = {
     "psmma_secular_dev": ([_I, _P, _P, _D, _P, _P, _P, _P, _P], _I),
     "psmma_rrqr_batched_dev": ([_I, _I, _I, _P, _I, ctypes.c_longlong, _D, _I, _P, _P, _P, _I,
                                 ctypes.c_longlong, _P, _I, ctypes.c_longlong, _P], _I),
+    "psmma_cauchyeig_blr_dev": ([_I, _I, _D, _P, _I, _P, _P, _P, _P, _P, _I, _D, _D, _P, _I, _P,
+                                 _P], _I),
     "psmma_batched_dev": ([_I, _P, _I, _I, _P], _I),
     "psmma_batched_plan_create": ([_I, _P, _I, _I, ctypes.POINTER(ctypes.c_void_p)], _I),
     "psmma_batched_plan_launch": ([_P, _P], _I),
@@ -315,6 +317,21 @@ def secular_dev(d, z, rho=1.0, loewner=True, stream=None):
 
 
 RRQR_KMAX = 32
+
+
+def cauchyeig_blr_dev(alpha, a, fpole, u, v, difl, difr, c, leaf=256, tol=1e-12, beta=0.0,
+                      stream=None):
+    """X = alpha * A * C + beta * X with the off-diagonal leaf blocks of the eigen-Cauchy
+    matrix C compressed by the batched RRQR.  a, c: column-major (m x n) device views.
+    Returns the stats dict (blocks, lowrank, dense, zero, rank_sum)."""
+    pa, m, n, lda = _colmajor(a)
+    pc, mc, nc, ldc = _colmajor(c)
+    st = (ctypes.c_longlong * 5)()
+    _check(lib().psmma_cauchyeig_blr_dev(m, n, alpha, pa, lda, fpole.data_ptr(), u.data_ptr(),
+                                         v.data_ptr(), difl.data_ptr(), difr.data_ptr(), leaf,
+                                         float(tol), beta, pc, ldc, st, _stream_ptr(stream)),
+           "psmma_cauchyeig_blr_dev")
+    return dict(zip(("blocks", "lowrank", "dense", "zero", "rank_sum"), list(st)))
 
 
 def rrqr_batched_dev(blocks, tol, kmax=RRQR_KMAX, stream=None):

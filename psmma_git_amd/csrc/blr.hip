@@ -1,0 +1,301 @@
+// Compressed form of the eigenvector-merge product X = alpha * A * C + beta * X,
+// C the eigen-Cauchy matrix of psmma_cauchyeig: C is split into leaf x leaf
+// blocks; every off-diagonal block is compressed by the batched truncated RRQR
+// (Compress_RRQR semantics, tolerance tol) straight from the generators, and
+// the product runs as
+//   diagonal and incompressible blocks:  X(:,J) += A(:,I) C(I,J)   (structured FP64 MFMA)
+//   low-rank blocks C(I,J) ~ Q_IJ R_IJ:  T_I = A(:,I) [Q_I1 .. Q_In]       (one GEMM per I)
+//                                        X(:,J) += [T_1J .. T_nJ] [R_1J; ..; R_nJ]
+// The T columns are regrouped from (I, J) to (J, I) order between the two
+// GEMM stages, so each stage is one plain GEMM per block row / block column.
+// This is the reference's off-diagonal ring step (PSTOEPLITZ_COMPUTE's
+// RRQR branch, SRC/pstoeplitz_compute.f90:263-288, which compresses every
+// off-diagonal tile with Compress_RRQR and multiplies A by [Q | R P^T] with two
+// DGEMMs) applied on one GPU to a grid of leaf-sized tiles, with the plain
+// GEMMs grouped per block row and column.
+#include <hip/hip_runtime.h>
+#include <rocblas/rocblas.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <mutex>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/psmma.h"
+#include "psmma_common.h"
+
+namespace psmma {
+hipError_t launch_right_f64(int kind, int cfg, int M, int N, int K, double alpha, const double* A,
+                            int lda, const GenPtrs& g, int r0, int c0, double beta, double* C,
+                            int ldc, hipStream_t s);
+hipError_t launch_rrqr_structured(int kind, const GenPtrs& g, const int* r0, const int* c0,
+                                  int nblk, int m, int n, double tol, int kmax, int* rank,
+                                  int* jpvt, long long jstride, double* q, int ldq,
+                                  long long qstride, double* rpt, int ldr, long long rstride,
+                                  hipStream_t s);
+}  // namespace psmma
+
+using namespace psmma;
+
+namespace {
+
+constexpr int kKmax = 32;
+
+// One low-rank block: where its Q and R P^T slots are, and where its columns
+// go in the two GEMM stages.
+struct PackEntry {
+  int slot;   // block index of the RRQR outputs
+  int r;      // rank
+  int offa;   // column offset in U (block-row order (I, J))
+  int offb;   // column offset in T' and row offset in R (block-column order (J, I))
+};
+
+// U(:, offa + c) = Q_slot(:, c) and R(offb + c, :) = (R P^T)_slot(c, :), one
+// workgroup per block.
+__global__ __launch_bounds__(256) void blr_pack_kernel(const PackEntry* __restrict__ tab, int leaf,
+                                                       const double* __restrict__ qs,
+                                                       const double* __restrict__ rs,
+                                                       double* __restrict__ U, double* __restrict__ R,
+                                                       long long S) {
+  const PackEntry e = tab[blockIdx.x];
+  const double* q = qs + (size_t)e.slot * leaf * kKmax;
+  const double* rp = rs + (size_t)e.slot * kKmax * leaf;
+  for (int x = threadIdx.x; x < leaf * e.r; x += 256) {
+    const int i = x % leaf, c = x / leaf;
+    U[(size_t)(e.offa + c) * leaf + i] = q[(size_t)c * leaf + i];
+    // R P^T is stored column-major with leading dimension kKmax
+    R[(size_t)i * S + e.offb + c] = rp[(size_t)i * kKmax + c];
+  }
+}
+
+// T'(:, offb + c) = T(:, offa + c) for every low-rank block: grid (blocks, column chunks).
+__global__ __launch_bounds__(256) void blr_regroup_kernel(const PackEntry* __restrict__ tab, int m,
+                                                          const double* __restrict__ T,
+                                                          double* __restrict__ Tp, int ldt) {
+  const PackEntry e = tab[blockIdx.x];
+  for (int c = 0; c < e.r; ++c) {
+    const double* src = T + (size_t)(e.offa + c) * ldt;
+    double* dst = Tp + (size_t)(e.offb + c) * ldt;
+    for (int i = blockIdx.y * 256 + threadIdx.x; i < m; i += 256 * gridDim.y) dst[i] = src[i];
+  }
+}
+
+struct DevWs {
+  void* p = nullptr;
+  size_t bytes = 0;
+};
+std::mutex g_mu;
+std::unordered_map<int, DevWs> g_ws[2];
+std::unordered_map<int, rocblas_handle> g_blas;
+
+// Grow-only per-device workspaces: part 0 holds the RRQR outputs, part 1 the
+// rank-dependent GEMM operands.  Calls on one device share them.
+void* workspace(int part, size_t bytes) {
+  int dev = 0;
+  hipGetDevice(&dev);
+  std::lock_guard<std::mutex> lk(g_mu);
+  DevWs& w = g_ws[part][dev];
+  if (w.bytes < bytes) {
+    if (w.p) {
+      hipDeviceSynchronize();
+      hipFree(w.p);
+    }
+    w.p = nullptr;
+    w.bytes = 0;
+    if (hipMalloc(&w.p, bytes) != hipSuccess) return nullptr;
+    w.bytes = bytes;
+  }
+  return w.p;
+}
+
+rocblas_handle blas() {
+  int dev = 0;
+  hipGetDevice(&dev);
+  std::lock_guard<std::mutex> lk(g_mu);
+  auto it = g_blas.find(dev);
+  if (it != g_blas.end()) return it->second;
+  rocblas_handle h = nullptr;
+  if (rocblas_create_handle(&h) != rocblas_status_success) return nullptr;
+  g_blas[dev] = h;
+  return h;
+}
+
+size_t align_up(size_t x) { return (x + 255) & ~size_t(255); }
+
+}  // namespace
+
+extern "C" int psmma_cauchyeig_blr_dev(int m, int nglob, double alpha, const double* a, int lda,
+                                       const double* fpole, const double* u, const double* v,
+                                       const double* difl, const double* difr, int leaf,
+                                       double tol, double beta, double* c, int ldc,
+                                       long long* stats, void* stream) {
+  if (m < 0) return -1;
+  if (nglob < 0) return -2;
+  if (lda < std::max(1, m)) return -5;
+  if (leaf < 1 || leaf > 256 || (nglob % leaf) != 0) return -11;
+  if (!(tol >= 0.0)) return -12;
+  if (ldc < std::max(1, m)) return -15;
+  if (m == 0 || nglob == 0) return 0;
+  if (!a || !c || !fpole || !u || !v || !difl || !difr) return -4;
+  hipStream_t s = (hipStream_t)stream;
+  const int nb = nglob / leaf;
+  const int nblk = nb * (nb - 1);
+  GenPtrs g{fpole, u, v, difl, difr, nglob, nglob, nglob};
+
+  // workspace: block lists, RRQR outputs
+  const size_t qbytes = (size_t)nblk * leaf * kKmax * sizeof(double);
+  size_t off_r0 = 0, off_c0 = align_up(off_r0 + sizeof(int) * nblk);
+  size_t off_rank = align_up(off_c0 + sizeof(int) * nblk);
+  size_t off_jp = align_up(off_rank + sizeof(int) * nblk);
+  size_t off_q = align_up(off_jp + sizeof(int) * (size_t)nblk * leaf);
+  size_t off_r = align_up(off_q + qbytes);
+  size_t fixed = align_up(off_r + qbytes);
+  char* ws = (char*)workspace(0, fixed);
+  if (!ws) return 1002;
+  std::vector<int> hr0(nblk), hc0(nblk);
+  for (int I = 0, b = 0; I < nb; ++I)
+    for (int J = 0; J < nb; ++J)
+      if (I != J) {
+        hr0[b] = I * leaf;
+        hc0[b] = J * leaf;
+        ++b;
+      }
+  int* dr0 = (int*)(ws + off_r0);
+  int* dc0 = (int*)(ws + off_c0);
+  int* drank = (int*)(ws + off_rank);
+  double* dq = (double*)(ws + off_q);
+  double* drp = (double*)(ws + off_r);
+  hipError_t e = hipSuccess;
+  if (nblk > 0) {
+    e = hipMemcpyAsync(dr0, hr0.data(), sizeof(int) * nblk, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(dc0, hc0.data(), sizeof(int) * nblk, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess)
+      e = launch_rrqr_structured(KIND_CAUCHYEIG, g, dr0, dc0, nblk, leaf, leaf, tol, kKmax, drank,
+                                 (int*)(ws + off_jp), leaf, dq, leaf, (long long)leaf * kKmax, drp,
+                                 kKmax, (long long)kKmax * leaf, s);
+  }
+  std::vector<int> hrank(nblk);
+  if (e == hipSuccess && nblk > 0) {
+    e = hipMemcpyAsync(hrank.data(), drank, sizeof(int) * nblk, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+  }
+  if (e != hipSuccess) {
+    fprintf(stderr, "psmma_cauchyeig_blr_dev: %s\n", hipGetErrorString(e));
+    return 1000 + (int)e;
+  }
+
+  // classify the blocks; offsets of the low-rank columns in both stage orders
+  std::vector<std::vector<int>> rowlist(nb), collist(nb);  // low-rank block ids per I / per J
+  std::vector<int> dense;
+  long long ranksum = 0, nzero = 0;
+  for (int b = 0; b < nblk; ++b) {
+    const int r = hrank[b];
+    if (r == 0) {
+      ++nzero;
+    } else if (r < 0 || r >= leaf) {
+      dense.push_back(b);
+    } else {
+      rowlist[hr0[b] / leaf].push_back(b);
+      collist[hc0[b] / leaf].push_back(b);
+      ranksum += r;
+    }
+  }
+  std::vector<PackEntry> tab(nblk);
+  std::vector<int> offa_I(nb + 1, 0), offb_J(nb + 1, 0);
+  long long S = 0;
+  for (int I = 0; I < nb; ++I) {
+    offa_I[I] = (int)S;
+    for (int b : rowlist[I]) {
+      tab[b].slot = b;
+      tab[b].r = hrank[b];
+      tab[b].offa = (int)S;
+      S += hrank[b];
+    }
+  }
+  offa_I[nb] = (int)S;
+  long long Sb = 0;
+  for (int J = 0; J < nb; ++J) {
+    offb_J[J] = (int)Sb;
+    for (int b : collist[J]) {
+      tab[b].offb = (int)Sb;
+      Sb += hrank[b];
+    }
+  }
+  offb_J[nb] = (int)Sb;
+  std::vector<PackEntry> packed;
+  for (int I = 0; I < nb; ++I)
+    for (int b : rowlist[I]) packed.push_back(tab[b]);
+  if (stats) {
+    stats[0] = nblk;
+    stats[1] = (long long)packed.size();
+    stats[2] = (long long)dense.size();
+    stats[3] = nzero;
+    stats[4] = ranksum;
+  }
+
+  // second workspace part: pack table, U (leaf x S), R (S x leaf), T and T' (m x S)
+  const size_t off_tab = 0;
+  const size_t off_u = align_up(off_tab + sizeof(PackEntry) * std::max<size_t>(1, packed.size()));
+  const size_t off_rr = align_up(off_u + sizeof(double) * (size_t)leaf * S);
+  const size_t off_t = align_up(off_rr + sizeof(double) * (size_t)S * leaf);
+  const size_t off_tp = align_up(off_t + sizeof(double) * (size_t)m * S);
+  const size_t total = align_up(off_tp + sizeof(double) * (size_t)m * S);
+  char* ws2 = (char*)workspace(1, total);
+  if (!ws2) return 1002;
+  PackEntry* dtab = (PackEntry*)(ws2 + off_tab);
+  double* U = (double*)(ws2 + off_u);
+  double* R = (double*)(ws2 + off_rr);
+  double* T = (double*)(ws2 + off_t);
+  double* Tp = (double*)(ws2 + off_tp);
+  // the table is copied before any launch of this call (the stream is idle here)
+  if (!packed.empty()) {
+    e = hipMemcpy(dtab, packed.data(), sizeof(PackEntry) * packed.size(), hipMemcpyHostToDevice);
+    if (e != hipSuccess) return 1000 + (int)e;
+  }
+
+  // diagonal blocks first (they carry beta), then incompressible blocks
+  for (int J = 0; J < nb && e == hipSuccess; ++J)
+    e = launch_right_f64(KIND_CAUCHYEIG, -1, m, leaf, leaf, alpha, a + (size_t)J * leaf * lda,
+                         lda, g, J * leaf, J * leaf, beta, c + (size_t)J * leaf * ldc, ldc, s);
+  for (size_t x = 0; x < dense.size() && e == hipSuccess; ++x) {
+    const int b = dense[x], I = hr0[b] / leaf, J = hc0[b] / leaf;
+    e = launch_right_f64(KIND_CAUCHYEIG, -1, m, leaf, leaf, alpha, a + (size_t)I * leaf * lda,
+                         lda, g, I * leaf, J * leaf, 1.0, c + (size_t)J * leaf * ldc, ldc, s);
+  }
+  if (e == hipSuccess && !packed.empty()) {
+    blr_pack_kernel<<<(unsigned)packed.size(), 256, 0, s>>>(dtab, leaf, dq, drp, U, R, S);
+    e = hipGetLastError();
+  }
+  if (e != hipSuccess) {
+    fprintf(stderr, "psmma_cauchyeig_blr_dev: %s\n", hipGetErrorString(e));
+    return 1000 + (int)e;
+  }
+  if (!packed.empty()) {
+    rocblas_handle h = blas();
+    if (!h) return 1004;
+    rocblas_set_stream(h, s);
+    const double one = 1.0, zero = 0.0;
+    for (int I = 0; I < nb; ++I) {  // T_I = A(:, I) U_I
+      const int w = offa_I[I + 1] - offa_I[I];
+      if (w == 0) continue;
+      if (rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, m, w, leaf, &one,
+                        a + (size_t)I * leaf * lda, lda, U + (size_t)offa_I[I] * leaf, leaf,
+                        &zero, T + (size_t)offa_I[I] * m, m) != rocblas_status_success)
+        return 1005;
+    }
+    dim3 grid((unsigned)packed.size(), (unsigned)std::min(64, (m + 255) / 256));
+    blr_regroup_kernel<<<grid, 256, 0, s>>>(dtab, m, T, Tp, m);
+    if ((e = hipGetLastError()) != hipSuccess) return 1000 + (int)e;
+    for (int J = 0; J < nb; ++J) {  // X(:, J) += alpha T'_J R_J
+      const int w = offb_J[J + 1] - offb_J[J];
+      if (w == 0) continue;
+      if (rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, m, leaf, w, &alpha,
+                        Tp + (size_t)offb_J[J] * m, m, R + offb_J[J], (int)S, &one,
+                        c + (size_t)J * leaf * ldc, ldc) != rocblas_status_success)
+        return 1005;
+    }
+  }
+  return 0;
+}
