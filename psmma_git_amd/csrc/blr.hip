@@ -255,10 +255,17 @@ extern "C" int psmma_cauchyeig_blr_dev(int m, int nglob, double alpha, const dou
     if (e != hipSuccess) return 1000 + (int)e;
   }
 
-  // diagonal blocks first (they carry beta), then incompressible blocks
-  for (int J = 0; J < nb && e == hipSuccess; ++J)
-    e = launch_right_f64(KIND_CAUCHYEIG, -1, m, leaf, leaf, alpha, a + (size_t)J * leaf * lda,
-                         lda, g, J * leaf, J * leaf, beta, c + (size_t)J * leaf * ldc, ldc, s);
+  // diagonal blocks first (they carry beta; disjoint outputs: one grouped
+  // launch), then incompressible blocks one by one (they accumulate)
+  {
+    std::vector<psmma_problem> diag(nb);
+    for (int J = 0; J < nb; ++J)
+      diag[J] = psmma_problem{PSMMA_KIND_CAUCHYEIG, m, leaf, leaf, alpha, beta,
+                              a + (size_t)J * leaf * lda, lda, fpole, u, v, difl, difr, nglob,
+                              0, J * leaf, J * leaf, c + (size_t)J * leaf * ldc, ldc};
+    int st = psmma_batched_dev(nb, diag.data(), PSMMA_PREC_FP64, 0, stream);
+    if (st) return st;
+  }
   for (size_t x = 0; x < dense.size() && e == hipSuccess; ++x) {
     const int b = dense[x], I = hr0[b] / leaf, J = hc0[b] / leaf;
     e = launch_right_f64(KIND_CAUCHYEIG, -1, m, leaf, leaf, alpha, a + (size_t)I * leaf * lda,

@@ -7,14 +7,14 @@
 //   half the digits; rank = number of leading |R(i,i)| above tol |R(0,0)|;
 //   output H = [Q (m x rank) | R P^T (rank x n)] when rank < min(m,n).
 //
-// One 256-thread workgroup per block (m, n <= 256), left-looking: the block
+// One 512-thread workgroup per block (m, n <= 256), left-looking: the block
 // is never updated in place.  After k reflectors the updated matrix is
 // A - V F^T (V the reflectors, F = A^T V T accumulated one column per step,
 // as LAPACK's blocked panel does), so each step reads the original block once
 // (A^T v), regenerating it from the generators when the block is a structured
-// one.  V lives in LDS; each thread owns one pivot position (its source
-// column, F row, R column and partial norms) in registers and, for the
-// row-wise work, one row.
+// one.  V lives in LDS; each pivot position (its source column, F row, R
+// column and partial norms) lives in the registers of two threads, which split
+// the row loops; the row-wise work takes one row per thread.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -29,13 +29,18 @@ constexpr int kRrqrKmax = 32;   // rank cap of the batched kernel
 constexpr int kRrqrMaxDim = 256;
 constexpr int kVld = kRrqrMaxDim + 1;  // padded LDS pitch: conflict-free row and column walks
 
+// Block sources.  bind() stages what the block needs in LDS; col(c) returns
+// the per-column context a thread keeps for its current source column; at(i,
+// ctx) is the entry (i, c).
 struct SrcExplicit {
   const double* a;
   int lda;
   long long stride;
   const double* base;
-  __device__ void bind(int b) { base = a + (size_t)b * stride; }
-  __device__ double operator()(int i, int j) const { return base[i + (size_t)j * lda]; }
+  using Ctx = const double*;
+  __device__ void bind(int b, double*, int, int) { base = a + (size_t)b * stride; }
+  __device__ Ctx col(int c) const { return base + (size_t)c * lda; }
+  __device__ double at(int i, Ctx p) const { return p[i]; }
 };
 
 template <int KIND>
@@ -44,12 +49,51 @@ struct SrcGen {
   const int* r0;
   const int* c0;
   int rb, cb;
-  __device__ void bind(int b) {
+  using Ctx = int;
+  __device__ void bind(int b, double*, int, int) {
     rb = r0[b];
     cb = c0[b];
   }
-  __device__ double operator()(int i, int j) const { return gen_entry<KIND>(g, rb + i, cb + j); }
+  __device__ Ctx col(int c) const { return cb + c; }
+  __device__ double at(int i, Ctx c) const { return gen_entry<KIND>(g, rb + i, c); }
 };
+
+// Eigen-Cauchy blocks: the row poles and weights staged in LDS, the column's
+// pole, next pole, DIFL, DIFR and normaliser in registers; the same
+// arithmetic as gen_entry<KIND_CAUCHYEIG>.
+struct SrcEigStaged {
+  GenPtrs g;
+  const int* r0;
+  const int* c0;
+  int rb, cb;
+  const double* s;
+  struct Ctx {
+    double fc, fc1, dl, dr, vc;
+    int cg;
+  };
+  __device__ void bind(int b, double* st, int m, int) {
+    rb = r0[b];
+    cb = c0[b];
+    const int t = threadIdx.x;
+    if (t < m) {
+      st[t] = g.p0[rb + t];
+      st[kRrqrMaxDim + t] = g.p1[rb + t];
+    }
+    s = st;
+  }
+  __device__ Ctx col(int c) const {
+    const int cg = cb + c;
+    return Ctx{g.p0[cg], g.p0[min(cg + 1, g.korder - 1)], g.p3[cg], g.p4[cg], g.p2[cg], cg};
+  }
+  __device__ double at(int i, const Ctx& x) const {
+    const double den = (rb + i <= x.cg) ? ((s[i] - x.fc) - x.dl) : ((s[i] - x.fc1) - x.dr);
+    return (s[kRrqrMaxDim + i] * x.vc) / den;
+  }
+};
+
+constexpr int kThreads = 512;  // two threads per pivot position (row halves)
+constexpr int kWaves = kThreads / 64;
+constexpr int kFld = kRrqrKmax + 1;  // F row pitch in LDS
 
 // Sum over the workgroup in a fixed order; every thread gets the same value.
 __device__ double block_sum(double x, double* red) {
@@ -58,49 +102,102 @@ __device__ double block_sum(double x, double* red) {
   __syncthreads();
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = x;
   __syncthreads();
-  return ((red[0] + red[1]) + red[2]) + red[3];
+  double r = red[0];
+  for (int w = 1; w < kWaves; ++w) r += red[w];
+  return r;
 }
 
+// Thread layout: thread t owns pivot position pos = t % 256 together with its
+// twin t ^ 256: the two keep identical copies of the position's source column,
+// column context, R column and partial norms, split the row loops in halves
+// and exchange the two partial sums through LDS (added in a fixed order, so the
+// copies stay identical).  The position's F row lives in LDS (written by the
+// first twin).  Row-wise work (the pivot column, the reflector, Q) runs on the
+// first 256 threads, one row each.
 template <int KMAX, class Src>
-__global__ __launch_bounds__(256, 1) void rrqr_kernel(Src src, int m, int n, double tol, int kmax,
-                                                      int* __restrict__ rank_out,
-                                                      int* __restrict__ jpvt_out, long long jstride,
-                                                      double* __restrict__ q, int ldq,
-                                                      long long qstride, double* __restrict__ rpt,
-                                                      int ldr, long long rstride) {
+__global__ __launch_bounds__(kThreads, 1) void rrqr_kernel(Src src, int m, int n, double tol,
+                                                           int kmax, int* __restrict__ rank_out,
+                                                           int* __restrict__ jpvt_out,
+                                                           long long jstride,
+                                                           double* __restrict__ q, int ldq,
+                                                           long long qstride,
+                                                           double* __restrict__ rpt, int ldr,
+                                                           long long rstride) {
+  static_assert(KMAX == kRrqrKmax, "F pitch");
   __shared__ double V[KMAX * kVld];
-  __shared__ double E[KMAX * kVld];  // Q accumulation; exchange/broadcast scratch before that
+  __shared__ double Fs[kRrqrMaxDim * kFld];  // F rows by position; T and W after the loop
+  __shared__ double Y[KMAX * KMAX];          // column l: V(:, 0:l)^T v_l
+  __shared__ double part[(kThreads / KMAX) * KMAX];
+  __shared__ double hpart[2 * kRrqrMaxDim];  // per-half partial sums of each position
+  __shared__ double X[2 * KMAX + 8];         // R columns of the two exchanged positions
+  __shared__ double s_alpha;
   __shared__ double s_tau[KMAX];
-  __shared__ double red[4];
-  __shared__ double s_key[4];
-  __shared__ int s_idx[4];
+  __shared__ double red[kWaves];
+  __shared__ double s_key[kWaves];
+  __shared__ int s_idx[kWaves];
+  __shared__ int s_col[2];
+  __shared__ double gst[2 * kRrqrMaxDim];  // row generators of a structured block
 
   const int b = blockIdx.x;
   const int t = threadIdx.x;
-  src.bind(b);
+  const int pos = t & (kRrqrMaxDim - 1), half = t >> 8;
+  src.bind(b, gst, m, n);
+  for (int x = t; x < KMAX * kVld; x += kThreads) V[x] = 0.0;
+  for (int x = t; x < kRrqrMaxDim * kFld; x += kThreads) Fs[x] = 0.0;
+  __syncthreads();
   const int minmn = min(m, n);
   const double tol3z = 1.0536712127723509e-08;  // sqrt(2^-53)
+  // this thread's half of rows [lo, m)
+  auto split = [&](int lo, int& i0, int& i1) {
+    const int mid = lo + ((m - lo) >> 1);
+    i0 = half ? mid : lo;
+    i1 = half ? m : mid;
+  };
+  // combine the two halves' partial sums (uniform call: contains barriers)
+  auto combine = [&](double mine) {
+    hpart[half * kRrqrMaxDim + pos] = mine;
+    __syncthreads();
+    const double r = hpart[pos] + hpart[kRrqrMaxDim + pos];
+    __syncthreads();
+    return r;
+  };
 
-  double F[KMAX], R[KMAX];
+  double R[KMAX];
 #pragma unroll
-  for (int l = 0; l < KMAX; ++l) F[l] = R[l] = 0.0;
-  int col = t;
+  for (int l = 0; l < KMAX; ++l) R[l] = 0.0;
+  int col = pos;
+  typename Src::Ctx cx = src.col(min(pos, max(n - 1, 0)));
+  double* Frow = Fs + pos * kFld;
   double vn1 = -1.0, vn2 = -1.0;
-  if (t < n) {
-    double s = 0.0;
-    for (int i = 0; i < m; ++i) {
-      double x = src(i, t);
-      s += x * x;
+  {
+    int i0, i1;
+    split(0, i0, i1);
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+    if (pos < n) {
+      int i = i0;
+      for (; i + 3 < i1; i += 4) {
+        const double x0 = src.at(i, cx), x1 = src.at(i + 1, cx), x2 = src.at(i + 2, cx),
+                     x3 = src.at(i + 3, cx);
+        s0 += x0 * x0;
+        s1 += x1 * x1;
+        s2 += x2 * x2;
+        s3 += x3 * x3;
+      }
+      for (; i < i1; ++i) {
+        const double x = src.at(i, cx);
+        s0 += x * x;
+      }
     }
-    vn1 = vn2 = sqrt(s);
+    const double s = combine((s0 + s1) + (s2 + s3));
+    if (pos < n) vn1 = vn2 = sqrt(s);
   }
 
   int rank = 0;
   double r00 = 0.0;
   for (int k = 0; k < minmn; ++k) {
     // (1) pivot: largest partial norm among positions k..n-1, first on ties
-    double key = (t >= k && t < n) ? vn1 : -1.0;
-    int idx = t;
+    double key = (half == 0 && pos >= k && pos < n) ? vn1 : -1.0;
+    int idx = pos;
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) {
       double k2 = __shfl_xor(key, o);
@@ -117,72 +214,68 @@ __global__ __launch_bounds__(256, 1) void rrqr_kernel(Src src, int m, int n, dou
     __syncthreads();
     int p = s_idx[0];
     double pk = s_key[0];
-    for (int w = 1; w < 4; ++w)
-      if (s_key[w] > pk) {
+    for (int w = 1; w < kWaves; ++w)
+      if (s_key[w] > pk || (s_key[w] == pk && s_idx[w] < p)) {
         pk = s_key[w];
         p = s_idx[w];
       }
-    // (2) exchange positions k and p (source column, F row, R column, norms)
-    double* xa = E;                 // written by the thread at p
-    double* xb = E + 2 * KMAX + 8;  // written by the thread at k
+    // (2) exchange positions k and p: source column, norms, R column, F row
     if (p != k) {
-      if (t == p || t == k) {
-        double* dst = (t == p) ? xa : xb;
+      const bool mine = pos == p || pos == k;
+      const int other = (pos == p) ? k : p;
+      double ovn1 = 0.0, ovn2 = 0.0, fo = 0.0;
+      if (half == 0 && mine) {
+        hpart[pos] = vn1;
+        hpart[kRrqrMaxDim + pos] = vn2;
+        s_col[pos == p ? 0 : 1] = col;
 #pragma unroll
-        for (int l = 0; l < KMAX; ++l) {
-          dst[l] = F[l];
-          dst[KMAX + l] = R[l];
-        }
-        dst[2 * KMAX] = vn1;
-        dst[2 * KMAX + 1] = vn2;
-        dst[2 * KMAX + 2] = (double)col;
+        for (int l = 0; l < KMAX; ++l) X[(pos == p ? 0 : KMAX + 4) + l] = R[l];
+      }
+      // F rows: thread l of position k's twin pair swaps entry l
+      if (t < KMAX) {
+        fo = Fs[k * kFld + t];
+        Fs[k * kFld + t] = Fs[p * kFld + t];
       }
       __syncthreads();
-      if (t == p || t == k) {
-        const double* srcx = (t == p) ? xb : xa;
+      if (t < KMAX) Fs[p * kFld + t] = fo;
+      if (mine) {
+        ovn1 = hpart[other];
+        ovn2 = hpart[kRrqrMaxDim + other];
+        col = s_col[pos == p ? 1 : 0];
 #pragma unroll
-        for (int l = 0; l < KMAX; ++l) {
-          F[l] = srcx[l];
-          R[l] = srcx[KMAX + l];
-        }
-        vn1 = srcx[2 * KMAX];
-        vn2 = srcx[2 * KMAX + 1];
-        col = (int)srcx[2 * KMAX + 2];
+        for (int l = 0; l < KMAX; ++l) R[l] = X[(pos == p ? KMAX + 4 : 0) + l];
+        vn1 = ovn1;
+        vn2 = ovn2;
+        cx = src.col(col);
       }
       __syncthreads();
     }
-    // (3) the pivot column, updated: a(:, col_k) - V F_k^T
-    double* fk = E + 4 * KMAX + 16;  // F row of position k, then its column index
-    if (t == k) {
-#pragma unroll
-      for (int l = 0; l < KMAX; ++l) fk[l] = F[l];
-      fk[KMAX] = (double)col;
-    }
+    // (3) the pivot column, updated: a(:, col_k) - V F_k^T (first 256 threads: rows)
+    if (t == k) s_col[0] = col;
     __syncthreads();
-    const int ck = (int)fk[KMAX];
+    const int ck = s_col[0];
     double ahat = 0.0;
-    if (t < m) {
-      ahat = src(t, ck);
-#pragma unroll
-      for (int l = 0; l < KMAX; ++l)
-        if (l < k) ahat -= V[l * kVld + t] * fk[l];
+    if (half == 0 && t < m) {
+      const typename Src::Ctx ckx = src.col(ck);
+      ahat = src.at(t, ckx);
+      const double* fk = Fs + k * kFld;
+      for (int l = 0; l < k; ++l) ahat -= V[l * kVld + t] * fk[l];
     }
     // (4) Householder reflector on rows k..m-1 of the pivot column
-    const double xn2 = block_sum((t > k && t < m) ? ahat * ahat : 0.0, red);
-    double* s_alpha = E + 5 * KMAX + 24;
-    if (t == k) *s_alpha = ahat;
+    const double xn2 = block_sum((half == 0 && t > k && t < m) ? ahat * ahat : 0.0, red);
+    if (t == k) s_alpha = ahat;
     __syncthreads();
-    const double alpha = *s_alpha;
+    const double alpha = s_alpha;
     double tau = 0.0, beta = alpha, sc = 0.0;
     if (xn2 != 0.0 && k < m - 1) {
       beta = -copysign(sqrt(alpha * alpha + xn2), alpha);
       tau = (beta - alpha) / beta;
       sc = 1.0 / (alpha - beta);
     }
-    V[k * kVld + t] = (t == k) ? 1.0 : ((t > k && t < m) ? ahat * sc : 0.0);
+    if (half == 0) V[k * kVld + t] = (t == k) ? 1.0 : ((t > k && t < m) ? ahat * sc : 0.0);
     if (t == 0) s_tau[k] = tau;
     if (k == 0) r00 = fabs(beta);
-    if (t == k) {
+    if (pos == k) {
 #pragma unroll
       for (int l = 0; l < KMAX; ++l)
         if (l == k) R[l] = beta;
@@ -201,86 +294,149 @@ __global__ __launch_bounds__(256, 1) void rrqr_kernel(Src src, int m, int n, dou
       rank = -1;  // kmax or more columns above the tolerance
       break;
     }
-    // (5) y = V(:, 0:k)^T v_k, then f_k = tau (A^T v_k - F y) and row k of R
-    double* y = E + 6 * KMAX + 32;
-    if (t < k) {
-      double s = 0.0;
-      for (int i = k; i < m; ++i) s += V[t * kVld + i] * V[k * kVld + i];
-      y[t] = s;
+    // (5) y = V(:, 0:k)^T v_k, segmented over the rows: thread (segment, l)
+    double* y = Y + k * KMAX;
+    {
+      constexpr int NSEG = kThreads / KMAX, ROWS = kRrqrMaxDim / NSEG;
+      const int l = t % KMAX, seg = t / KMAX;
+      double sy = 0.0;
+      if (l < k) {
+        const int i0 = max(k, seg * ROWS), i1 = min(m, seg * ROWS + ROWS);
+        for (int i = i0; i < i1; ++i) sy += V[l * kVld + i] * V[k * kVld + i];
+      }
+      part[seg * KMAX + l] = sy;
+      __syncthreads();
+      if (t < k) {
+        double yy = 0.0;
+        for (int g2 = 0; g2 < NSEG; ++g2) yy += part[g2 * KMAX + t];
+        y[t] = yy;
+      }
+      __syncthreads();
     }
-    __syncthreads();
-    if (t > k && t < n) {
-      double g = 0.0;
-      for (int i = k; i < m; ++i) g += src(i, col) * V[k * kVld + i];
+    // (6) f_k = tau (A^T v_k - F y) and row k of R, for positions after k
+    const bool live = pos > k && pos < n;
+    double g = 0.0;
+    if (live) {
+      int i0, i1;
+      split(k, i0, i1);
+      double g0 = 0.0, g1 = 0.0, g2 = 0.0, g3 = 0.0;
+      const double* vk = V + k * kVld;
+      int i = i0;
+      for (; i + 3 < i1; i += 4) {
+        g0 += src.at(i, cx) * vk[i];
+        g1 += src.at(i + 1, cx) * vk[i + 1];
+        g2 += src.at(i + 2, cx) * vk[i + 2];
+        g3 += src.at(i + 3, cx) * vk[i + 3];
+      }
+      for (; i < i1; ++i) g0 += src.at(i, cx) * vk[i];
+      g = (g0 + g1) + (g2 + g3);
+    }
+    g = combine(g);
+    bool recompute = false;
+    if (live) {
       double fy = 0.0, vf = 0.0;
-#pragma unroll
-      for (int l = 0; l < KMAX; ++l)
-        if (l < k) {
-          fy += F[l] * y[l];
-          vf += V[l * kVld + k] * F[l];
-        }
+      for (int l = 0; l < k; ++l) {
+        const double fl = Frow[l];
+        fy += fl * y[l];
+        vf += V[l * kVld + k] * fl;
+      }
       const double f = tau * (g - fy);
-      const double rk = (src(k, col) - vf) - f;
+      const double rk = (src.at(k, cx) - vf) - f;
 #pragma unroll
       for (int l = 0; l < KMAX; ++l)
-        if (l == k) {
-          F[l] = f;
-          R[l] = rk;
-        }
-      // (6) partial norm downdate, or recomputation from the updated column
+        if (l == k) R[l] = rk;
+      if (half == 0) Frow[k] = f;
+      // (7) partial norm downdate, or recomputation from the updated column
       if (vn1 != 0.0) {
         double tt = fabs(rk) / vn1;
         tt = fmax(1.0 - tt * tt, 0.0);
         const double r12 = vn1 / vn2;
-        if (tt * r12 * r12 <= tol3z) {
-          double s = 0.0;
-          for (int i = k + 1; i < m; ++i) {
-            double x = src(i, col);
-#pragma unroll
-            for (int l = 0; l < KMAX; ++l)
-              if (l <= k) x -= V[l * kVld + i] * F[l];
-            s += x * x;
-          }
-          vn1 = vn2 = sqrt(s);
-        } else {
+        if (tt * r12 * r12 <= tol3z)
+          recompute = true;
+        else
           vn1 *= sqrt(tt);
-        }
       }
     }
-    __syncthreads();
+    __syncthreads();  // F(pos, k) visible to both twins
+    double sr = 0.0;
+    if (recompute) {
+      int i0, i1;
+      split(k + 1, i0, i1);
+      double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+      int i = i0;
+      for (; i + 3 < i1; i += 4) {
+        double x0 = src.at(i, cx), x1 = src.at(i + 1, cx), x2 = src.at(i + 2, cx),
+               x3 = src.at(i + 3, cx);
+        for (int l = 0; l <= k; ++l) {
+          const double fl = Frow[l];
+          const double* vl = V + l * kVld + i;
+          x0 -= vl[0] * fl;
+          x1 -= vl[1] * fl;
+          x2 -= vl[2] * fl;
+          x3 -= vl[3] * fl;
+        }
+        s0 += x0 * x0;
+        s1 += x1 * x1;
+        s2 += x2 * x2;
+        s3 += x3 * x3;
+      }
+      for (; i < i1; ++i) {
+        double x = src.at(i, cx);
+        for (int l = 0; l <= k; ++l) x -= V[l * kVld + i] * Frow[l];
+        s0 += x * x;
+      }
+      sr = (s0 + s1) + (s2 + s3);
+    }
+    sr = combine(sr);
+    if (recompute) vn1 = vn2 = sqrt(sr);
   }
 
   if (t == 0) rank_out[b] = rank;
-  if (t < n) jpvt_out[(size_t)b * jstride + t] = col + 1;
+  if (half == 0 && t < n) jpvt_out[(size_t)b * jstride + t] = col + 1;
   if (rank <= 0 || rank >= minmn) return;  // uniform: nothing more to write
 
   // R P^T: position t's column of R goes to column col
-  if (t < n) {
+  if (half == 0 && t < n) {
     double* dst = rpt + (size_t)b * rstride + (size_t)col * ldr;
 #pragma unroll
     for (int l = 0; l < KMAX; ++l)
       if (l < rank) dst[l] = R[l];
   }
-  // Q = H(0) ... H(rank-1) [I; 0], accumulated backwards in LDS
-  for (int c = 0; c < rank; ++c) E[c * kVld + t] = (t == c) ? 1.0 : 0.0;
-  __shared__ double s_w[KMAX];
+  // Q = H(0) ... H(rank-1) [I; 0] = [I; 0] - V T V(0:rank, :)^T, with T the
+  // upper-triangular factor of the compact WY form (T(l,l) = tau_l,
+  // T(0:l, l) = -tau_l T(0:l, 0:l) y_l), then W = T V(0:rank, 0:rank)^T.
+  // T and W reuse the F rows' LDS.
   __syncthreads();
-  for (int l = rank - 1; l >= 0; --l) {
-    if (t >= l && t < rank) {
-      double s = 0.0;
-      for (int i = l; i < m; ++i) s += V[l * kVld + i] * E[t * kVld + i];
-      s_w[t] = s_tau[l] * s;
+  double* Tm = Fs;
+  double* Wm = Fs + KMAX * KMAX;
+  for (int x = t; x < KMAX * KMAX; x += kThreads) Tm[x] = 0.0;
+  __syncthreads();
+  if (t == 0) {
+    for (int l = 0; l < rank; ++l) {
+      Tm[l * KMAX + l] = s_tau[l];
+      for (int j = 0; j < l; ++j) {
+        double acc = 0.0;
+        for (int p2 = j; p2 < l; ++p2) acc += Tm[p2 * KMAX + j] * Y[l * KMAX + p2];
+        Tm[l * KMAX + j] = -s_tau[l] * acc;  // stored as Tm[col * KMAX + row]
+      }
     }
-    __syncthreads();
-    if (t >= l && t < m) {
-      const double vi = V[l * kVld + t];
-      for (int c = l; c < rank; ++c) E[c * kVld + t] -= vi * s_w[c];
-    }
-    __syncthreads();
   }
-  if (t < m) {
+  __syncthreads();
+  for (int e2 = t; e2 < rank * rank; e2 += kThreads) {
+    const int l = e2 / rank, c = e2 % rank;
+    double acc = 0.0;
+    for (int j = l; j <= c; ++j) acc += Tm[j * KMAX + l] * V[j * kVld + c];  // V(c,j) = 0 for j > c
+    Wm[c * KMAX + l] = acc;
+  }
+  __syncthreads();
+  // rows pos, columns split between the halves
+  if (pos < m) {
     double* qb = q + (size_t)b * qstride;
-    for (int c = 0; c < rank; ++c) qb[t + (size_t)c * ldq] = E[c * kVld + t];
+    for (int c = half; c < rank; c += 2) {
+      double acc = (pos == c) ? 1.0 : 0.0;
+      for (int l = 0; l < rank; ++l) acc -= V[l * kVld + pos] * Wm[c * KMAX + l];
+      qb[pos + (size_t)c * ldq] = acc;
+    }
   }
 }
 
@@ -289,8 +445,8 @@ hipError_t launch_rrqr(const Src& src, int nblk, int m, int n, double tol, int k
                        int* jpvt, long long jstride, double* q, int ldq, long long qstride,
                        double* rpt, int ldr, long long rstride, hipStream_t s) {
   if (nblk <= 0) return hipSuccess;
-  rrqr_kernel<kRrqrKmax, Src><<<nblk, 256, 0, s>>>(src, m, n, tol, kmax, rank, jpvt, jstride, q,
-                                                   ldq, qstride, rpt, ldr, rstride);
+  rrqr_kernel<kRrqrKmax, Src><<<nblk, kThreads, 0, s>>>(src, m, n, tol, kmax, rank, jpvt, jstride,
+                                                        q, ldq, qstride, rpt, ldr, rstride);
   return hipGetLastError();
 }
 
@@ -310,8 +466,8 @@ hipError_t launch_rrqr_structured(int kind, const GenPtrs& g, const int* r0, con
       return launch_rrqr(SrcGen<KIND_HANKEL>{g, r0, c0, 0, 0}, nblk, m, n, tol, kmax, rank, jpvt,
                          jstride, q, ldq, qstride, rpt, ldr, rstride, s);
     default:
-      return launch_rrqr(SrcGen<KIND_CAUCHYEIG>{g, r0, c0, 0, 0}, nblk, m, n, tol, kmax, rank,
-                         jpvt, jstride, q, ldq, qstride, rpt, ldr, rstride, s);
+      return launch_rrqr(SrcEigStaged{g, r0, c0, 0, 0, nullptr}, nblk, m, n, tol, kmax, rank, jpvt,
+                         jstride, q, ldq, qstride, rpt, ldr, rstride, s);
   }
 }
 
