@@ -353,7 +353,7 @@ __global__ void gen_residue_kernel(const double* __restrict__ g, long long lo, l
 // D lane l element r holds D[4*(l/16)+r][l%16].
 enum { BMODE_DENSE = 0, BMODE_TOE = 1, BMODE_HANK = 2 };
 
-template <int BMODE, int BM, int BN, int BK, int WGM, int WGN, int KMAX>
+template <int BMODE, int BM, int BN, int BK, int WGM, int WGN, int KMAX, bool STAGE2 = true>
 __device__ __forceinline__ void gemm_i8_body(
     int M, int N, int kp, const signed char* __restrict__ Ares, const signed char* __restrict__ Bres,
     size_t aplane, size_t bplane, int lpad, const CrtParams& cp, unsigned char* __restrict__ out,
@@ -491,7 +491,42 @@ __device__ __forceinline__ void gemm_i8_body(
   };
 
   const int nkt = kp / BK;
-  uint4 ra0[AQ], rb0[BQ], ra1[AQ], rb1[BQ];
+  uint4 ra0[AQ], rb0[BQ], ra1[STAGE2 ? AQ : 1], rb1[STAGE2 ? BQ : 1];
+
+  if constexpr (!STAGE2) {
+    // One register stage (large tiles: the registers go to the accumulators):
+    // the loads of tile kt+1 are issued at the start of tile kt and stored to
+    // LDS after its MFMAs were issued, which cover their latency.
+    gload(0, ra0, rb0);
+    lstore(0, ra0, rb0);
+    __syncthreads();
+    frags(0, 0, 0, 0);
+    for (int kt = 0; kt < nkt; ++kt) {
+      const int cur = kt & 1;
+      const int kglob = kt * BK;
+      const int slot0 = (kt * KS) & 1;
+      if (kt + 1 < nkt) gload((kt + 1) * BK, ra0, rb0);
+#pragma unroll
+      for (int ks = 0; ks < KS; ++ks) {
+        const int slot = (slot0 + ks) & 1;
+        if (ks + 1 < KS) frags(cur, kglob, ks + 1, slot ^ 1);
+        v2l bf[TN];
+#pragma unroll
+        for (int j = 0; j < TN; ++j) bf[j] = bfrag(slot, j, kglob, ks);
+#pragma unroll
+        for (int i = 0; i < TM; ++i)
+#pragma unroll
+          for (int j = 0; j < TN; ++j)
+            acc[i][j] =
+                __builtin_amdgcn_mfma_i32_16x16x64_i8(af[slot][i], bf[j], acc[i][j], 0, 0, 0);
+        if (ks + 1 == KS && kt + 1 < nkt) {
+          lstore(cur ^ 1, ra0, rb0);
+          __syncthreads();
+          frags(cur ^ 1, kglob + BK, 0, slot ^ 1);
+        }
+      }
+    }
+  } else {
 
   // One stage: hold = registers with stage kt+1, into = registers for stage kt+2.
   auto stage = [&](int kt, int slot0, uint4* hold_a, uint4* hold_b, uint4* into_a,
@@ -530,6 +565,7 @@ __device__ __forceinline__ void gemm_i8_body(
     stage(kt, 0, ra1, rb1, ra0, rb0);
     if (kt + 1 < nkt) stage(kt + 1, SLOT1, ra0, rb0, ra1, rb1);
   }
+  }
 
 #pragma unroll
   for (int i = 0; i < TM; ++i)
@@ -555,7 +591,8 @@ __device__ __forceinline__ void gemm_i8_body(
 }
 
 
-template <int BMODE, int BM, int BN, int BK, int WGM, int WGN, int MINB, int KMAX>
+template <int BMODE, int BM, int BN, int BK, int WGM, int WGN, int MINB, int KMAX,
+          bool STAGE2 = true>
 __global__ __launch_bounds__(WGM* WGN * 64, MINB) void gemm_i8_kernel(
     int M, int N, int kp, const signed char* __restrict__ Ares, const signed char* __restrict__ Bres,
     size_t aplane, size_t bplane, int lpad, CrtParams cp, unsigned char* __restrict__ out,
@@ -563,8 +600,9 @@ __global__ __launch_bounds__(WGM* WGN * 64, MINB) void gemm_i8_kernel(
   const int nbn = (N + BN - 1) / BN, nbm = (M + BM - 1) / BM;
   int bmi, bni;
   patch_remap<4, 8>(blockIdx.x, nbm, nbn, bmi, bni);
-  gemm_i8_body<BMODE, BM, BN, BK, WGM, WGN, KMAX>(M, N, kp, Ares, Bres, aplane, bplane, lpad, cp,
-                                                  out, oplane, bmi, bni, blockIdx.y);
+  gemm_i8_body<BMODE, BM, BN, BK, WGM, WGN, KMAX, STAGE2>(M, N, kp, Ares, Bres, aplane, bplane,
+                                                          lpad, cp, out, oplane, bmi, bni,
+                                                          blockIdx.y);
 }
 
 // The error-free transformations below rely on every rounding being the one
@@ -616,15 +654,21 @@ __device__ __forceinline__ double crt_garner(const unsigned char* __restrict__ r
 }
 
 template <int T>
-__device__ __forceinline__ void crt_body(int M, int N, const unsigned char* __restrict__ res,
+__device__ __forceinline__ void crt_elem(int M, const unsigned char* __restrict__ res,
                                          size_t oplane, const CrtParams& cp,
                                          const int* __restrict__ ea, const int* __restrict__ eh,
                                          int abits, int hbits, double alpha, double beta,
-                                         double* __restrict__ C, int ldc, int bx) {
+                                         double* __restrict__ C, int ldc, size_t off) {
 #pragma clang fp contract(off)
-  size_t off = (size_t)bx * blockDim.x + threadIdx.x;
-  if (off >= (size_t)M * N) return;
-  int i = (int)(off % M), j = (int)(off / M);
+  int i, j;
+  if (off < 0x80000000ull) {  // 32-bit index arithmetic whenever it suffices
+    const unsigned o = (unsigned)off, mu = (unsigned)M;
+    j = (int)(o / mu);
+    i = (int)(o - (unsigned)j * mu);
+  } else {
+    i = (int)(off % M);
+    j = (int)(off / M);
+  }
   double shi = 0.0, slo = 0.0;
 #pragma unroll
   for (int l = 0; l < T; ++l) {
@@ -655,6 +699,17 @@ __device__ __forceinline__ void crt_body(int M, int N, const unsigned char* __re
   double* p = C + (size_t)j * ldc + i;
   if (beta != 0.0) v += beta * (*p);
   *p = v;
+}
+
+template <int T>
+__device__ __forceinline__ void crt_body(int M, int N, const unsigned char* __restrict__ res,
+                                         size_t oplane, const CrtParams& cp,
+                                         const int* __restrict__ ea, const int* __restrict__ eh,
+                                         int abits, int hbits, double alpha, double beta,
+                                         double* __restrict__ C, int ldc, int bx) {
+  const size_t off = (size_t)bx * blockDim.x + threadIdx.x;
+  if (off >= (size_t)M * N) return;
+  crt_elem<T>(M, res, oplane, cp, ea, eh, abits, hbits, alpha, beta, C, ldc, off);
 }
 
 template <int T>
@@ -705,7 +760,9 @@ static void launch_gemm_i8(int t, int M, int N, int kp, const signed char* ares,
     return e ? atoi(e) : -1;
   }();
   int cfg = forced;
-  if (cfg < 0) cfg = 0;
+  // 256 x 256 tiles (128 x 128 per wave, accumulators in AGPRs) once the
+  // product has enough of them to fill the chip; 128 x 128 below that.
+  if (cfg < 0) cfg = (M >= 2048 && N >= 2048) ? 4 : 0;
   auto nblk = [&](int bm, int bn) { return ((M + bm - 1) / bm) * ((N + bn - 1) / bn); };
   switch (cfg) {
     case 1:
@@ -716,6 +773,16 @@ static void launch_gemm_i8(int t, int M, int N, int kp, const signed char* ares,
     case 2:
       hipLaunchKernelGGL((gemm_i8_kernel<BMODE, 256, 128, 128, 2, 2, 1, KMAX>),
                          dim3(nblk(256, 128), t), dim3(256), 0, s, M, N, kp, ares, bres, aplane,
+                         bplane, lpad, cp, ores, oplane);
+      break;
+    case 3:
+      hipLaunchKernelGGL((gemm_i8_kernel<BMODE, 256, 256, 64, 2, 2, 1, KMAX, false>),
+                         dim3(nblk(256, 256), t), dim3(256), 0, s, M, N, kp, ares, bres, aplane,
+                         bplane, lpad, cp, ores, oplane);
+      break;
+    case 4:
+      hipLaunchKernelGGL((gemm_i8_kernel<BMODE, 256, 256, 128, 2, 2, 1, KMAX, false>),
+                         dim3(nblk(256, 256), t), dim3(256), 0, s, M, N, kp, ares, bres, aplane,
                          bplane, lpad, cp, ores, oplane);
       break;
     default:
@@ -1115,20 +1182,24 @@ struct SplitProblem {
   int ub[NPH];  // first unit of this problem in the phase
 };
 
-__device__ __forceinline__ int find_split_prob(const SplitProblem* P, int np, int ph, int u) {
+// The problem owning unit u of a phase: binary search over the phase's
+// contiguous array of first units (ubv[ph * np + i] = P[i].ub[ph]).
+__device__ __forceinline__ int find_split_prob(const int* __restrict__ ub, int np, int u) {
   int lo = 0, hi = np - 1;
   while (lo < hi) {
     int mid = (lo + hi + 1) >> 1;
-    if (P[mid].ub[ph] <= u) lo = mid; else hi = mid - 1;
+    if (ub[mid] <= u) lo = mid; else hi = mid - 1;
   }
   return lo;
 }
 
 __global__ __launch_bounds__(256) void gsplit_phase_kernel(const SplitProblem* __restrict__ P,
-                                                           int np, int ph, CrtParams cp) {
+                                                           const int* __restrict__ ubv, int np,
+                                                           int ph, CrtParams cp) {
   const int u = blockIdx.x;
-  const SplitProblem& q = P[find_split_prob(P, np, ph, u)];
-  const int lu = u - q.ub[ph];
+  const int pi = find_split_prob(ubv + ph * np, np, u);
+  const SplitProblem& q = P[pi];
+  const int lu = u - ubv[ph * np + pi];
   const int bx = lu % q.gx[ph], by = lu / q.gx[ph];
   const bool dense = q.bmode == BMODE_DENSE;
   switch (ph) {
@@ -1176,10 +1247,12 @@ constexpr int kBatchKmax = 16384;  // longest contraction of a generator-window 
 
 template <int BMODE>
 __global__ __launch_bounds__(256, 2) void gsplit_gemm_kernel(const SplitProblem* __restrict__ P,
-                                                             int np, int ph, CrtParams cp) {
+                                                             const int* __restrict__ ubv, int np,
+                                                             int ph, CrtParams cp) {
   const int u = blockIdx.x;
-  const SplitProblem& q = P[find_split_prob(P, np, ph, u)];
-  const int lu = u - q.ub[ph];
+  const int pi = find_split_prob(ubv + ph * np, np, u);
+  const SplitProblem& q = P[pi];
+  const int lu = u - ubv[ph * np + pi];
   const int ntn = (q.N + 127) / 128, ntm = (q.M + 127) / 128;
   const int l = lu / (ntm * ntn), r = lu % (ntm * ntn);
   gemm_i8_body<BMODE, 128, 128, 128, 2, 2, kBatchKmax>(q.M, q.N, q.kp, q.ares, q.bres, q.aplane,
@@ -1187,13 +1260,24 @@ __global__ __launch_bounds__(256, 2) void gsplit_gemm_kernel(const SplitProblem*
                                                       r / ntn, r % ntn, l);
 }
 
+constexpr int kCrtPerThread = 4;  // elements per thread of a grouped CRT unit
+
 template <int T>
-__global__ __launch_bounds__(256) void gsplit_crt_kernel(const SplitProblem* __restrict__ P, int np,
+__global__ __launch_bounds__(256) void gsplit_crt_kernel(const SplitProblem* __restrict__ P,
+                                                         const int* __restrict__ ubv, int np,
                                                          CrtParams cp) {
   const int u = blockIdx.x;
-  const SplitProblem& q = P[find_split_prob(P, np, PH_CRT, u)];
-  crt_body<T>(q.M, q.N, q.ores, q.oplane, cp, q.ea, q.eh, q.abits, q.abits, q.alpha, q.beta, q.C,
-              q.ldc, u - q.ub[PH_CRT]);
+  const int pi = find_split_prob(ubv + PH_CRT * np, np, u);
+  const SplitProblem& q = P[pi];
+  const size_t base = (size_t)(u - ubv[PH_CRT * np + pi]) * (256 * kCrtPerThread) + threadIdx.x;
+  const size_t mn = (size_t)q.M * q.N;
+#pragma unroll
+  for (int r = 0; r < kCrtPerThread; ++r) {
+    const size_t off = base + (size_t)r * 256;
+    if (off < mn)
+      crt_elem<T>(q.M, q.ores, q.oplane, cp, q.ea, q.eh, q.abits, q.abits, q.alpha, q.beta, q.C,
+                  q.ldc, off);
+  }
 }
 
 // Host plan of a grouped split batch (table, unit totals, workspace size).
@@ -1263,7 +1347,8 @@ void split_batch_layout(const BatchProblem* probs, int nprob, int t, char* ws, S
     else { gx[PH_SPLITB] = (q.lpad + 255) / 256; gy[PH_SPLITB] = 1; }
     int gph = dense ? PH_GEMM_D : (q.bmode == BMODE_TOE ? PH_GEMM_T : PH_GEMM_H);
     gx[gph] = ((q.M + 127) / 128) * ((q.N + 127) / 128) * t; gy[gph] = 1;
-    gx[PH_CRT] = (int)(((size_t)q.M * q.N + 255) / 256); gy[PH_CRT] = 1;
+    gx[PH_CRT] = (int)(((size_t)q.M * q.N + 256 * kCrtPerThread - 1) / (256 * kCrtPerThread));
+    gy[PH_CRT] = 1;
     for (int ph = 0; ph < NPH; ++ph) {
       q.gx[ph] = gx[ph];
       q.ub[ph] = sb.total[ph];
@@ -1277,6 +1362,7 @@ void split_batch_layout(const BatchProblem* probs, int nprob, int t, char* ws, S
 // Enqueues the grouped pipeline; dtab is the device copy of sb.tab.
 int split_batch_enqueue(const SplitBatch& sb, const SplitProblem* dtab, hipStream_t s, int* nodes) {
   const int np = (int)sb.tab.size();
+  const int* dub = (const int*)(dtab + np);  // the phases' unit prefixes follow the table
   int n = 0;
   if (np == 0) {
     if (nodes) *nodes = 0;
@@ -1284,28 +1370,29 @@ int split_batch_enqueue(const SplitBatch& sb, const SplitProblem* dtab, hipStrea
   }
   for (int ph : {PH_FILL, PH_ROWEXP, PH_COLEXP, PH_EHFILL, PH_SPLITA, PH_SPLITB}) {
     if (sb.total[ph] == 0) continue;
-    hipLaunchKernelGGL(gsplit_phase_kernel, dim3(sb.total[ph]), dim3(256), 0, s, dtab, np, ph, sb.cp);
+    hipLaunchKernelGGL(gsplit_phase_kernel, dim3(sb.total[ph]), dim3(256), 0, s, dtab, dub, np, ph,
+                       sb.cp);
     ++n;
   }
   if (sb.total[PH_GEMM_D]) {
     hipLaunchKernelGGL(gsplit_gemm_kernel<BMODE_DENSE>, dim3(sb.total[PH_GEMM_D]), dim3(256), 0, s,
-                       dtab, np, (int)PH_GEMM_D, sb.cp);
+                       dtab, dub, np, (int)PH_GEMM_D, sb.cp);
     ++n;
   }
   if (sb.total[PH_GEMM_T]) {
     hipLaunchKernelGGL(gsplit_gemm_kernel<BMODE_TOE>, dim3(sb.total[PH_GEMM_T]), dim3(256), 0, s,
-                       dtab, np, (int)PH_GEMM_T, sb.cp);
+                       dtab, dub, np, (int)PH_GEMM_T, sb.cp);
     ++n;
   }
   if (sb.total[PH_GEMM_H]) {
     hipLaunchKernelGGL(gsplit_gemm_kernel<BMODE_HANK>, dim3(sb.total[PH_GEMM_H]), dim3(256), 0, s,
-                       dtab, np, (int)PH_GEMM_H, sb.cp);
+                       dtab, dub, np, (int)PH_GEMM_H, sb.cp);
     ++n;
   }
   dim3 cg(sb.total[PH_CRT]);
 #define PSMMA_GCRT_CASE(TT)                                                              \
   case TT:                                                                               \
-    hipLaunchKernelGGL(gsplit_crt_kernel<TT>, cg, dim3(256), 0, s, dtab, np, sb.cp);     \
+    hipLaunchKernelGGL(gsplit_crt_kernel<TT>, cg, dim3(256), 0, s, dtab, dub, np, sb.cp); \
     break;
   switch (sb.t) {
     PSMMA_GCRT_CASE(2) PSMMA_GCRT_CASE(3) PSMMA_GCRT_CASE(4) PSMMA_GCRT_CASE(5) PSMMA_GCRT_CASE(6)
@@ -1350,9 +1437,15 @@ int gsplit_bind(void* hv, void* ws) {
   GSplit* h = (GSplit*)hv;
   split_batch_layout(h->probs.data(), (int)h->probs.size(), h->sb.t, (char*)ws, h->sb);
   if (h->sb.tab.empty()) return 0;
-  size_t bytes = sizeof(SplitProblem) * h->sb.tab.size();
-  if (hipMalloc(&h->dtab, bytes) != hipSuccess) return 1002;
+  const size_t np = h->sb.tab.size();
+  size_t bytes = sizeof(SplitProblem) * np;
+  std::vector<int> ubv(NPH * np);
+  for (int ph = 0; ph < NPH; ++ph)
+    for (size_t i = 0; i < np; ++i) ubv[ph * np + i] = h->sb.tab[i].ub[ph];
+  if (hipMalloc(&h->dtab, bytes + sizeof(int) * ubv.size()) != hipSuccess) return 1002;
   hipError_t e = hipMemcpy(h->dtab, h->sb.tab.data(), bytes, hipMemcpyHostToDevice);
+  if (e == hipSuccess)
+    e = hipMemcpy(h->dtab + np, ubv.data(), sizeof(int) * ubv.size(), hipMemcpyHostToDevice);
   return e == hipSuccess ? 0 : 1000 + (int)e;
 }
 
