@@ -81,6 +81,8 @@ def _declare(L):
     L.ora_mdgeqp3.restype = I
     L.ora_compress_rrqr.argtypes = [I, I, P, I, P, D]
     L.ora_compress_rrqr.restype = I
+    L.ora_compress_cauchy.argtypes = [P, P, P, P, D, I, I, P]
+    L.ora_compress_cauchy.restype = I
 
 
 def _f64(x):
@@ -263,6 +265,26 @@ def compress_rrqr(a, tol: float = RRQR_TOL):
     q = h[:m * rk].reshape((m, rk), order="F").copy()
     r = h[m * rk:m * rk + rk * n].reshape((rk, n), order="F").copy()
     return rk, q, r
+
+
+RRLU_TOL = float(np.float32(1e-17))  # ConstCauchylowrank's TOL: the default-real literal 1.0E-17
+
+
+def compress_cauchy(d, f, u, v, tol: float = RRLU_TOL, h_init=None):
+    """SRC/Cauchylowrank.f90:387-491 ('R'): row ID of U_i V_j / (D_i - F_j).
+    Returns (rk, X (m x rk), Y (rk x n), h): X, Y are None when rk >= m (H unwritten);
+    h is the raw buffer (initialised from h_init, to observe what is left unwritten)."""
+    d, f, u, v = (np.ascontiguousarray(x, dtype=np.float64) for x in (d, f, u, v))
+    m, n = len(d), len(f)
+    h = np.zeros(max(1, 2 * m * n)) if h_init is None else np.array(h_init, np.float64)
+    rk = lib().ora_compress_cauchy(_ptr(d), _ptr(f), _ptr(u), _ptr(v), tol, m, n, _ptr(h))
+    if rk < 0:
+        raise RuntimeError("ora_compress_cauchy failed")
+    if rk >= m:
+        return rk, None, None, h
+    x = h[:m * rk].reshape((m, rk), order="F").copy()
+    y = h[m * rk:m * rk + rk * n].reshape((rk, n), order="F").copy()
+    return rk, x, y, h
 
 
 def rel_fro(x, ref) -> float:

@@ -53,5 +53,10 @@ int ora_lowrank_tile(const ora_problem* p, int kbp, int nq, const int* ridx, con
 int ora_mdgeqp3(int m, int n, double* a, int lda, int* jpvt, double* tau, double tol, int nb,
                 int* rk);
 /* Compress_RRQR (SRC/Auxil_2p5D.f90:1060-1189): returns the rank; h = [Q | R P^T]. */
+int ora_rrlu_cauchy(double* D, double* F, double* A, double* V, double tol, double* U,
+                    double* W, int* PL, int* PU, int M, int N);
+/* compress_cauchy('R') (SRC/Cauchylowrank.f90:387-491): returns Rk; h = [X | Y] when Rk < M. */
+int ora_compress_cauchy(const double* D, const double* F, const double* U, const double* V,
+                        double tol, int M, int N, double* H);
 int ora_compress_rrqr(int m, int n, double* a, int lda, double* h, double tol);
 #endif

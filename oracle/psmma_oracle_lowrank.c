@@ -3,6 +3,9 @@
  * (SRC/Auxil_2p5D.f90:1060-1189) and ConstToeplitzlowrank
  * (SRC/Auxil_2p5D.f90:968-1057).
  * TEST INFRASTRUCTURE ONLY: used by tests/ and by bench.py's CPU baseline.
+ * Also the plain Cauchy generator RRLU: rrluCauchy, CauchyPivtG(_CP),
+ * searchMax2, CauchyMax, compress_cauchy, invp, Cauchylike2
+ * (SRC/Cauchylowrank.f90:7-559) behind ConstCauchylowrank.
  *
  * The factorisation is Householder QR with column pivoting on the largest
  * partial column norm, with the partial norms downdated from the new row of R
@@ -201,17 +204,263 @@ int ora_compress_rrqr(int m, int n, double* a, int lda, double* h, double tol) {
   return rk;
 }
 
+/* ------------------------------------------------------------------------- */
+/* Generator-only structured RRLU of a Cauchy-like block (SRC/Cauchylowrank.f90),
+ * restated with 0-based indices; each expression keeps the reference's
+ * evaluation order. */
+
+/* CauchyMax (SRC/Cauchylowrank.f90:166-188): largest |u_i / (D_i - f)| over n
+ * entries and its first index; n = 0 gives -DBL_MAX (Fortran's maxval of an
+ * empty array) and index -1. */
+static double cauchy_max(const double* d, double f, const double* u, int n, int* at) {
+  double best = -1.79769313486231570815e+308;
+  *at = -1;
+  for (int i = 0; i < n; ++i) {
+    double x = fabs(u[i] / (d[i] - f));
+    if (*at < 0 || x > best) {
+      best = x;
+      *at = i;
+    }
+  }
+  return best;
+}
+
+static void dswp(double* x, int i, int j) {
+  double t = x[i];
+  x[i] = x[j];
+  x[j] = t;
+}
+static void iswp(int* x, int i, int j) {
+  int t = x[i];
+  x[i] = x[j];
+  x[j] = t;
+}
+
+/* CauchyPivtG (SRC/Cauchylowrank.f90:191-243): alternating row/column pivot
+ * search from position k.  The reference's CONTINUE after a row swap is a
+ * no-op, so a row swap falls through to a column swap; that order is kept.
+ * The loop ends when the current pivot is the largest of its column or row
+ * (an iteration cap guards the restatement against cycling on ties). */
+static void cauchy_pivt_g(double* D, double* F, double* u, double* v, int* PL, int* PU, int k,
+                          double* A, int M, int N) {
+  int jjL, jjU;
+  double junkL = cauchy_max(D + k, F[k], u + k, M - k, &jjL);
+  double junkU = cauchy_max(F + k, D[k], v + k, N - k, &jjU);
+  junkL = junkL * fabs(v[k]);
+  junkU = junkU * fabs(u[k]);
+  double piv = fabs(u[k] * v[k] / (D[k] - F[k]));
+  if (junkL <= piv && junkU <= piv) return;
+  int flg = (junkL > junkU) ? 1 : 0;
+  for (long it = 0; it < 4L * (M + N) + 16; ++it) {
+    if (flg == 1) {
+      int jj = jjL;
+      dswp(D, k, jj + k);
+      dswp(u, k, jj + k);
+      dswp(A, k, jj + k);
+      iswp(PL, k, jj + k);
+      cauchy_max(F + k, D[k], v + k, N - k, &jjU);
+      if (jjU == 0) return;
+      flg = 0;
+    }
+    int jj = jjU;
+    dswp(F, k, jj + k);
+    dswp(v, k, jj + k);
+    iswp(PU, k, jj + k);
+    cauchy_max(D + k, F[k], u + k, M - k, &jjL);
+    if (jjL == 0) return;
+    flg = 1;
+  }
+}
+
+/* CauchyPivtG_CP (SRC/Cauchylowrank.f90:246-296): complete pivoting on the
+ * k-th Schur complement. */
+static void cauchy_pivt_cp(double* D, double* F, double* u, double* v, int* PL, int* PU, int k,
+                           double* A, int M, int N) {
+  int ii = 0, jj = 0;
+  double junk = 0.0;
+  for (int j = 0; j < N - k; ++j) {
+    int at;
+    double x = cauchy_max(D + k, F[k + j], u + k, M - k, &at);
+    x = fabs(x * v[k + j]);
+    if (j == 0 || x > junk) {
+      junk = x;
+      ii = at;
+      jj = j;
+    }
+  }
+  dswp(D, k, ii + k);
+  dswp(u, k, ii + k);
+  dswp(A, k, ii + k);
+  iswp(PL, k, ii + k);
+  dswp(F, k, jj + k);
+  dswp(v, k, jj + k);
+  iswp(PU, k, jj + k);
+}
+
+/* rrluCauchy (SRC/Cauchylowrank.f90:7-134).  D, F, A, V are modified in place
+ * (pivoted; V and the returned U hold the Schur-complement generators, W the
+ * interpolation generators).  PL/PU: 1-based permutations.  Returns Rk. */
+int ora_rrlu_cauchy(double* D, double* F, double* A, double* V, double tol, double* U,
+                    double* W, int* PL, int* PU, int M, int N) {
+  for (int i = 0; i < M; ++i) U[i] = A[i];
+  int rk = M < N ? M : N;
+  const int mn = rk;
+  for (int i = 0; i < M; ++i) PL[i] = i + 1;
+  for (int j = 0; j < N; ++j) PU[j] = j + 1;
+  double amax = 0.0;
+  const double rho = 1.12;
+  const int prd = 10;
+  for (int k = 0; k < mn; ++k) { /* the reference's k = this k + 1 */
+    cauchy_pivt_g(D, F, U, V, PL, PU, k, A, M, N);
+    double ukk = U[k] * V[k] / (D[k] - F[k]);
+    amax = fmax(amax, fabs(ukk));
+    if (fabs(ukk) < tol * amax) {
+      cauchy_pivt_cp(D, F, U, V, PL, PU, k, A, M, N);
+      ukk = U[k] * V[k] / (D[k] - F[k]);
+      amax = fmax(amax, fabs(ukk));
+      if (fabs(ukk) < tol * amax) {
+        rk = k;
+        break;
+      }
+    }
+    for (int i = k + 1; i < M; ++i) U[i] = U[i] * (D[i] - D[k]) / (D[i] - F[k]);
+    for (int j = k + 1; j < N; ++j) V[j] = V[j] * (F[j] - F[k]) / (F[j] - D[k]);
+    for (int j = 0; j < k; ++j) W[j] = W[j] * ((F[k] - D[j]) / (D[k] - D[j]));
+    W[k] = (D[k] - F[k]) / A[k];
+    for (int j = 0; j < k; ++j) W[k] = W[k] * (F[j] - D[k]) / (D[j] - D[k]);
+    /* the swap pass runs every prd-th step (flgl = mod(k, prd) < 1) */
+    int again = ((k + 1) % prd) < 1;
+    while (again) {
+      again = 0;
+      /* searchMax2 (SRC/Cauchylowrank.f90:137-163) over the k+1 chosen rows */
+      int ii = 0, jjl;
+      double nmax = cauchy_max(D + k + 1, D[0], U + k + 1, M - k - 1, &jjl) * fabs(W[0]);
+      int jj = jjl;
+      for (int j = 1; j <= k; ++j) {
+        double x = cauchy_max(D + k + 1, D[j], U + k + 1, M - k - 1, &jjl) * fabs(W[j]);
+        if (x > nmax) {
+          nmax = x;
+          ii = j;
+          jj = jjl;
+        }
+      }
+      if (nmax > rho) {
+        again = 1;
+        jj = jj + k + 1; /* 0-based row of the Schur complement */
+        for (int j = k + 1; j < N; ++j) V[j] = V[j] * ((F[j] - D[ii]) / (F[j] - D[jj]));
+        for (int i = k + 1; i < jj; ++i) U[i] = U[i] * ((D[i] - D[jj]) / (D[i] - D[ii]));
+        for (int i = jj + 1; i < M; ++i) U[i] = U[i] * ((D[i] - D[jj]) / (D[i] - D[ii]));
+        for (int j = 0; j < ii; ++j) W[j] = W[j] * ((D[j] - D[ii]) / (D[j] - D[jj]));
+        for (int j = ii + 1; j <= k; ++j) W[j] = W[j] * ((D[j] - D[ii]) / (D[j] - D[jj]));
+        U[jj] = A[ii] * ((D[ii] - D[jj]) / (D[ii] - F[ii]));
+        W[ii] = (D[jj] - F[ii]) / A[jj];
+        for (int j = 0; j < ii; ++j) {
+          U[jj] = U[jj] * ((D[ii] - D[j]) / (D[ii] - F[j]));
+          W[ii] = W[ii] * ((F[j] - D[jj]) / (D[j] - D[jj]));
+        }
+        for (int j = ii + 1; j <= k; ++j) {
+          U[jj] = U[jj] * ((D[ii] - D[j]) / (D[ii] - F[j]));
+          W[ii] = W[ii] * ((F[j] - D[jj]) / (D[j] - D[jj]));
+        }
+        iswp(PL, ii, jj);
+        dswp(D, ii, jj);
+        dswp(A, ii, jj);
+      }
+    }
+  }
+  return rk;
+}
+
+/* compress_cauchy('R', ...) (SRC/Cauchylowrank.f90:387-491): row ID of the
+ * Cauchy-like block U_i V_j / (D_i - F_j) (M x N).  When Rk < M:
+ * H = [X (M x Rk) | Y (Rk x N)], X = P^-1 [I; Z_i W_j / (D_i - D_j)],
+ * Y = the chosen rows of the block from the original F and V; otherwise H is
+ * left unwritten (the reference prints a full-rank warning).  Returns Rk. */
+int ora_compress_cauchy(const double* D, const double* F, const double* U, const double* V,
+                        double tol, int M, int N, double* H) {
+  int mn = M < N ? M : N;
+  double* LD = (double*)malloc(sizeof(double) * (size_t)(M + 1));
+  double* LU = (double*)malloc(sizeof(double) * (size_t)(M + 1));
+  double* LF = (double*)malloc(sizeof(double) * (size_t)(N + 1));
+  double* LV = (double*)malloc(sizeof(double) * (size_t)(N + 1));
+  double* Z = (double*)calloc((size_t)(M + 1), sizeof(double));
+  double* W = (double*)calloc((size_t)(mn + 1), sizeof(double));
+  int* PL = (int*)malloc(sizeof(int) * (size_t)(M + 1));
+  int* PU = (int*)malloc(sizeof(int) * (size_t)(N + 1));
+  int* IP = (int*)malloc(sizeof(int) * (size_t)(M + 1));
+  int rk = -1;
+  if (LD && LU && LF && LV && Z && W && PL && PU && IP) {
+    memcpy(LD, D, sizeof(double) * (size_t)M);
+    memcpy(LU, U, sizeof(double) * (size_t)M);
+    memcpy(LF, F, sizeof(double) * (size_t)N);
+    memcpy(LV, V, sizeof(double) * (size_t)N);
+    rk = ora_rrlu_cauchy(LD, LF, LU, LV, tol, Z, W, PL, PU, M, N);
+    for (int i = 0; i < M; ++i) IP[PL[i] - 1] = i; /* invp */
+    if (rk < M) {
+      double* X = H;
+      for (int j = 0; j < rk; ++j)
+        for (int i = 0; i < M; ++i) {
+          const int p = IP[i]; /* row of the pivoted order that is original row i */
+          double x;
+          if (p < rk)
+            x = (p == j) ? 1.0 : 0.0;
+          else
+            x = Z[p] * W[j] / (LD[p] - LD[j]); /* Cauchylike2 */
+          X[(size_t)j * M + i] = x;
+        }
+      double* Y = H + (size_t)M * rk;
+      for (int j = 0; j < N; ++j)
+        for (int i = 0; i < rk; ++i) Y[(size_t)j * rk + i] = LU[i] * V[j] / (LD[i] - F[j]);
+    }
+  }
+  free(LD);
+  free(LU);
+  free(LF);
+  free(LV);
+  free(Z);
+  free(W);
+  free(PL);
+  free(PU);
+  free(IP);
+  return rk;
+}
+
 /* Compress_RRQR's TOL is the default-real literal 1.0E-13 stored into a DOUBLE
  * PRECISION variable (SRC/Auxil_2p5D.f90:1140): its value is the float nearest
  * to 1e-13. */
 #define ORA_RRQR_TOL ((double)1.0e-13f)
 
-/* Off-diagonal ring step of the faithful mode.  Toeplitz: ConstToeplitzlowrank
- * (explicit tile -> Compress_RRQR, SRC/Auxil_2p5D.f90:1050-1056; full rank ->
- * the explicit tile again).  The Cauchy kinds' generator
- * RRLU is not restated: those kinds report -2. */
+/* Off-diagonal ring step of the faithful mode.  Cauchy: the generator RRLU
+ * above.  Toeplitz: ConstToeplitzlowrank (explicit tile -> Compress_RRQR,
+ * SRC/Auxil_2p5D.f90:1050-1056; full rank -> the explicit tile again).  The
+ * eigen-Cauchy kind's RRLU (rrluCauchyEigrow) is not restated: it reports -2. */
 int ora_lowrank_tile(const ora_problem* p, int kbp, int nq, const int* ridx, const int* cidx,
                      double* hbuf, unsigned long hcap, int* rk) {
+  if (p->kind == ORA_CAUCHY) {
+    /* ConstCauchylowrank (SRC/Auxil_2p5D.f90:876-963): generators gathered by
+     * the index sets, compress_cauchy('R') at the default-real literal
+     * TOL = 1.0E-17 (:946).  At full rank H keeps whatever the previous step
+     * left there and the ring multiplies that stale tile
+     * (SRC/pscauchy_compute.f90:379-381). */
+    if ((unsigned long)kbp * (unsigned long)nq * 2ul > hcap) return -3;
+    const size_t L = (size_t)p->ldgen;
+    double* g = (double*)malloc(sizeof(double) * (size_t)(2 * kbp + 2 * nq + 4));
+    if (!g) return -1;
+    double *d = g, *u = g + kbp, *f = g + 2 * kbp, *v = g + 2 * kbp + nq;
+    for (int i = 0; i < kbp; ++i) {
+      d[i] = p->gen[ridx[i] - 1];
+      u[i] = p->gen[2 * L + ridx[i] - 1];
+    }
+    for (int j = 0; j < nq; ++j) {
+      f[j] = p->gen[L + cidx[j] - 1];
+      v[j] = p->gen[3 * L + cidx[j] - 1];
+    }
+    int r = ora_compress_cauchy(d, f, u, v, (double)1.0e-17f, kbp, nq, hbuf);
+    free(g);
+    if (r < 0) return -1;
+    *rk = r;
+    return (r >= kbp) ? ORA_TILE_STALE : ORA_TILE_OK;
+  }
   if (p->kind != ORA_TOEPLITZ) return -2;
   if ((unsigned long)kbp * (unsigned long)nq * 2ul > hcap) return -3;
   double* t = (double*)malloc(sizeof(double) * (size_t)kbp * (size_t)(nq > 0 ? nq : 1));

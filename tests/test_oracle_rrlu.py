@@ -1,0 +1,125 @@
+"""The oracle's generator-only RRLU of Cauchy-like blocks (rrluCauchy /
+compress_cauchy restatement) and the faithful Cauchy ring that uses it.
+
+Pinned by: exact rank deficiency (rows with u_i = 0) found and the row ID
+reconstructing the block to rounding level; numerically low-rank blocks
+(separated poles) reconstructed within the tolerance; the swap pass (every
+10th step) exercised; the reference's tolerance literal; full rank leaving H
+unwritten; and, through the ring, the reference's stale-tile product on a 2 x 2
+grid recomputed independently in numpy."""
+import numpy as np
+import pytest
+
+from oracle import inputs
+
+
+def _cauchy(d, f, u, v):
+    return (u[:, None] * v[None, :]) / (d[:, None] - f[None, :])
+
+
+def test_rrlu_finds_exact_rank_deficiency(ora):
+    rng = np.random.default_rng(0)
+    m, n = 40, 30
+    b = inputs.cauchy_generators(m, rng)  # interlaced poles: numerically full rank
+    d, u = b[:, 0], b[:, 2].copy()
+    f, v = b[:n, 1], b[:n, 3]
+    assert ora.compress_cauchy(d, f, u, v)[0] == n
+    zero = rng.choice(m, 12, replace=False)
+    u[zero] = 0.0
+    rk, x, y, _ = ora.compress_cauchy(d, f, u, v)
+    c = _cauchy(d, f, u, v)
+    assert rk == 28  # min(m - 12, n) rows carry the block
+    assert ora.rel_fro(x @ y, c) < 1e-13
+    # the zero rows are interpolated by zero
+    assert np.all(x[zero] == 0.0)
+
+
+@pytest.mark.parametrize("tol", [1e-6, 1e-10, 1e-14])
+def test_rrlu_separated_poles_within_tolerance(ora, tol):
+    rng = np.random.default_rng(1)
+    m, n = 80, 70
+    d = np.sort(rng.random(m))
+    f = 1.05 + np.sort(rng.random(n))
+    u = rng.standard_normal(m)
+    v = rng.standard_normal(n)
+    c = _cauchy(d, f, u, v)
+    rk, x, y, _ = ora.compress_cauchy(d, f, u, v, tol)
+    assert 0 < rk < m
+    sv = np.linalg.svd(c, compute_uv=False)
+    assert rk >= int(np.sum(sv > 10 * tol * sv[0]))  # no fewer rows than the spectrum needs
+    assert np.linalg.norm(c - x @ y, 2) <= 1e3 * tol * sv[0]
+    if tol == 1e-14:
+        assert rk >= 10  # the swap pass of step 10 has run
+
+
+def test_rrlu_full_rank_leaves_h_unwritten(ora):
+    rng = np.random.default_rng(2)
+    n = 32
+    b = inputs.cauchy_generators(n, rng)
+    sentinel = np.full(2 * n * n, 7.25)
+    rk, x, y, h = ora.compress_cauchy(b[:, 0], b[:, 1], b[:, 2], b[:, 3], h_init=sentinel)
+    assert rk == n and x is None
+    assert np.all(h == 7.25)
+
+
+def test_rrlu_tolerance_is_the_default_real_literal(ora):
+    assert ora.RRLU_TOL == float(np.float32(1e-17)) and ora.RRLU_TOL != 1e-17
+
+
+def _local(ora, n, nb, P, Q, r, c):
+    rows = ora.constindex(n, nb, r, P) - 1
+    cols = ora.constindex(n, nb, c, Q) - 1
+    return rows, cols
+
+
+def test_faithful_cauchy_ring_multiplies_the_stale_tile(ora):
+    """Cauchy generators on a 2 x 2 grid: an off-diagonal step whose tile the RRLU
+    finds full rank multiplies the tile its previous step left in the work buffer,
+    as the reference does (compressed steps use the row ID, ~exact at TOL 1e-17);
+    the dense mode gives the true product."""
+    rng = np.random.default_rng(3)
+    n, nb, P, Q = 64, 8, 2, 2
+    b = inputs.cauchy_generators(n, rng)
+    a = rng.standard_normal((n, n))
+    h = _cauchy(b[:, 0], b[:, 1], b[:, 2], b[:, 3])
+    got, st = ora.ps_compute(ora.CAUCHY, P, Q, nb, 1.0, a, b, mode=ora.MODE_FAITHFUL)
+    dense, _ = ora.ps_compute(ora.CAUCHY, P, Q, nb, 1.0, a, b, mode=ora.MODE_DENSE)
+    assert ora.rel_fro(dense, a @ h) < 1e-14
+    want = np.zeros((n, n))
+    stale = 0
+    for r in range(P):
+        for c in range(Q):
+            rows, cols = _local(ora, n, nb, P, Q, r, c)
+            prev = None
+            acc = np.zeros((len(rows), len(cols)))
+            for k1 in range(Q):
+                ridx = ora.constindex(n, nb, (c + k1) % Q, Q) - 1
+                if k1 == 0:  # own block column: intersecting index sets, dense tile
+                    tile = h[np.ix_(ridx, cols)]
+                else:
+                    rk = ora.compress_cauchy(b[ridx, 0], b[cols, 1], b[ridx, 2], b[cols, 3])[0]
+                    if rk >= len(ridx):  # full rank: the buffer still holds the last tile
+                        tile = prev
+                        stale += 1
+                    else:
+                        tile = h[np.ix_(ridx, cols)]
+                acc = acc + a[np.ix_(rows, ridx)] @ tile
+                prev = tile
+            want[np.ix_(rows, cols)] = acc
+    assert stale == st[3] and stale > 0
+    assert ora.rel_fro(got, want) < 1e-13
+    assert ora.rel_fro(got, dense) > 1e-3  # the quirk changes the result
+
+
+def test_faithful_cauchy_ring_low_rank_branch(ora):
+    """Rows with u_i = 0 make the off-diagonal tiles rank deficient: the ring takes
+    the compressed branch and reproduces the true product."""
+    rng = np.random.default_rng(4)
+    n, nb = 64, 8
+    b = inputs.cauchy_generators(n, rng)
+    b[rng.choice(n, 40, replace=False), 2] = 0.0
+    a = rng.standard_normal((n, n))
+    h = _cauchy(b[:, 0], b[:, 1], b[:, 2], b[:, 3])
+    got, st = ora.ps_compute(ora.CAUCHY, 2, 2, nb, 1.0, a, b, mode=ora.MODE_FAITHFUL)
+    assert st[1] > 0
+    assert ora.rel_fro(got, a @ h) < 1e-12
