@@ -775,11 +775,6 @@ static void launch_gemm_i8(int t, int M, int N, int kp, const signed char* ares,
                          dim3(nblk(256, 128), t), dim3(256), 0, s, M, N, kp, ares, bres, aplane,
                          bplane, lpad, cp, ores, oplane);
       break;
-    case 3:
-      hipLaunchKernelGGL((gemm_i8_kernel<BMODE, 256, 256, 64, 2, 2, 1, KMAX, false>),
-                         dim3(nblk(256, 256), t), dim3(256), 0, s, M, N, kp, ares, bres, aplane,
-                         bplane, lpad, cp, ores, oplane);
-      break;
     case 4:
       hipLaunchKernelGGL((gemm_i8_kernel<BMODE, 256, 256, 128, 2, 2, 1, KMAX, false>),
                          dim3(nblk(256, 256), t), dim3(256), 0, s, M, N, kp, ares, bres, aplane,
@@ -1161,7 +1156,9 @@ static int launch_right_split_bf16(int kind, int s, int M, int N, int K, double 
 // CrtParams); each keeps its own power-of-two scales and abits.
 enum {
   PH_FILL = 0, PH_ROWEXP, PH_COLEXP, PH_EHFILL, PH_SPLITA, PH_SPLITB,
-  PH_GEMM_D, PH_GEMM_T, PH_GEMM_H, PH_CRT, NPH
+  PH_GEMM_D, PH_GEMM_T, PH_GEMM_H,     // 128 x 128 tiles
+  PH_GEMM_DB, PH_GEMM_TB, PH_GEMM_HB,  // 256 x 256 tiles (large problems)
+  PH_CRT, NPH
 };
 
 struct SplitProblem {
@@ -1260,6 +1257,26 @@ __global__ __launch_bounds__(256, 2) void gsplit_gemm_kernel(const SplitProblem*
                                                       r / ntn, r % ntn, l);
 }
 
+// Problems with min(M, N) >= kBigTile and K >= 4096 use 256 x 256 tiles
+// (128 x 128 per wave).
+constexpr int kBigTile = 2048;
+
+template <int BMODE>
+__global__ __launch_bounds__(256, 1) void gsplit_gemm_big_kernel(const SplitProblem* __restrict__ P,
+                                                                 const int* __restrict__ ubv,
+                                                                 int np, int ph, CrtParams cp) {
+  const int u = blockIdx.x;
+  const int pi = find_split_prob(ubv + ph * np, np, u);
+  const SplitProblem& q = P[pi];
+  const int lu = u - ubv[ph * np + pi];
+  const int ntn = (q.N + 255) / 256, ntm = (q.M + 255) / 256;
+  const int l = lu / (ntm * ntn), r = lu % (ntm * ntn);
+  gemm_i8_body<BMODE, 256, 256, 128, 2, 2, kBatchKmax, false>(q.M, q.N, q.kp, q.ares, q.bres,
+                                                              q.aplane, q.bplane, q.lpad, cp,
+                                                              q.ores, q.oplane, r / ntn, r % ntn,
+                                                              l);
+}
+
 constexpr int kCrtPerThread = 4;  // elements per thread of a grouped CRT unit
 
 template <int T>
@@ -1345,8 +1362,13 @@ void split_batch_layout(const BatchProblem* probs, int nprob, int t, char* ws, S
     gx[PH_SPLITA] = (q.M + 63) / 64; gy[PH_SPLITA] = q.kp / 64;
     if (dense) { gx[PH_SPLITB] = (q.N + 15) / 16; gy[PH_SPLITB] = q.kp / 256; }
     else { gx[PH_SPLITB] = (q.lpad + 255) / 256; gy[PH_SPLITB] = 1; }
+    // the large class pays off once the contraction is long enough to amortise
+    // its prologue and epilogue (measured: no gain at K <= 2048)
+    const bool big = std::min(q.M, q.N) >= kBigTile && q.K >= 4096;
     int gph = dense ? PH_GEMM_D : (q.bmode == BMODE_TOE ? PH_GEMM_T : PH_GEMM_H);
-    gx[gph] = ((q.M + 127) / 128) * ((q.N + 127) / 128) * t; gy[gph] = 1;
+    if (big) gph += PH_GEMM_DB - PH_GEMM_D;
+    const int tile = big ? 256 : 128;
+    gx[gph] = ((q.M + tile - 1) / tile) * ((q.N + tile - 1) / tile) * t; gy[gph] = 1;
     gx[PH_CRT] = (int)(((size_t)q.M * q.N + 256 * kCrtPerThread - 1) / (256 * kCrtPerThread));
     gy[PH_CRT] = 1;
     for (int ph = 0; ph < NPH; ++ph) {
@@ -1387,6 +1409,21 @@ int split_batch_enqueue(const SplitBatch& sb, const SplitProblem* dtab, hipStrea
   if (sb.total[PH_GEMM_H]) {
     hipLaunchKernelGGL(gsplit_gemm_kernel<BMODE_HANK>, dim3(sb.total[PH_GEMM_H]), dim3(256), 0, s,
                        dtab, dub, np, (int)PH_GEMM_H, sb.cp);
+    ++n;
+  }
+  if (sb.total[PH_GEMM_DB]) {
+    hipLaunchKernelGGL(gsplit_gemm_big_kernel<BMODE_DENSE>, dim3(sb.total[PH_GEMM_DB]), dim3(256),
+                       0, s, dtab, dub, np, (int)PH_GEMM_DB, sb.cp);
+    ++n;
+  }
+  if (sb.total[PH_GEMM_TB]) {
+    hipLaunchKernelGGL(gsplit_gemm_big_kernel<BMODE_TOE>, dim3(sb.total[PH_GEMM_TB]), dim3(256), 0,
+                       s, dtab, dub, np, (int)PH_GEMM_TB, sb.cp);
+    ++n;
+  }
+  if (sb.total[PH_GEMM_HB]) {
+    hipLaunchKernelGGL(gsplit_gemm_big_kernel<BMODE_HANK>, dim3(sb.total[PH_GEMM_HB]), dim3(256), 0,
+                       s, dtab, dub, np, (int)PH_GEMM_HB, sb.cp);
     ++n;
   }
   dim3 cg(sb.total[PH_CRT]);

@@ -11,14 +11,14 @@ pytestmark = pytest.mark.gpu
 KINDS = ["cauchy", "toeplitz", "hankel", "cauchyeig"]
 
 
-def make_batch(rng, nprob, nmax, beta_zero=True):
+def make_batch(rng, nprob, nmax, beta_zero=True, nmin=1):
     import torch
 
     probs, keep = [], []
     for i in range(nprob):
         kind = KINDS[i % 4]
-        n = int(rng.integers(1, nmax + 1))
-        m = int(rng.integers(1, nmax + 1))
+        n = int(rng.integers(nmin, nmax + 1))
+        m = int(rng.integers(nmin, nmax + 1))
         nglob = n + 3
         if kind == "cauchy":
             b = inputs.cauchy_generators(nglob, rng)
@@ -133,3 +133,21 @@ def test_batched_split_one_shot_with_zero_sized(gpu):
     errs = errors(keep)
     print("split one-shot errs max", max(errs))
     assert max(errs) <= 1e-12
+
+
+def test_batched_split_large_tiles(gpu):
+    """Problems with min(m, n) >= 2048 and k >= 4096 take the 256 x 256 tile class of the
+    grouped split GEMM (ragged edges included); small ones keep 128 x 128 in the same
+    plan."""
+    import torch
+
+    rng = np.random.default_rng(25)
+    big, kb = make_batch(rng, 4, 2100, nmin=2048)
+    small, ks = make_batch(rng, 4, 200)
+    plan = api.BatchedPlan(big + small, prec=api.PREC_SPLIT_I8, slices=0)
+    plan.launch()
+    torch.cuda.synchronize()
+    errs = errors(kb + ks)
+    print("split large-tile errs max", max(errs))
+    assert max(errs) <= 1e-12
+    plan.close()

@@ -13,6 +13,13 @@ from psmma_git_amd import api
 pytestmark = pytest.mark.gpu
 
 TOL = 1e-13
+
+
+def case_seed(*key):
+    """A seed fixed by the test case (str hashes vary between processes)."""
+    import zlib
+
+    return zlib.crc32(repr(key).encode())
 KINDS = ["cauchy", "toeplitz", "hankel", "cauchyeig"]
 
 
@@ -72,7 +79,7 @@ def test_right_fp64_parity(gpu, ora, kind, m, n, k):
     import torch
 
     print(f"case kind={kind} m={m} n={n} k={k}", flush=True)
-    rng = np.random.default_rng(hash((kind, m, n, k)) % (1 << 31))
+    rng = np.random.default_rng(case_seed(kind, m, n, k))
     nglob = max(k, n) + 5
     gens, dev = make_gens(kind, nglob, rng)
     r0, c0 = (3, 2) if nglob > max(k, n) + 3 else (0, 0)

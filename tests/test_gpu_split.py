@@ -8,7 +8,7 @@ import numpy as np
 import pytest
 
 from psmma_git_amd import api
-from test_gpu_parity import ORA_KIND, dev_colmajor, host, make_gens, run_dev
+from test_gpu_parity import ORA_KIND, case_seed, dev_colmajor, host, make_gens, run_dev
 
 pytestmark = pytest.mark.gpu
 
@@ -19,7 +19,7 @@ def test_split_i8_parity(gpu, ora, kind, m, n, k):
     import torch
 
     print(f"case split kind={kind} m={m} n={n} k={k}", flush=True)
-    rng = np.random.default_rng(hash(("s", kind, m, n, k)) % (1 << 31))
+    rng = np.random.default_rng(case_seed("s", kind, m, n, k))
     nglob = max(k, n) + 5
     gens, dev = make_gens(kind, nglob, rng)
     a = rng.uniform(-1, 1, (m, k))
@@ -34,7 +34,7 @@ def test_split_i8_parity(gpu, ora, kind, m, n, k):
         errs[t] = ora.rel_fro(host(dc), want)
     print(f"errs={errs}", flush=True)
     assert errs[13] <= 1e-12 and errs[14] <= 1e-12
-    assert errs[14] <= errs[10]
+    assert errs[14] <= max(errs[10], 1e-15)  # more moduli never lose accuracy beyond rounding
 
 
 def test_split_i8_zero_rows_and_columns(gpu, ora):
@@ -93,7 +93,7 @@ def test_split_bf16_parity(gpu, ora, kind, m, n, k):
     import torch
 
     print(f"case bf16 kind={kind} m={m} n={n} k={k}", flush=True)
-    rng = np.random.default_rng(hash(("b", kind, m, n, k)) % (1 << 31))
+    rng = np.random.default_rng(case_seed("b", kind, m, n, k))
     nglob = max(k, n) + 5
     gens, dev = make_gens(kind, nglob, rng)
     a = rng.uniform(-1, 1, (m, k))
