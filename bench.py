@@ -282,6 +282,33 @@ def batched_config(seed, torch, api, inputs):
             "split_err": {"median": q(es, 0.5), "p90": q(es, 0.9), "max": max(es)}}
 
 
+def profile_leg(leg, seed, torch, api, inputs):
+    """One workload alone, for kernel traces and counters."""
+    if leg == "batched":
+        return batched_config(seed, torch, api, inputs)
+    if leg == "split":
+        n = 8192
+        g_np, a_np = inputs.config1(n, seed=seed)
+        g = torch.from_numpy(g_np).cuda()
+        a = colmajor_from_numpy(a_np, torch)
+        c = api.colmajor_empty(n, n)
+        ms = ev_time(lambda: api.toeplitz_dev(1.0, a, g, n, n, c, prec=api.PREC_SPLIT_I8, slices=13),
+                     2, torch)
+        return {"leg": leg, "n": n, "slices": 13, "ms": round(ms, 3)}
+    n = 16384
+    d_np, z_np, _ = inputs.config3_dz(n, seed)
+    td, tz = torch.from_numpy(d_np).cuda(), torch.from_numpy(z_np).cuda()
+    difl, difr, u, v = api.secular_dev(td, tz, 1.0, loewner=True)
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(seed)
+    a = torch.randn((n, n), dtype=torch.float64, device="cuda", generator=gen).t()
+    x = api.colmajor_empty(n, n)
+    stats = {}
+    ms = ev_time(lambda: stats.update(api.cauchyeig_blr_dev(1.0, a, td, u, v, difl, difr, x,
+                                                            leaf=256, tol=1e-12)), 2, torch)
+    return {"leg": leg, "n": n, "ms": round(ms, 3), "stats": stats}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -295,6 +322,10 @@ def main():
                     help="skip the other benchmark configurations (reported beside the headline)")
     ap.add_argument("--copy-calib", action="store_true",
                     help="run one 512 MiB device copy first (calibrates HBM byte counters)")
+    ap.add_argument("--profile-leg", choices=["split", "batched", "blr"],
+                    help="run only one workload (for rocprofv3): the N=8192 Toeplitz split product "
+                         "(t=13, 3 launches), the 512-problem batch, or the N=16384 compressed "
+                         "merge product (3 launches); prints a short JSON line, not the bench line")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -309,6 +340,9 @@ def main():
     from psmma_git_amd import api
 
     api.lib()
+    if args.profile_leg:
+        print(json.dumps(profile_leg(args.profile_leg, args.seed, torch, api, inputs)), flush=True)
+        return
     n = args.n
     log(f"[bench] generating Toeplitz x dense N={n} seed={args.seed}")
     g_np, a_np = inputs.config1(n, seed=args.seed)
