@@ -266,8 +266,8 @@ __device__ __forceinline__ void right_tile(int M, int N, int K, double alpha,
   }
 }
 
-template <int KIND, int BM, int BN, int BK, int WGM, int WGN>
-__global__ __launch_bounds__(WGM* WGN * 64, 2) void sgemm_f64_right_kernel(
+template <int KIND, int BM, int BN, int BK, int WGM, int WGN, int MINB = 2>
+__global__ __launch_bounds__(WGM* WGN * 64, MINB) void sgemm_f64_right_kernel(
     int M, int N, int K, double alpha, const double* __restrict__ A, int lda, GenPtrs g, int r0,
     int c0, double beta, double* __restrict__ C, int ldc) {
   using L = RightLds<KIND, BM, BN, BK>;
@@ -528,12 +528,12 @@ struct RightTile {
   int bm, bn;
 };
 
-template <int KIND, int BM, int BN, int BK, int WGM, int WGN>
+template <int KIND, int BM, int BN, int BK, int WGM, int WGN, int MINB = 2>
 static hipError_t launch_right_t(int M, int N, int K, double alpha, const double* A, int lda,
                                  const GenPtrs& g, int r0, int c0, double beta, double* C, int ldc,
                                  hipStream_t s) {
   int nb = ((M + BM - 1) / BM) * ((N + BN - 1) / BN);
-  hipLaunchKernelGGL((sgemm_f64_right_kernel<KIND, BM, BN, BK, WGM, WGN>), dim3(nb),
+  hipLaunchKernelGGL((sgemm_f64_right_kernel<KIND, BM, BN, BK, WGM, WGN, MINB>), dim3(nb),
                      dim3(WGM * WGN * 64), 0, s, M, N, K, alpha, A, lda, g, r0, c0, beta, C, ldc);
   return hipGetLastError();
 }
@@ -552,6 +552,9 @@ static hipError_t launch_right_kind(int cfg, int M, int N, int K, double alpha, 
     case 3:  // 64x256: fewer dense-operand re-reads per flop
       return launch_right_t<KIND, 64, 256, 16, 1, 4>(M, N, K, alpha, A, lda, g, r0, c0, beta, C,
                                                      ldc, s);
+    case 4:  // 256x128, 8 waves of 64x64: half the generated entries per flop (Cauchy kinds)
+      return launch_right_t<KIND, 256, 128, 16, 4, 2, 1>(M, N, K, alpha, A, lda, g, r0, c0, beta,
+                                                         C, ldc, s);
     default:  // small problems: 64x64 workgroup tile, 4 waves of 32x32
       return launch_right_t<KIND, 64, 64, 16, 2, 2>(M, N, K, alpha, A, lda, g, r0, c0, beta, C,
                                                     ldc, s);
@@ -559,7 +562,7 @@ static hipError_t launch_right_kind(int cfg, int M, int N, int K, double alpha, 
 }
 
 // Picks a tile shape: the big tile when it still yields >= 2 workgroups per CU.
-// PSMMA_F64_CFG (0..3) forces one, for tuning sweeps.
+// PSMMA_F64_CFG (0..4) forces one, for tuning sweeps.
 int choose_right_cfg(int kind, int M, int N) {
   static int forced = [] {
     const char* e = getenv("PSMMA_F64_CFG");
@@ -567,7 +570,11 @@ int choose_right_cfg(int kind, int M, int N) {
   }();
   if (forced >= 0) return forced;
   long long big = (long long)((M + 127) / 128) * ((N + 127) / 128);
-  if (big >= 512) return (kind == KIND_TOEPLITZ || kind == KIND_HANKEL) ? 3 : 0;
+  const bool win = kind == KIND_TOEPLITZ || kind == KIND_HANKEL;
+  // Cauchy kinds: one 8-wave 256x128 workgroup per CU (measured +5% at N = 16384
+  // over 128x128) once there are >= 4 of them per CU
+  if (!win && big >= 2048) return 4;
+  if (big >= 512) return win ? 3 : 0;
   return 2;
 }
 
