@@ -83,6 +83,8 @@ def _declare(L):
     L.ora_compress_rrqr.restype = I
     L.ora_compress_cauchy.argtypes = [P, P, P, P, D, I, I, P]
     L.ora_compress_cauchy.restype = I
+    L.ora_compress_cauchyeig2.argtypes = [P, P, P, P, P, P, P, D, I, I, P, P]
+    L.ora_compress_cauchyeig2.restype = I
 
 
 def _f64(x):
@@ -285,6 +287,33 @@ def compress_cauchy(d, f, u, v, tol: float = RRLU_TOL, h_init=None):
     x = h[:m * rk].reshape((m, rk), order="F").copy()
     y = h[m * rk:m * rk + rk * n].reshape((rk, n), order="F").copy()
     return rk, x, y, h
+
+
+def compress_cauchyeig2(fpole, difl, difr, u, v, rows, cols, tol: float = RRLU_TOL):
+    """SRC/CauchyHssEig.f90:2640-2829: row ID of the eigen-Cauchy block with global
+    0-based rows / cols and the global generators u, v (gathered here, as
+    ConstCauchyEiglowrank does).  Returns (rk, Q (m x rk), Y (rk x n), nswap); at
+    full rank Q is None and Y is the explicit block."""
+    ff, dl, dr, u, v = (np.ascontiguousarray(x, dtype=np.float64)
+                        for x in (fpole, difl, difr, u, v))
+    rows = np.asarray(rows, np.int64)
+    cols = np.asarray(cols, np.int64)
+    m, n = len(rows), len(cols)
+    pl = np.ascontiguousarray(rows + 1, np.int32)
+    pu = np.ascontiguousarray(cols + 1, np.int32)
+    ug = np.ascontiguousarray(u[rows])
+    vg = np.ascontiguousarray(v[cols])
+    h = np.zeros(max(1, 2 * m * n))
+    nsw = np.zeros(1, np.int32)
+    rk = lib().ora_compress_cauchyeig2(_ptr(ff), _ptr(dl), _ptr(dr), _ptr(ug), _ptr(vg),
+                                       _ptr(pl), _ptr(pu), tol, m, n, _ptr(h), _ptr(nsw))
+    if rk < 0:
+        raise RuntimeError("ora_compress_cauchyeig2 failed")
+    if rk == min(m, n):
+        return rk, None, h[:m * n].reshape((m, n), order="F").copy(), int(nsw[0])
+    q = h[:m * rk].reshape((m, rk), order="F").copy()
+    y = h[m * rk:m * rk + rk * n].reshape((rk, n), order="F").copy()
+    return rk, q, y, int(nsw[0])
 
 
 def rel_fro(x, ref) -> float:

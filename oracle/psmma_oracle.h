@@ -59,4 +59,13 @@ int ora_rrlu_cauchy(double* D, double* F, double* A, double* V, double tol, doub
 int ora_compress_cauchy(const double* D, const double* F, const double* U, const double* V,
                         double tol, int M, int N, double* H);
 int ora_compress_rrqr(int m, int n, double* a, int lda, double* h, double tol);
+/* rrluCauchyEigrow (SRC/CauchyHssEig.f90:7-212): returns Rk. */
+int ora_rrlu_cauchyeig_row(const double* ff, const double* difl, const double* difr, double* F,
+                           double* A, double* V, double tol, double* U, double* W, int* PL,
+                           int* PU, int* PQ, int M, int N, int* nswap);
+/* Compress_cauchyEig2 (SRC/CauchyHssEig.f90:2640-2829): returns Rk; h = [Q | Y]
+ * when Rk < min(M, N), the explicit block otherwise. */
+int ora_compress_cauchyeig2(const double* ff, const double* difl, const double* difr,
+                            const double* u, const double* v, const int* pl, const int* pu,
+                            double tol, int M, int N, double* H, int* nswap);
 #endif

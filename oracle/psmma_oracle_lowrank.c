@@ -425,6 +425,306 @@ int ora_compress_cauchy(const double* D, const double* F, const double* U, const
   return rk;
 }
 
+/* ------------------------------------------------------------------------- */
+/* Generator-only RRLU of an eigenvector (eigen-Cauchy) block
+ * (SRC/CauchyHssEig.f90), restated with 0-based positions.  The row and column
+ * poles are never formed as differences of stored values: every F(i) - D(j) and
+ * D(i) - D(j) is rebuilt from the global old poles FF and the DIFL / DIFR gaps
+ * through the global indices PL(i) / PU(j) (1-based, as in the reference), so
+ * the block entry is U_i V_j / fd(PL_i, PU_j).  The column poles D themselves
+ * are only permuted by the reference, never read, and are not carried here. */
+
+/* computfd (SRC/CauchyHssEig.f90:519-568): F(row) - D(col) from global indices. */
+static double eig_fd(const double* ff, const double* difl, const double* difr, int pl, int pu) {
+  if (pl <= pu) return (ff[pl - 1] - ff[pu - 1]) - difl[pu - 1];
+  return (ff[pl - 1] - ff[pu]) - difr[pu - 1];
+}
+
+/* computdd (SRC/CauchyHssEig.f90:470-516): D(j) - D(k) from global indices. */
+static double eig_dd(const double* ff, const double* difl, const double* difr, int pj, int pk) {
+  if (pj < pk) return (ff[pj] - ff[pk - 1]) + difr[pj - 1] - difl[pk - 1];
+  return (ff[pj - 1] - ff[pk]) + difl[pj - 1] - difr[pk - 1];
+}
+
+/* CauchyMaxEigmc (SRC/CauchyHssEig.f90:1507-1589): largest |u_i / fd(PL_i, pu)|
+ * down one column over n rows, first index of the maximum. */
+static double eig_max_col(const double* ff, const double* difl, const double* difr,
+                          const int* pl, int pu, const double* u, int n, int* at) {
+  double best = -1.79769313486231570815e+308;
+  *at = -1;
+  for (int i = 0; i < n; ++i) {
+    double x = fabs(u[i] / eig_fd(ff, difl, difr, pl[i], pu));
+    if (*at < 0 || x > best) {
+      best = x;
+      *at = i;
+    }
+  }
+  return best;
+}
+
+/* CauchyMaxEigm (SRC/CauchyHssEig.f90:1419-1503): largest |v_j / fd(pl, PU_j)|
+ * along one row over n columns. */
+static double eig_max_row(const double* ff, const double* difl, const double* difr, int pl,
+                          const int* pu, const double* v, int n, int* at) {
+  double best = -1.79769313486231570815e+308;
+  *at = -1;
+  for (int j = 0; j < n; ++j) {
+    double x = fabs(v[j] / eig_fd(ff, difl, difr, pl, pu[j]));
+    if (*at < 0 || x > best) {
+      best = x;
+      *at = j;
+    }
+  }
+  return best;
+}
+
+typedef struct {
+  const double *ff, *difl, *difr;
+  double *F, *U, *V, *A;
+  int *PL, *PU, *PQ;
+  int M, N;
+} eig_rrlu;
+
+static void eig_swap_rows(eig_rrlu* e, int i, int j) {
+  dswp(e->F, i, j);
+  dswp(e->U, i, j);
+  dswp(e->A, i, j);
+  iswp(e->PL, i, j);
+  iswp(e->PQ, i, j);
+}
+static void eig_swap_cols(eig_rrlu* e, int i, int j) {
+  dswp(e->V, i, j);
+  iswp(e->PU, i, j);
+}
+
+/* CauchyPivtEig (SRC/CauchyHssEig.f90:571-717): alternating row/column pivot
+ * search from position k; as in CauchyPivtG the CONTINUE after a row swap
+ * falls through to a column swap.  The early-return test's pivot magnitude is
+ * written F(PL(k)) in the reference (:675-678), i.e. the local row-pole array indexed by
+ * a global row index (past its end for every block row after the first); the
+ * restatement uses the value of the surrounding terms' convention, FF(PL(k)),
+ * which is the pivot entry itself.  An iteration cap guards against cycling
+ * on ties. */
+static void eig_pivt(eig_rrlu* e, int k) {
+  const int M = e->M, N = e->N;
+  int jjL, jjU;
+  double junkL = eig_max_col(e->ff, e->difl, e->difr, e->PL + k, e->PU[k], e->U + k, M - k, &jjL);
+  double junkU = eig_max_row(e->ff, e->difl, e->difr, e->PL[k], e->PU + k, e->V + k, N - k, &jjU);
+  junkL = junkL * fabs(e->V[k]);
+  junkU = junkU * fabs(e->U[k]);
+  double piv = fabs(e->U[k] * e->V[k] / eig_fd(e->ff, e->difl, e->difr, e->PL[k], e->PU[k]));
+  if (junkL <= piv && junkU <= piv) return;
+  int flg = (junkL > junkU) ? 1 : 0;
+  for (long it = 0; it < 4L * (M + N) + 16; ++it) {
+    if (flg == 1) {
+      eig_swap_rows(e, k, jjL + k);
+      eig_max_row(e->ff, e->difl, e->difr, e->PL[k], e->PU + k, e->V + k, N - k, &jjU);
+      if (jjU == 0) return;
+      flg = 0;
+    }
+    eig_swap_cols(e, k, jjU + k);
+    eig_max_col(e->ff, e->difl, e->difr, e->PL + k, e->PU[k], e->U + k, M - k, &jjL);
+    if (jjL == 0) return;
+    flg = 1;
+  }
+}
+
+/* CauchyPivtEig_CP (SRC/CauchyHssEig.f90:1004-1134): complete pivoting on the
+ * k-th Schur complement. */
+static void eig_pivt_cp(eig_rrlu* e, int k) {
+  int ii = 0, jj = 0;
+  double junk = 0.0;
+  for (int j = 0; j < e->N - k; ++j) {
+    int at;
+    double x = eig_max_col(e->ff, e->difl, e->difr, e->PL + k, e->PU[k + j], e->U + k, e->M - k,
+                           &at);
+    x = fabs(x * e->V[k + j]);
+    if (j == 0 || x > junk) {
+      junk = x;
+      ii = at;
+      jj = j;
+    }
+  }
+  eig_swap_rows(e, k, ii + k);
+  eig_swap_cols(e, k, jj + k);
+}
+
+/* rrluCauchyEigrow (SRC/CauchyHssEig.f90:7-212).  F: the block's row poles
+ * FF(PL_i); A: its row generators (both permuted with the rows); V: column
+ * generators (permuted and turned into Schur-complement generators); U, W:
+ * Schur-complement and interpolation generators.  PL / PU: global 1-based
+ * row / column indices, permuted.  PQ: the row permutation (1-based).
+ * Returns Rk; *nswap counts the swap passes that moved a row. */
+int ora_rrlu_cauchyeig_row(const double* ff, const double* difl, const double* difr, double* F,
+                           double* A, double* V, double tol, double* U, double* W, int* PL,
+                           int* PU, int* PQ, int M, int N, int* nswap) {
+  eig_rrlu e = {ff, difl, difr, F, U, V, A, PL, PU, PQ, M, N};
+#define FD(j, k) eig_fd(ff, difl, difr, PL[j], PU[k])
+  for (int i = 0; i < M; ++i) U[i] = A[i];
+  for (int i = 0; i < M; ++i) PQ[i] = i + 1;
+  int rk = M < N ? M : N;
+  const int mn = rk;
+  int swaps = 0;
+  double amax = 0.0;
+  const double rho = 1.2;
+  const int prd = 5;
+  for (int k = 0; k < mn; ++k) { /* the reference's k = this k + 1 */
+    eig_pivt(&e, k);
+    double temp = FD(k, k);
+    double ukk = U[k] * V[k] / temp;
+    amax = fmax(amax, fabs(ukk));
+    if (fabs(ukk) < tol * amax) {
+      eig_pivt_cp(&e, k);
+      temp = FD(k, k);
+      ukk = U[k] * V[k] / temp;
+      amax = fmax(amax, fabs(ukk));
+      if (fabs(ukk) < tol * amax) {
+        rk = k;
+        break;
+      }
+    }
+    for (int j = k + 1; j < M; ++j) {
+      temp = FD(j, k);
+      temp = (F[j] - F[k]) / temp;
+      U[j] = U[j] * temp;
+    }
+    for (int j = k + 1; j < N; ++j) {
+      temp = eig_dd(ff, difl, difr, PU[j], PU[k]);
+      double temp2 = -FD(k, j);
+      temp = temp / temp2;
+      V[j] = V[j] * temp;
+    }
+    for (int j = 0; j < k; ++j) {
+      temp = -FD(j, k);
+      W[j] = W[j] * (temp / (F[k] - F[j]));
+    }
+    temp = FD(k, k);
+    W[k] = temp / A[k];
+    for (int j = 0; j < k; ++j) {
+      temp = -FD(k, j);
+      W[k] = W[k] * (temp / (F[j] - F[k]));
+    }
+    /* the swap pass runs every prd-th step (flgl = mod(k, prd) < 1) */
+    int again = ((k + 1) % prd) < 1;
+    while (again) {
+      again = 0;
+      /* searchMax2Eig (SRC/CauchyHssEig.f90:1137-1203) over the k+1 chosen rows */
+      int ii = 0, jjl;
+      double nmax = cauchy_max(F + k + 1, F[0], U + k + 1, M - k - 1, &jjl) * fabs(W[0]);
+      int jj = jjl;
+      for (int j = 1; j <= k; ++j) {
+        double x = cauchy_max(F + k + 1, F[j], U + k + 1, M - k - 1, &jjl) * fabs(W[j]);
+        if (x > nmax) {
+          nmax = x;
+          ii = j;
+          jj = jjl;
+        }
+      }
+      if (nmax > rho) {
+        swaps++;
+        again = 1;
+        jj = jj + k + 1; /* 0-based row of the Schur complement */
+        for (int j = k + 1; j < N; ++j) {
+          temp = -FD(ii, j);
+          double temp2 = -FD(jj, j);
+          temp = temp / temp2;
+          V[j] = V[j] * temp;
+        }
+        for (int i = k + 1; i < jj; ++i) U[i] = U[i] * ((F[i] - F[jj]) / (F[i] - F[ii]));
+        for (int i = jj + 1; i < M; ++i) U[i] = U[i] * ((F[i] - F[jj]) / (F[i] - F[ii]));
+        for (int j = 0; j < ii; ++j) W[j] = W[j] * ((F[j] - F[ii]) / (F[j] - F[jj]));
+        for (int j = ii + 1; j <= k; ++j) W[j] = W[j] * ((F[j] - F[ii]) / (F[j] - F[jj]));
+        temp = FD(ii, ii);
+        U[jj] = A[ii] * ((F[ii] - F[jj]) / temp);
+        temp = FD(jj, ii);
+        W[ii] = temp / A[jj];
+        for (int j = 0; j <= k; ++j) {
+          if (j == ii) continue;
+          temp = FD(ii, j);
+          U[jj] = U[jj] * ((F[ii] - F[j]) / temp);
+          temp = -FD(jj, j);
+          W[ii] = W[ii] * (temp / (F[j] - F[jj]));
+        }
+        iswp(PL, ii, jj);
+        iswp(PQ, ii, jj);
+        dswp(F, ii, jj);
+        dswp(A, ii, jj);
+      }
+    }
+  }
+#undef FD
+  if (nswap) *nswap = swaps;
+  return rk;
+}
+
+/* Compress_cauchyEig2 (SRC/CauchyHssEig.f90:2640-2829): row ID of the M x N
+ * eigen-Cauchy block with global rows PL / columns PU and gathered generators
+ * u (M) / v (N).  When Rk < min(M, N): H = [Q (M x Rk) | Y (Rk x N)],
+ * Q = P^-1 [I; Z_i W_j / (F_i - F_j)] (CauchylikeEigQ, :1882-1952) and Y the
+ * chosen rows of the block rebuilt from the original u / v (CauchylikeEig,
+ * :1693-1774); otherwise H is the explicit block.  Returns Rk. */
+int ora_compress_cauchyeig2(const double* ff, const double* difl, const double* difr,
+                            const double* u, const double* v, const int* pl, const int* pu,
+                            double tol, int M, int N, double* H, int* nswap) {
+  int mn = M < N ? M : N;
+  double* F = (double*)malloc(sizeof(double) * (size_t)(M + 1));
+  double* A = (double*)malloc(sizeof(double) * (size_t)(M + 1));
+  double* V = (double*)malloc(sizeof(double) * (size_t)(N + 1));
+  double* Z = (double*)calloc((size_t)(M + 1), sizeof(double));
+  double* W = (double*)calloc((size_t)(mn + 1), sizeof(double));
+  int* PL = (int*)malloc(sizeof(int) * (size_t)(M + 1));
+  int* PU = (int*)malloc(sizeof(int) * (size_t)(N + 1));
+  int* PQ = (int*)malloc(sizeof(int) * (size_t)(M + 1));
+  int* IP = (int*)malloc(sizeof(int) * (size_t)(M + 1));
+  int rk = -1;
+  if (F && A && V && Z && W && PL && PU && PQ && IP) {
+    for (int i = 0; i < M; ++i) {
+      F[i] = ff[pl[i] - 1];
+      A[i] = u[i];
+      PL[i] = pl[i];
+    }
+    for (int j = 0; j < N; ++j) {
+      V[j] = v[j];
+      PU[j] = pu[j];
+    }
+    rk = ora_rrlu_cauchyeig_row(ff, difl, difr, F, A, V, tol, Z, W, PL, PU, PQ, M, N, nswap);
+    for (int i = 0; i < M; ++i) IP[PQ[i] - 1] = i; /* invp */
+    if (rk < mn) {
+      double* Q = H;
+      for (int j = 0; j < rk; ++j)
+        for (int i = 0; i < M; ++i) {
+          const int p = IP[i]; /* row of the pivoted order that is original row i */
+          double x;
+          if (p < rk)
+            x = (p == j) ? 1.0 : 0.0;
+          else
+            x = Z[p] * W[j] / (F[p] - F[j]);
+          Q[(size_t)j * M + i] = x;
+        }
+      double* Y = H + (size_t)M * rk;
+      for (int j = 0; j < N; ++j)
+        for (int i = 0; i < rk; ++i) {
+          const int pj = pu[j], pi = PL[i];
+          Y[(size_t)j * rk + i] = A[i] * v[j] / eig_fd(ff, difl, difr, pi, pj);
+        }
+    } else {
+      for (int j = 0; j < N; ++j)
+        for (int i = 0; i < M; ++i)
+          H[(size_t)j * M + i] = u[i] * v[j] / eig_fd(ff, difl, difr, pl[i], pu[j]);
+    }
+  }
+  free(F);
+  free(A);
+  free(V);
+  free(Z);
+  free(W);
+  free(PL);
+  free(PU);
+  free(PQ);
+  free(IP);
+  return rk;
+}
+
 /* Compress_RRQR's TOL is the default-real literal 1.0E-13 stored into a DOUBLE
  * PRECISION variable (SRC/Auxil_2p5D.f90:1140): its value is the float nearest
  * to 1e-13. */
@@ -432,10 +732,28 @@ int ora_compress_cauchy(const double* D, const double* F, const double* U, const
 
 /* Off-diagonal ring step of the faithful mode.  Cauchy: the generator RRLU
  * above.  Toeplitz: ConstToeplitzlowrank (explicit tile -> Compress_RRQR,
- * SRC/Auxil_2p5D.f90:1050-1056; full rank -> the explicit tile again).  The
- * eigen-Cauchy kind's RRLU (rrluCauchyEigrow) is not restated: it reports -2. */
+ * SRC/Auxil_2p5D.f90:1050-1056; full rank -> the explicit tile again).
+ * Eigen-Cauchy: ConstCauchyEiglowrank (SRC/Auxil_2p5D.f90:642-748) ->
+ * Compress_cauchyEig2, which rebuilds the explicit tile at full rank. */
 int ora_lowrank_tile(const ora_problem* p, int kbp, int nq, const int* ridx, const int* cidx,
                      double* hbuf, unsigned long hcap, int* rk) {
+  if (p->kind == ORA_CAUCHYEIG) {
+    /* generators gathered by the index sets (:731-735), TOL = the default-real
+     * literal 1.0E-17 (:729); gen = [F D U V DIFL DIFR] with stride ldgen */
+    if ((unsigned long)kbp * (unsigned long)nq * 2ul > hcap) return -3;
+    const size_t L = (size_t)p->ldgen;
+    const double* g = p->gen;
+    double* uv = (double*)malloc(sizeof(double) * (size_t)(kbp + nq + 2));
+    if (!uv) return -1;
+    for (int i = 0; i < kbp; ++i) uv[i] = g[2 * L + ridx[i] - 1];
+    for (int j = 0; j < nq; ++j) uv[kbp + j] = g[3 * L + cidx[j] - 1];
+    int r = ora_compress_cauchyeig2(g, g + 4 * L, g + 5 * L, uv, uv + kbp, ridx, cidx,
+                                    (double)1.0e-17f, kbp, nq, hbuf, NULL);
+    free(uv);
+    if (r < 0) return -1;
+    *rk = r;
+    return ORA_TILE_OK;
+  }
   if (p->kind == ORA_CAUCHY) {
     /* ConstCauchylowrank (SRC/Auxil_2p5D.f90:876-963): generators gathered by
      * the index sets, compress_cauchy('R') at the default-real literal

@@ -6,7 +6,14 @@ reconstructing the block to rounding level; numerically low-rank blocks
 (separated poles) reconstructed within the tolerance; the swap pass (every
 10th step) exercised; the reference's tolerance literal; full rank leaving H
 unwritten; and, through the ring, the reference's stale-tile product on a 2 x 2
-grid recomputed independently in numpy."""
+grid recomputed independently in numpy.
+
+The eigen-Cauchy RRLU (rrluCauchyEigrow / Compress_cauchyEig2 restatement) is
+pinned on blocks of the eigenvector matrix of a rank-one update (orthogonal to
+rounding, an independent check of the tile formula): ranks against the singular
+values, reconstruction at each tolerance, exact deficiency from zero rows, the
+swap pass, the explicit block at full rank, and the faithful ring against the
+dense product."""
 import numpy as np
 import pytest
 
@@ -123,3 +130,94 @@ def test_faithful_cauchy_ring_low_rank_branch(ora):
     got, st = ora.ps_compute(ora.CAUCHY, 2, 2, nb, 1.0, a, b, mode=ora.MODE_FAITHFUL)
     assert st[1] > 0
     assert ora.rel_fro(got, a @ h) < 1e-12
+
+
+# ---------------------------------------------------------------------------
+# Eigen-Cauchy blocks (rrluCauchyEigrow / Compress_cauchyEig2).
+
+def _eig(n, seed):
+    e = inputs.config3(n, seed=seed, with_q=False)
+    f, dl, dr = e["f"], e["difl"], e["difr"]
+    zh = inputs.loewner_z(f, e["u"], dl, dr)
+    v = inputs.column_normalisers(f, zh, dl, dr)
+    return f, dl, dr, zh, v
+
+
+def test_eig_rrlu_blocks_of_an_orthogonal_matrix(ora):
+    n = 200
+    f, dl, dr, zh, v = _eig(n, 5)
+    c = ora.cauchyeig_matrix(f, zh, v, dl, dr, 0, n, 0, n)
+    assert np.max(np.abs(c.T @ c - np.eye(n))) < 1e-13
+    # interlaced rows / columns: numerically full rank, the explicit block comes back
+    rows, cols = np.arange(0, 120, 2), np.arange(1, 120, 2)
+    rk, q, y, _ = ora.compress_cauchyeig2(f, dl, dr, zh, v, rows, cols)
+    assert rk == 60 and q is None
+    assert np.max(np.abs(y - c[np.ix_(rows, cols)]) / np.abs(c[np.ix_(rows, cols)])) < 1e-15
+
+
+@pytest.mark.parametrize("tol", [1e-6, 1e-10, 1e-14, float(np.float32(1e-17))])
+def test_eig_rrlu_separated_blocks_within_tolerance(ora, tol):
+    n = 200
+    f, dl, dr, zh, v = _eig(n, 5)
+    rows, cols = np.arange(0, 80), np.arange(120, 200)
+    blk = ora.cauchyeig_matrix(f, zh, v, dl, dr, 0, n, 0, n)[np.ix_(rows, cols)]
+    rk, q, y, _ = ora.compress_cauchyeig2(f, dl, dr, zh, v, rows, cols, tol)
+    sv = np.linalg.svd(blk, compute_uv=False)
+    assert 0 < rk < 80
+    assert int(np.sum(sv > 10 * tol * sv[0])) <= rk <= int(np.sum(sv > 0.01 * tol * sv[0])) + 2
+    assert np.linalg.norm(blk - q @ y, 2) <= 1e3 * tol * sv[0] + 1e-15 * sv[0]
+    # a row ID: Q holds an identity on the chosen rows, Y is those rows of the block
+    unit = np.all((q == 0.0) | (q == 1.0), axis=1) & (np.sum(q == 1.0, axis=1) == 1)
+    chosen = np.flatnonzero(unit)
+    assert len(chosen) == rk
+    order = np.argmax(q[chosen], axis=1)
+    assert np.array_equal(y[order], blk[chosen])
+
+
+def test_eig_rrlu_exact_rank_deficiency_and_swap_pass(ora):
+    n = 200
+    f, dl, dr, zh, v = _eig(n, 5)
+    rows, cols = np.arange(0, 120, 2), np.arange(1, 120, 2)
+    u = zh.copy()
+    zero = np.random.default_rng(0).choice(rows, 12, replace=False)
+    u[zero] = 0.0
+    rk, q, y, nswap = ora.compress_cauchyeig2(f, dl, dr, u, v, rows, cols)
+    blk = ora.cauchyeig_matrix(f, u, v, dl, dr, 0, n, 0, n)[np.ix_(rows, cols)]
+    assert rk == 48 and nswap > 0  # the swap pass of step 5 moved a row
+    assert ora.rel_fro(q @ y, blk) < 1e-14
+    assert np.all(q[np.searchsorted(rows, zero)] == 0.0)
+
+
+def test_eig_rrlu_block_cyclic_tiles(ora):
+    """The index sets of an off-diagonal ring step (block-cyclic, disjoint)."""
+    n = 200
+    f, dl, dr, zh, v = _eig(n, 5)
+    c = ora.cauchyeig_matrix(f, zh, v, dl, dr, 0, n, 0, n)
+    swaps = 0
+    for nb in (4, 8, 16):
+        rows = ora.constindex(n, nb, 0, 4) - 1
+        cols = ora.constindex(n, nb, 1, 4) - 1
+        blk = c[np.ix_(rows, cols)]
+        rk, q, y, nsw = ora.compress_cauchyeig2(f, dl, dr, zh, v, rows, cols)
+        swaps += nsw
+        sv = np.linalg.svd(blk, compute_uv=False)
+        assert rk >= int(np.sum(sv > 1e-15 * sv[0]))
+        got = y if q is None else q @ y
+        assert np.linalg.norm(blk - got, 2) <= 1e-14 * sv[0]
+    assert swaps > 0
+
+
+@pytest.mark.parametrize("P,Q,nb", [(1, 4, 16), (2, 2, 8), (1, 8, 4), (2, 4, 16)])
+def test_faithful_eig_ring_matches_dense(ora, P, Q, nb):
+    """The eigen-Cauchy ring's low-rank branch (compressed steps, or the explicit
+    tile rebuilt at full rank: no stale tile for this kind) gives the dense product."""
+    n = 200
+    f, dl, dr, zh, v = _eig(n, 5)
+    gen = np.asfortranarray(np.stack([f, f + dl, zh, v, dl, dr], axis=1))
+    a = np.random.default_rng(9).standard_normal((n, n))
+    c = ora.cauchyeig_matrix(f, zh, v, dl, dr, 0, n, 0, n)
+    dense, _ = ora.ps_compute(ora.CAUCHYEIG, P, Q, nb, 1.0, a, gen, mode=ora.MODE_DENSE)
+    faith, st = ora.ps_compute(ora.CAUCHYEIG, P, Q, nb, 1.0, a, gen, mode=ora.MODE_FAITHFUL)
+    assert st[1] + st[2] > 0 and st[3] == 0
+    assert ora.rel_fro(dense, a @ c) < 1e-14
+    assert ora.rel_fro(faith, dense) < 1e-14
