@@ -268,6 +268,26 @@ def batched_config(seed, torch, api, inputs):
             h = gens[0][idx[:, None] + idx[None, :]]
         refs.append(a @ h)
     e64 = [rel(c, r) for (_, _, _, c), r in zip(keep, refs)]
+
+    # the configuration's stated comparison: a per-problem loop of the plain FP64
+    # kernel, eager (one C-ABI call per problem) and captured once in a hipGraph
+    def loop():
+        for kind, a, gens, c in keep:
+            n = c.shape[0]
+            if kind == 0:
+                api.cauchy_dev(1.0, a, *gens, n, c)
+            elif kind == 1:
+                api.toeplitz_dev(1.0, a, gens[0], n, n, c)
+            else:
+                api.hankel_dev(1.0, a, gens[0], n, c)
+
+    ms_loop = ev_time(loop, 3, torch)
+    eloop = max(rel(c, r) for (_, _, _, c), r in zip(keep, refs))
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(graph):
+        loop()
+    ms_loop_graph = ev_time(graph.replay, 5, torch)
+    del graph
     plans = api.BatchedPlan(probs, prec=api.PREC_SPLIT_I8, slices=0)
     mss = ev_time(plans.launch, 5, torch)
     es = [rel(c, r) for (_, _, _, c), r in zip(keep, refs)]
@@ -277,8 +297,11 @@ def batched_config(seed, torch, api, inputs):
     return {"problems": 512, "flops": flops, "fp64_grouped_ms": round(ms64, 3),
             "fp64_tflops": round(flops / ms64 / 1e9, 2), "fp64_tile_cfg": cfg,
             "fp64_err_max": max(e64),
+            "fp64_loop_ms": round(ms_loop, 3), "fp64_loop_graph_ms": round(ms_loop_graph, 3),
+            "fp64_loop_err_max": eloop,
             "split_i8_ms": round(mss, 3), "split_tflops": round(flops / mss / 1e9, 2),
             "split_speedup_vs_fp64": round(ms64 / mss, 3),
+            "split_speedup_vs_fp64_loop": round(min(ms_loop, ms_loop_graph) / mss, 3),
             "split_err": {"median": q(es, 0.5), "p90": q(es, 0.9), "max": max(es)}}
 
 

@@ -57,6 +57,10 @@ struct CrtParams {
   double inv_lo[kMaxModuli];
   double M_hi, M_lo;          // M = prod m_l as a double-double
   double rm[kMaxModuli];      // 1/m_l rounded (for the residue quotient)
+  // y_l / m_l = wa + wb + wc: wa on the 2^-41 grid and wb on the 2^-81 grid, so
+  // that a residue (< 2^8) times either is exact and sums of up to 16 such
+  // products stay exact in FP64; wc is the rounded rest (< 2^-81).
+  double wa[kMaxModuli], wb[kMaxModuli], wc[kMaxModuli];
   unsigned char minv[kMaxModuli][kMaxModuli];  // minv[j][l] = m_j^-1 mod m_l (j < l)
 };
 
@@ -89,6 +93,15 @@ static CrtParams make_crt(int t) {
     p.inv_hi[l] = hi;
     p.inv_lo[l] = lo;
     p.rm[l] = 1.0 / m;
+    {
+      const unsigned long long num = (unsigned long long)p.y[l] << 41;  // < 2^49
+      const unsigned long long wa = num / (unsigned)m, r1 = num % (unsigned)m;
+      const unsigned long long num2 = r1 << 40;  // < 2^48
+      const unsigned long long wb = num2 / (unsigned)m, r2 = num2 % (unsigned)m;
+      p.wa[l] = std::ldexp((double)wa, -41);
+      p.wb[l] = std::ldexp((double)wb, -81);
+      p.wc[l] = std::ldexp((double)r2 / m, -81);
+    }
     for (int j = 0; j < l; ++j) p.minv[j][l] = (unsigned char)inv_mod(kModuli[j] % m, m);
   }
   // M as hi + lo (M < 2^128): split the 128-bit integer exactly.
@@ -338,7 +351,8 @@ __global__ void gen_residue_kernel(const double* __restrict__ g, long long lo, l
 
 // ---------------------------------------------------------------------------
 // int8 modular GEMM, one modulus per blockIdx.y:
-//   R_l = (Ares_l * B_l) mod m_l, stored as uint8, column-major M x N.
+//   R_l = (Ares_l * B_l) mod m_l, stored as uint8, column-major with leading
+//   dimension ldo = M rounded up to 4 (every lane stores whole dwords).
 // Ares_l: M x kp row-major int8.  B_l is either streamed (BMODE_DENSE: kp x N
 // column-major residues) or assembled from a generator window (BMODE_TOE /
 // BMODE_HANK: residues R_l[u] of the scaled generator, u = k + (N-1-j) or
@@ -352,12 +366,13 @@ __global__ void gen_residue_kernel(const double* __restrict__ g, long long lo, l
 // A[l%16][16*(l/16) .. +15], B lane l supplies B[16*(l/16) .. +15][l%16],
 // D lane l element r holds D[4*(l/16)+r][l%16].
 enum { BMODE_DENSE = 0, BMODE_TOE = 1, BMODE_HANK = 2 };
+constexpr int kFastModK = 16384;  // longest padded contraction of the FP32 reduction mod m
 
 template <int BMODE, int BM, int BN, int BK, int WGM, int WGN, int KMAX, bool STAGE2 = true>
 __device__ __forceinline__ void gemm_i8_body(
     int M, int N, int kp, const signed char* __restrict__ Ares, const signed char* __restrict__ Bres,
     size_t aplane, size_t bplane, int lpad, const CrtParams& cp, unsigned char* __restrict__ out,
-    size_t oplane, int bmi, int bni, int l) {
+    size_t oplane, int ldo, int bmi, int bni, int l) {
   constexpr bool WIN = BMODE != BMODE_DENSE;
   constexpr int NT = WGM * WGN * 64;
   constexpr int WM = BM / WGM, WN = BN / WGN;
@@ -567,26 +582,37 @@ __device__ __forceinline__ void gemm_i8_body(
   }
   }
 
+  // Reduction mod m.  Residues lie in [-128, 127], so |acc| <= kp * 2^14; for
+  // kp <= kFastModK that is <= 2^28, where q = floor(acc * fl(1/m)) in FP32 is
+  // within one of the true quotient (relative error <= 3 * 2^-24, quotient
+  // < 2^21) and |q| < 2^23 makes the 24-bit multiply exact: one correction on
+  // each side gives the residue in [0, m).  Longer contractions use %.
+  const bool fastmod = kp <= kFastModK;
+  const float rmf = 1.0f / (float)m;
+  auto modm = [&](int x) -> unsigned {
+    int y;
+    if (fastmod) {
+      const int q = (int)floorf((float)x * rmf);
+      y = x - __mul24(q, m);
+      y += (y < 0) ? m : 0;
+      y -= (y >= m) ? m : 0;
+    } else {
+      y = x % m;
+      if (y < 0) y += m;
+    }
+    return (unsigned)y;
+  };
 #pragma unroll
   for (int i = 0; i < TM; ++i)
 #pragma unroll
     for (int j = 0; j < TN; ++j) {
       int gc = n0 + wn0 + j * 16 + lr;
-      int gr = m0 + wm0 + i * 16 + lg * 4;
+      int gr = m0 + wm0 + i * 16 + lg * 4;  // a multiple of 4: gr + 3 < ldo when gr < M
       if (gc >= N || gr >= M) continue;
       unsigned word = 0;
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        int x = acc[i][j][r] % m;
-        if (x < 0) x += m;
-        word |= (unsigned)x << (8 * r);
-      }
-      unsigned char* dst = Ol + (size_t)gc * M + gr;
-      if (gr + 3 < M && ((M & 3) == 0)) {
-        *(unsigned*)dst = word;
-      } else {
-        for (int r = 0; r < 4 && gr + r < M; ++r) dst[r] = (unsigned char)(word >> (8 * r));
-      }
+      for (int r = 0; r < 4; ++r) word |= modm(acc[i][j][r]) << (8 * r);
+      *(unsigned*)(Ol + (size_t)gc * ldo + gr) = word;
     }
 }
 
@@ -596,12 +622,12 @@ template <int BMODE, int BM, int BN, int BK, int WGM, int WGN, int MINB, int KMA
 __global__ __launch_bounds__(WGM* WGN * 64, MINB) void gemm_i8_kernel(
     int M, int N, int kp, const signed char* __restrict__ Ares, const signed char* __restrict__ Bres,
     size_t aplane, size_t bplane, int lpad, CrtParams cp, unsigned char* __restrict__ out,
-    size_t oplane) {
+    size_t oplane, int ldo) {
   const int nbn = (N + BN - 1) / BN, nbm = (M + BM - 1) / BM;
   int bmi, bni;
   patch_remap<4, 8>(blockIdx.x, nbm, nbn, bmi, bni);
   gemm_i8_body<BMODE, BM, BN, BK, WGM, WGN, KMAX, STAGE2>(M, N, kp, Ares, Bres, aplane, bplane,
-                                                          lpad, cp, out, oplane, bmi, bni,
+                                                          lpad, cp, out, oplane, ldo, bmi, bni,
                                                           blockIdx.y);
 }
 
@@ -617,10 +643,12 @@ __device__ __forceinline__ void two_sum(double a, double b, double& s, double& e
 
 // CRT recombination + scaling: C(i,j) = alpha * X * 2^-(sa_i + sh_j) + beta * C(i,j).
 //
-// Fast form: X/M = frac(sum_l n_l / m_l), n_l = c_l y_l mod m_l, y_l = (M/m_l)^-1
-// mod m_l, summed in double-double.  The fraction is known to ~2^-100 absolute,
-// so it gives X to full FP64 precision whenever |X/M| >= 2^-40.
-// Exact form (taken only for the rare |X/M| < 2^-40): Garner's mixed-radix
+// Fast form: X/M = frac(sum_l r_l y_l / m_l) with r_l the stored residue and
+// y_l = (M/m_l)^-1 mod m_l.  With y_l / m_l = wa_l + wb_l + wc_l (CrtParams),
+// sum r_l wa_l and sum r_l wb_l are formed exactly by FMAs and sum r_l wc_l
+// (< 2^-69) in plain FP64, so the fraction is known to ~2^-106 absolute and
+// gives X to full FP64 precision whenever |X/M| >= 2^-48.
+// Exact form (taken only for the rare |X/M| < 2^-48): Garner's mixed-radix
 // conversion with signed digits d_l in [-m_l/2, m_l/2) (exact small-integer
 // arithmetic, moduli known at compile time), X = d_0 + m_0 (d_1 + m_1 (...)),
 // evaluated by Horner in double-double from the most significant digit, so a
@@ -653,73 +681,91 @@ __device__ __forceinline__ double crt_garner(const unsigned char* __restrict__ r
   return yh + yl;
 }
 
+// Four consecutive rows i0 .. i0+3 (i0 a multiple of 4) of column j: one dword
+// of residues per modulus.  Rows >= M are padding and are not written.
 template <int T>
-__device__ __forceinline__ void crt_elem(int M, const unsigned char* __restrict__ res,
-                                         size_t oplane, const CrtParams& cp,
+__device__ __forceinline__ void crt_quad(int M, const unsigned char* __restrict__ res,
+                                         size_t oplane, int ldo, const CrtParams& cp,
                                          const int* __restrict__ ea, const int* __restrict__ eh,
                                          int abits, int hbits, double alpha, double beta,
-                                         double* __restrict__ C, int ldc, size_t off) {
+                                         double* __restrict__ C, int ldc, int i0, int j) {
 #pragma clang fp contract(off)
-  int i, j;
-  if (off < 0x80000000ull) {  // 32-bit index arithmetic whenever it suffices
-    const unsigned o = (unsigned)off, mu = (unsigned)M;
-    j = (int)(o / mu);
-    i = (int)(o - (unsigned)j * mu);
-  } else {
-    i = (int)(off % M);
-    j = (int)(off / M);
-  }
-  double shi = 0.0, slo = 0.0;
+  const size_t off = (size_t)j * ldo + i0;
+  unsigned w[T];
 #pragma unroll
-  for (int l = 0; l < T; ++l) {
-    const int m = kModuli[l];
-    int nl = ((int)res[l * oplane + off] * cp.y[l]) % m;
-    double p = (double)nl * cp.inv_hi[l];
-    double pe = fma((double)nl, cp.inv_hi[l], -p) + (double)nl * cp.inv_lo[l];
-    double sh, se;
-    two_sum(shi, p, sh, se);
-    shi = sh;
-    slo += se + pe;
+  for (int l = 0; l < T; ++l) w[l] = *(const unsigned*)(res + l * oplane + off);
+  const int s_h = hbits - eh[j];
+  double* cj = C + (size_t)j * ldc;
+  unsigned slow = 0;  // rows that need the exact form
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int i = i0 + r;
+    if (i >= M) break;
+    double sa = 0.0, sb = 0.0, sc = 0.0;
+#pragma unroll
+    for (int l = 0; l < T; ++l) {
+      const double x = (double)((w[l] >> (8 * r)) & 255u);
+      sa = fma(x, cp.wa[l], sa);  // exact
+      sb = fma(x, cp.wb[l], sb);  // exact
+      sc = fma(x, cp.wc[l], sc);
+    }
+    double f, fl;
+    two_sum(sa - rint(sa), sb, f, fl);
+    fl += sc;
+    if (f > 0.5) f -= 1.0;
+    else if (f < -0.5) f += 1.0;
+    if (fabs(f) < 0x1p-48) {
+      slow |= 1u << r;
+      continue;
+    }
+    const double xh = f * cp.M_hi;
+    const double x = xh + (fma(f, cp.M_hi, -xh) + f * cp.M_lo + fl * cp.M_hi);
+    double v = alpha * ldexp(x, -(abits - ea[i] + s_h));
+    if (beta != 0.0) v += beta * cj[i];
+    cj[i] = v;
   }
-  double fh = shi - rint(shi);
-  double f, fl;
-  two_sum(fh, slo, f, fl);
-  if (f > 0.5) f -= 1.0;
-  else if (f < -0.5) f += 1.0;
-  double x;
-  if (fabs(f) >= 0x1p-40) {
-    double xh = f * cp.M_hi;
-    x = xh + (fma(f, cp.M_hi, -xh) + f * cp.M_lo + fl * cp.M_hi);
-  } else {
-    x = crt_garner<T>(res, oplane, off, cp);
+  // one copy of the exact form, looped over the rows that need it
+  while (slow) {
+    const int r = __builtin_ctz(slow);
+    slow &= slow - 1;
+    const int i = i0 + r;
+    const double x = crt_garner<T>(res, oplane, off + r, cp);
+    double v = alpha * ldexp(x, -(abits - ea[i] + s_h));
+    if (beta != 0.0) v += beta * cj[i];
+    cj[i] = v;
   }
-  int s_a = abits - ea[i];
-  int s_h = hbits - eh[j];
-  double v = alpha * ldexp(x, -(s_a + s_h));
-  double* p = C + (size_t)j * ldc + i;
-  if (beta != 0.0) v += beta * (*p);
-  *p = v;
 }
 
+// Thread e of the padded ldo x N residue grid handles rows 4e .. 4e+3 (mod ldo).
 template <int T>
-__device__ __forceinline__ void crt_body(int M, int N, const unsigned char* __restrict__ res,
-                                         size_t oplane, const CrtParams& cp,
+__device__ __forceinline__ void crt_unit(int M, int N, const unsigned char* __restrict__ res,
+                                         size_t oplane, int ldo, const CrtParams& cp,
                                          const int* __restrict__ ea, const int* __restrict__ eh,
                                          int abits, int hbits, double alpha, double beta,
-                                         double* __restrict__ C, int ldc, int bx) {
-  const size_t off = (size_t)bx * blockDim.x + threadIdx.x;
-  if (off >= (size_t)M * N) return;
-  crt_elem<T>(M, res, oplane, cp, ea, eh, abits, hbits, alpha, beta, C, ldc, off);
+                                         double* __restrict__ C, int ldc, size_t e) {
+  const size_t el = e * 4;
+  int i0, j;
+  if (el < 0x80000000ull) {  // 32-bit index arithmetic whenever it suffices
+    const unsigned o = (unsigned)el, lu = (unsigned)ldo;
+    j = (int)(o / lu);
+    i0 = (int)(o - (unsigned)j * lu);
+  } else {
+    j = (int)(el / ldo);
+    i0 = (int)(el - (size_t)j * ldo);
+  }
+  if (j >= N) return;
+  crt_quad<T>(M, res, oplane, ldo, cp, ea, eh, abits, hbits, alpha, beta, C, ldc, i0, j);
 }
 
 template <int T>
 __global__ __launch_bounds__(256) void crt_kernel(int M, int N, const unsigned char* __restrict__ res,
-                                                  size_t oplane, CrtParams cp,
+                                                  size_t oplane, int ldo, CrtParams cp,
                                                   const int* __restrict__ ea,
                                                   const int* __restrict__ eh, int abits, int hbits,
                                                   double alpha, double beta, double* __restrict__ C,
                                                   int ldc) {
-  crt_body<T>(M, N, res, oplane, cp, ea, eh, abits, hbits, alpha, beta, C, ldc, blockIdx.x);
+  crt_unit<T>(M, N, res, oplane, ldo, cp, ea, eh, abits, hbits, alpha, beta, C, ldc,
+              (size_t)blockIdx.x * blockDim.x + threadIdx.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -753,7 +799,8 @@ void* split_workspace(size_t bytes) {
 template <int BMODE>
 static void launch_gemm_i8(int t, int M, int N, int kp, const signed char* ares,
                            const signed char* bres, size_t aplane, size_t bplane, int lpad,
-                           const CrtParams& cp, unsigned char* ores, size_t oplane, hipStream_t s) {
+                           const CrtParams& cp, unsigned char* ores, size_t oplane, int ldo,
+                           hipStream_t s) {
   constexpr int KMAX = 16384;
   static int forced = [] {
     const char* e = getenv("PSMMA_I8_CFG");
@@ -768,34 +815,34 @@ static void launch_gemm_i8(int t, int M, int N, int kp, const signed char* ares,
     case 1:
       hipLaunchKernelGGL((gemm_i8_kernel<BMODE, 128, 256, 128, 2, 2, 1, KMAX>),
                          dim3(nblk(128, 256), t), dim3(256), 0, s, M, N, kp, ares, bres, aplane,
-                         bplane, lpad, cp, ores, oplane);
+                         bplane, lpad, cp, ores, oplane, ldo);
       break;
     case 2:
       hipLaunchKernelGGL((gemm_i8_kernel<BMODE, 256, 128, 128, 2, 2, 1, KMAX>),
                          dim3(nblk(256, 128), t), dim3(256), 0, s, M, N, kp, ares, bres, aplane,
-                         bplane, lpad, cp, ores, oplane);
+                         bplane, lpad, cp, ores, oplane, ldo);
       break;
     case 4:
       hipLaunchKernelGGL((gemm_i8_kernel<BMODE, 256, 256, 128, 2, 2, 1, KMAX, false>),
                          dim3(nblk(256, 256), t), dim3(256), 0, s, M, N, kp, ares, bres, aplane,
-                         bplane, lpad, cp, ores, oplane);
+                         bplane, lpad, cp, ores, oplane, ldo);
       break;
     default:
       hipLaunchKernelGGL((gemm_i8_kernel<BMODE, 128, 128, 128, 2, 2, 2, KMAX>),
                          dim3(nblk(128, 128), t), dim3(256), 0, s, M, N, kp, ares, bres, aplane,
-                         bplane, lpad, cp, ores, oplane);
+                         bplane, lpad, cp, ores, oplane, ldo);
   }
 }
 
-static int finish_crt(int t, int M, int N, const unsigned char* ores, size_t oplane,
+static int finish_crt(int t, int M, int N, const unsigned char* ores, size_t oplane, int ldo,
                       const CrtParams& cp, const int* ea, const int* eh, int abits, double alpha,
                       double beta, double* C, int ldc, hipStream_t s) {
-  size_t nel = (size_t)M * N;
-  dim3 cg((unsigned)((nel + 255) / 256));
+  const size_t nquad = (size_t)ldo * N / 4;
+  dim3 cg((unsigned)((nquad + 255) / 256));
 #define PSMMA_CRT_CASE(TT)                                                                     \
   case TT:                                                                                     \
-    hipLaunchKernelGGL(crt_kernel<TT>, cg, dim3(256), 0, s, M, N, ores, oplane, cp, ea, eh,     \
-                       abits, abits, alpha, beta, C, ldc);                                     \
+    hipLaunchKernelGGL(crt_kernel<TT>, cg, dim3(256), 0, s, M, N, ores, oplane, ldo, cp, ea,    \
+                       eh, abits, abits, alpha, beta, C, ldc);                                 \
     break;
   switch (t) {
     PSMMA_CRT_CASE(2) PSMMA_CRT_CASE(3) PSMMA_CRT_CASE(4) PSMMA_CRT_CASE(5) PSMMA_CRT_CASE(6)
@@ -816,7 +863,8 @@ static int run_split_i8(int t, int M, int N, int K, double alpha, const double* 
   const int abits = choose_abits(t, K);
   if (abits < 8) return -16;
   const int kp = ((K + 255) / 256) * 256;
-  const size_t aplane = (size_t)M * kp, hplane = (size_t)kp * N, oplane = (size_t)M * N;
+  const int ldo = (M + 3) & ~3;
+  const size_t aplane = (size_t)M * kp, hplane = (size_t)kp * N, oplane = (size_t)ldo * N;
   auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
   size_t bytes = al(sizeof(int) * M) + al(sizeof(int) * N) + al(t * aplane) + al(t * hplane) +
                  al(t * oplane);
@@ -854,16 +902,16 @@ static int run_split_i8(int t, int M, int N, int K, double alpha, const double* 
       hipLaunchKernelGGL(gen_residue_kernel, dim3((lpad + 255) / 256), dim3(256), 0, s, g.p0, glo,
                          ghi, TOE ? 1 : 0, e1, abits, cp, gres, (size_t)lpad, lpad);
       launch_gemm_i8<TOE ? BMODE_TOE : BMODE_HANK>(t, M, N, kp, ares, gres, aplane, (size_t)lpad,
-                                                     lpad, cp, ores, oplane, s);
-      return finish_crt(t, M, N, ores, oplane, cp, ea, eh, abits, alpha, beta, C, ldc, s);
+                                                     lpad, cp, ores, oplane, ldo, s);
+      return finish_crt(t, M, N, ores, oplane, ldo, cp, ea, eh, abits, alpha, beta, C, ldc, s);
     }
   }
   hipLaunchKernelGGL(col_exp_kernel<KIND>, dim3((N + 255) / 256, (K + kchunk - 1) / kchunk),
                      dim3(256), 0, s, K, N, g, r0, c0, kchunk, eh);
   hipLaunchKernelGGL(split_h_kernel<KIND>, dim3((N + 15) / 16, (kp + 255) / 256), dim3(256), 0, s, K, N, g,
                      r0, c0, eh, abits, cp, hres, kp, hplane);
-  launch_gemm_i8<BMODE_DENSE>(t, M, N, kp, ares, hres, aplane, hplane, 0, cp, ores, oplane, s);
-  return finish_crt(t, M, N, ores, oplane, cp, ea, eh, abits, alpha, beta, C, ldc, s);
+  launch_gemm_i8<BMODE_DENSE>(t, M, N, kp, ares, hres, aplane, hplane, 0, cp, ores, oplane, ldo, s);
+  return finish_crt(t, M, N, ores, oplane, ldo, cp, ea, eh, abits, alpha, beta, C, ldc, s);
 }
 
 hipError_t launch_right_f64(int kind, int cfg, int M, int N, int K, double alpha, const double* A,
@@ -1175,6 +1223,7 @@ struct SplitProblem {
   signed char *ares, *bres;
   unsigned char* ores;
   size_t aplane, bplane, oplane;
+  int ldo;  // residue leading dimension: M rounded up to 4
   int gx[NPH];  // x-extent of the phase's unit grid
   int ub[NPH];  // first unit of this problem in the phase
 };
@@ -1254,7 +1303,7 @@ __global__ __launch_bounds__(256, 2) void gsplit_gemm_kernel(const SplitProblem*
   const int l = lu / (ntm * ntn), r = lu % (ntm * ntn);
   gemm_i8_body<BMODE, 128, 128, 128, 2, 2, kBatchKmax>(q.M, q.N, q.kp, q.ares, q.bres, q.aplane,
                                                       q.bplane, q.lpad, cp, q.ores, q.oplane,
-                                                      r / ntn, r % ntn, l);
+                                                      q.ldo, r / ntn, r % ntn, l);
 }
 
 // Problems with min(M, N) >= kBigTile and K >= 4096 use 256 x 256 tiles
@@ -1273,12 +1322,11 @@ __global__ __launch_bounds__(256, 1) void gsplit_gemm_big_kernel(const SplitProb
   const int l = lu / (ntm * ntn), r = lu % (ntm * ntn);
   gemm_i8_body<BMODE, 256, 256, 128, 2, 2, kBatchKmax, false>(q.M, q.N, q.kp, q.ares, q.bres,
                                                               q.aplane, q.bplane, q.lpad, cp,
-                                                              q.ores, q.oplane, r / ntn, r % ntn,
-                                                              l);
+                                                              q.ores, q.oplane, q.ldo, r / ntn,
+                                                              r % ntn, l);
 }
 
-constexpr int kCrtPerThread = 4;  // elements per thread of a grouped CRT unit
-
+// A grouped CRT unit is 256 threads of four rows each.
 template <int T>
 __global__ __launch_bounds__(256) void gsplit_crt_kernel(const SplitProblem* __restrict__ P,
                                                          const int* __restrict__ ubv, int np,
@@ -1286,15 +1334,9 @@ __global__ __launch_bounds__(256) void gsplit_crt_kernel(const SplitProblem* __r
   const int u = blockIdx.x;
   const int pi = find_split_prob(ubv + PH_CRT * np, np, u);
   const SplitProblem& q = P[pi];
-  const size_t base = (size_t)(u - ubv[PH_CRT * np + pi]) * (256 * kCrtPerThread) + threadIdx.x;
-  const size_t mn = (size_t)q.M * q.N;
-#pragma unroll
-  for (int r = 0; r < kCrtPerThread; ++r) {
-    const size_t off = base + (size_t)r * 256;
-    if (off < mn)
-      crt_elem<T>(q.M, q.ores, q.oplane, cp, q.ea, q.eh, q.abits, q.abits, q.alpha, q.beta, q.C,
-                  q.ldc, off);
-  }
+  const size_t e = (size_t)(u - ubv[PH_CRT * np + pi]) * 256 + threadIdx.x;
+  crt_unit<T>(q.M, q.N, q.ores, q.oplane, q.ldo, cp, q.ea, q.eh, q.abits, q.abits, q.alpha,
+              q.beta, q.C, q.ldc, e);
 }
 
 // Host plan of a grouped split batch (table, unit totals, workspace size).
@@ -1338,7 +1380,8 @@ void split_batch_layout(const BatchProblem* probs, int nprob, int t, char* ws, S
     q.lpad = (int)((L + 15) / 16 * 16);
     q.aplane = (size_t)q.M * q.kp;
     q.bplane = (q.bmode == BMODE_DENSE) ? (size_t)q.kp * q.N : (size_t)q.lpad;
-    q.oplane = (size_t)q.M * q.N;
+    q.ldo = (q.M + 3) & ~3;
+    q.oplane = (size_t)q.ldo * q.N;
     auto take = [&](size_t bytes) {
       char* p = ws ? ws + off : nullptr;
       off += al(bytes);
@@ -1363,13 +1406,13 @@ void split_batch_layout(const BatchProblem* probs, int nprob, int t, char* ws, S
     if (dense) { gx[PH_SPLITB] = (q.N + 15) / 16; gy[PH_SPLITB] = q.kp / 256; }
     else { gx[PH_SPLITB] = (q.lpad + 255) / 256; gy[PH_SPLITB] = 1; }
     // the large class pays off once the contraction is long enough to amortise
-    // its prologue and epilogue (measured: no gain at K <= 2048)
+    // its prologue and epilogue (measured: no gain for K <= 2048 down to n = 512)
     const bool big = std::min(q.M, q.N) >= kBigTile && q.K >= 4096;
     int gph = dense ? PH_GEMM_D : (q.bmode == BMODE_TOE ? PH_GEMM_T : PH_GEMM_H);
     if (big) gph += PH_GEMM_DB - PH_GEMM_D;
     const int tile = big ? 256 : 128;
     gx[gph] = ((q.M + tile - 1) / tile) * ((q.N + tile - 1) / tile) * t; gy[gph] = 1;
-    gx[PH_CRT] = (int)(((size_t)q.M * q.N + 256 * kCrtPerThread - 1) / (256 * kCrtPerThread));
+    gx[PH_CRT] = (int)(((size_t)q.ldo * q.N / 4 + 255) / 256);
     gy[PH_CRT] = 1;
     for (int ph = 0; ph < NPH; ++ph) {
       q.gx[ph] = gx[ph];
@@ -1511,7 +1554,8 @@ size_t split_workspace_bytes(int prec, int slices, int M, int N, int K) {
   auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
   if (prec == 1) {
     const int kp = ((K + 255) / 256) * 256;
-    const size_t aplane = (size_t)M * kp, hplane = (size_t)kp * N, oplane = (size_t)M * N;
+    const size_t aplane = (size_t)M * kp, hplane = (size_t)kp * N,
+                 oplane = (size_t)((M + 3) & ~3) * N;
     return al(sizeof(int) * M) + al(sizeof(int) * N) + al(slices * aplane) + al(slices * hplane) +
            al(slices * oplane);
   }
