@@ -33,6 +33,7 @@
 #include <cstdlib>
 #include <mutex>
 #include <unordered_map>
+#include <type_traits>
 #include <vector>
 
 #include "psmma_common.h"
@@ -617,6 +618,257 @@ __device__ __forceinline__ void gemm_i8_body(
 }
 
 
+// ---------------------------------------------------------------------------
+// int8 modular GEMM with direct-to-LDS staging: 256 x 256 workgroup tiles,
+// eight waves of 128 x 64 (32 accumulators each, two waves per SIMD), k steps
+// of 64 through a ring of kDmaStages LDS stages filled by global_load_lds
+// (no staging registers, no LDS store instructions).  The fragments of step
+// k+1 are read from LDS while the MFMAs of step k run; one barrier per step.
+// (Measured on MI355X at N = 8192: 17% faster end to end than the register-
+// staged 256 x 256 form; four waves of 128 x 128 are slower, one wave per
+// SIMD cannot keep the MFMA pipe busy.)
+//
+// LDS layout of a stage: 64 bytes (one k step) per row (A) or column (B); the
+// four 16-byte chunks of row r are stored in the order c ^ g(r), with
+// g(r) = (r >> 1) & 3: the fragment reads (lane l: row l % 16, chunk l / 16)
+// then run without bank conflicts (measured with SQ_LDS_BANK_CONFLICT on
+// gfx950; the unswizzled layout and other XOR patterns spend half of their
+// LDS cycles in conflicts).
+// One global_load_lds_dwordx4 fills 16 rows: lane i writes LDS bytes
+// [16 i, 16 i + 16) of the group, i.e. row i / 4, stored chunk i % 4, so it
+// loads the global chunk (i % 4) ^ g(i / 4).  Rows past M (columns past N)
+// load the last valid row; their products only reach outputs that are not
+// stored.  Residues are zero for k >= K, so every k step is read whole.
+constexpr int kDmaStages = 5;  // 5 x 32 KB: DMA issued four k steps ahead
+#pragma clang diagnostic ignored "-Winline-asm"  // M0 is listed as clobbered on purpose
+
+__device__ __forceinline__ int dma_swz(int r) { return (r >> 1) & 3; }
+
+// s_waitcnt with only the vector-memory count bounded (expcnt, lgkmcnt left free).
+template <int N>
+__device__ __forceinline__ void wait_vmcnt() {
+  static_assert(N >= 0 && N < 64, "vmcnt range");
+  __builtin_amdgcn_s_waitcnt((N & 15) | ((N >> 4) << 14) | (7 << 4) | (15 << 8));
+}
+
+// NW = 8: waves of 128 x 64 (two per SIMD); NW = 4: waves of 128 x 128 (one
+// per SIMD, accumulators in AGPRs).
+template <int BMODE, int KMAX, int NW>
+__device__ __forceinline__ void gemm_i8_dma_body(
+    int M, int N, int kp, const signed char* __restrict__ Ares, const signed char* __restrict__ Bres,
+    size_t aplane, size_t bplane, int lpad, const CrtParams& cp, unsigned char* __restrict__ out,
+    size_t oplane, int ldo, int bmi, int bni, int l) {
+  static_assert(NW == 4 || NW == 8, "wave count");
+  constexpr bool WIN = BMODE != BMODE_DENSE;
+  constexpr int BM = 256, BN = 256, BK = 64, NS = kDmaStages;
+  constexpr int WGN = NW == 8 ? 4 : 2, WM = 128, WN = NW == 8 ? 64 : 128;
+  constexpr int TM = WM / 16, TN = WN / 16;
+  constexpr int NT = NW * 64;
+  constexpr int GPW = 16 / NW;  // 16-row DMA groups per wave and operand
+  constexpr int SA = BM * BK, SB = WIN ? 16 : BN * BK;
+  constexpr int DPS = WIN ? GPW : 2 * GPW;  // DMA instructions per wave per k step
+  constexpr int WBYTES = WIN ? KMAX + BN + 32 : 16;
+
+  // one LDS allocation: NS stages of A, then NS stages of B or the window
+  constexpr int BYTES_B = WIN ? WBYTES : NS * SB;
+  static_assert(NS * SA + BYTES_B <= 160 * 1024, "LDS of one CU");
+  __shared__ __align__(1024) signed char lds[NS * SA + BYTES_B];
+  signed char(*Ast)[SA] = (signed char(*)[SA])lds;
+  signed char(*Bst)[SB] = (signed char(*)[SB])(lds + NS * SA);
+  unsigned int* Win = (unsigned int*)(lds + NS * SA);
+
+  const int m = cp.m[l];
+  const signed char* Al = Ares + l * aplane;
+  const signed char* Bl = Bres + l * bplane;
+  unsigned char* Ol = out + l * oplane;
+  const int m0 = bmi * BM, n0 = bni * BN;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm0 = (wave / WGN) * WM, wn0 = (wave % WGN) * WN;
+  const int lr = lane & 15, lg = lane >> 4;
+
+  if constexpr (WIN) {
+    const int ubase = (BMODE == BMODE_TOE) ? (N - n0 - BN) : n0;
+    unsigned char* wb = (unsigned char*)Win;
+    for (int x = tid; x < kp + BN + 16; x += NT) {
+      int u = ubase + x;
+      wb[x] = (u >= 0 && u < lpad) ? (unsigned char)Bl[u] : 0;
+    }
+  }
+
+  // DMA sources: wave w fills row groups GPW*w .. GPW*w+GPW-1 (16 rows each)
+  // of A and the same column groups of B.  Offsets fit 32 bits (checked on the host).
+  const int rg = lane >> 2;                                   // row within a group
+  const int gch = (lane & 3) ^ dma_swz(rg);                   // global chunk this lane loads
+  unsigned aoff[GPW], boff[GPW];
+#pragma unroll
+  for (int q = 0; q < GPW; ++q) {
+    const int r = 16 * (GPW * wave + q) + rg;
+    aoff[q] = (unsigned)min(m0 + r, M - 1) * (unsigned)kp + 16u * gch;
+    boff[q] = WIN ? 0u : (unsigned)min(n0 + r, N - 1) * (unsigned)kp + 16u * gch;
+  }
+  // DMA of k step kt into stage kt % NS.  Written as inline assembly: the
+  // compiler's builtin makes every later LDS read wait for all outstanding
+  // DMA (it cannot tell the stages apart), which would serialise the ring.
+  // The instruction has no register results; its completion is tracked by
+  // the explicit vmcnt waits below.  Source: uniform base (SGPRs) + 32-bit
+  // lane offset; destination: M0 + 16 * lane.
+  auto dma16 = [&](const signed char* gbase, unsigned voff, const signed char* ldst) {
+    const unsigned la = __builtin_amdgcn_readfirstlane((unsigned)(size_t)ldst);
+    asm volatile("s_mov_b32 m0, %0\n\ts_nop 0\n\tglobal_load_lds_dwordx4 %1, %2"
+                 :
+                 : "s"(la), "v"(voff), "s"(gbase)
+                 : "memory", "m0");
+  };
+  auto issue = [&](int kt, int st) {
+    const signed char* ak = Al + (size_t)kt * BK;
+#pragma unroll
+    for (int q = 0; q < GPW; ++q) dma16(ak, aoff[q], &Ast[st][(GPW * wave + q) * 1024]);
+    if constexpr (!WIN) {
+      const signed char* bk = Bl + (size_t)kt * BK;
+#pragma unroll
+      for (int q = 0; q < GPW; ++q) dma16(bk, boff[q], &Bst[st][(GPW * wave + q) * 1024]);
+    }
+  };
+
+  v4i acc[TM][TN];
+#pragma unroll
+  for (int i = 0; i < TM; ++i)
+#pragma unroll
+    for (int j = 0; j < TN; ++j) acc[i][j] = (v4i){0, 0, 0, 0};
+
+  int cj[TN];
+#pragma unroll
+  for (int j = 0; j < TN; ++j)
+    cj[j] = (BMODE == BMODE_TOE) ? (BN - 1 - wn0 - 16 * j - lr) : (wn0 + 16 * j + lr);
+
+  const int lofs = lr * 64 + ((lg ^ dma_swz(lr)) * 16);  // this lane's fragment bytes in a block
+  // A and streamed B fragments are read one step ahead into two register
+  // slots; window B words are read at the start of their own step.
+  v2l af[2][TM];
+  v2l bfd[2][WIN ? 1 : TN];
+  unsigned bw[WIN ? TN : 1][5];
+  auto frags = [&](int kt, int slot) {
+    const signed char* sa = Ast[kt % NS];
+#pragma unroll
+    for (int i = 0; i < TM; ++i) af[slot][i] = *(const v2l*)(sa + (wm0 + 16 * i) * 64 + lofs);
+    if constexpr (!WIN) {
+      const signed char* sb = Bst[kt % NS];
+#pragma unroll
+      for (int j = 0; j < TN; ++j)
+        bfd[slot][j] = *(const v2l*)(sb + (wn0 + 16 * j) * 64 + lofs);
+    }
+  };
+  auto window_words = [&](int kt) {
+#pragma unroll
+    for (int j = 0; j < TN; ++j) {
+      int w0 = (kt * BK + lg * 16 + cj[j]) >> 2;
+#pragma unroll
+      for (int d = 0; d < 5; ++d) bw[j][d] = Win[w0 + d];
+    }
+  };
+  auto bfrag = [&](int slot, int j, int kt) -> v2l {
+    if constexpr (WIN) {
+      int sh = (kt * BK + lg * 16 + cj[j]) & 3;
+      unsigned o0 = __builtin_amdgcn_alignbyte(bw[j][1], bw[j][0], sh);
+      unsigned o1 = __builtin_amdgcn_alignbyte(bw[j][2], bw[j][1], sh);
+      unsigned o2 = __builtin_amdgcn_alignbyte(bw[j][3], bw[j][2], sh);
+      unsigned o3 = __builtin_amdgcn_alignbyte(bw[j][4], bw[j][3], sh);
+      v2l r;
+      r[0] = (long)(((unsigned long)o1 << 32) | o0);
+      r[1] = (long)(((unsigned long)o3 << 32) | o2);
+      return r;
+    } else {
+      return bfd[slot][j];
+    }
+  };
+
+  const int nkt = kp / BK;
+  // prologue: steps 0 .. NS-2 in flight (nkt >= 4 = NS - 1), step 0 landed everywhere
+#pragma unroll
+  for (int q = 0; q < NS - 1; ++q) issue(q, q);
+  wait_vmcnt<(NS - 2) * DPS>();
+  if constexpr (WIN) __syncthreads();  // the window was written with LDS stores
+  asm volatile("" ::: "memory");
+  __builtin_amdgcn_s_barrier();
+  asm volatile("" ::: "memory");
+  frags(0, 0);
+
+  // one k step; the register slot (kt parity) is a compile-time constant
+  auto step = [&](int kt, auto slotc) {
+    constexpr int slot = decltype(slotc)::value;
+    // step kt+1 landed for this wave (steps kt+2 .. kt+NS-2 may still be in
+    // flight), then for every wave; every wave has also finished reading
+    // stage (kt-1) % NS.
+    // The loop body has no conditions: past the last step, the DMA re-reads
+    // step nkt-1 into the free stage and the fragment prefetch reads a landed
+    // stage; neither result is used.
+    wait_vmcnt<(NS - 3) * DPS>();
+    asm volatile("" ::: "memory");
+    __builtin_amdgcn_s_barrier();
+    asm volatile("" ::: "memory");
+    if constexpr (WIN) window_words(kt);
+    issue(min(kt + NS - 1, nkt - 1), (kt + NS - 1) % NS);
+    frags(kt + 1, slot ^ 1);
+    // keep the prefetch ahead of the MFMAs (the scheduler would otherwise sink
+    // it below them and fold the two register slots into one)
+    __builtin_amdgcn_sched_barrier(0);
+    v2l bf[TN];
+#pragma unroll
+    for (int j = 0; j < TN; ++j) bf[j] = bfrag(slot, j, kt);
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+      for (int j = 0; j < TN; ++j)
+        acc[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(af[slot][i], bf[j], acc[i][j], 0, 0, 0);
+    __builtin_amdgcn_sched_barrier(0);
+  };
+  for (int kt = 0; kt < nkt; kt += 2) {  // kp is a multiple of 256: nkt is even, >= 4
+    step(kt, std::integral_constant<int, 0>{});
+    step(kt + 1, std::integral_constant<int, 1>{});
+  }
+  wait_vmcnt<0>();  // no DMA outlives the loop
+
+  const bool fastmod = kp <= kFastModK;
+  const float rmf = 1.0f / (float)m;
+  auto modm = [&](int x) -> unsigned {
+    int y;
+    if (fastmod) {
+      const int q = (int)floorf((float)x * rmf);
+      y = x - __mul24(q, m);
+      y += (y < 0) ? m : 0;
+      y -= (y >= m) ? m : 0;
+    } else {
+      y = x % m;
+      if (y < 0) y += m;
+    }
+    return (unsigned)y;
+  };
+#pragma unroll
+  for (int i = 0; i < TM; ++i)
+#pragma unroll
+    for (int j = 0; j < TN; ++j) {
+      int gc = n0 + wn0 + j * 16 + lr;
+      int gr = m0 + wm0 + i * 16 + lg * 4;
+      if (gc >= N || gr >= M) continue;
+      unsigned word = 0;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) word |= modm(acc[i][j][r]) << (8 * r);
+      *(unsigned*)(Ol + (size_t)gc * ldo + gr) = word;
+    }
+}
+
+template <int BMODE, int KMAX, int NW>
+__global__ __launch_bounds__(NW * 64, 1) void gemm_i8_dma_kernel(
+    int M, int N, int kp, const signed char* __restrict__ Ares, const signed char* __restrict__ Bres,
+    size_t aplane, size_t bplane, int lpad, CrtParams cp, unsigned char* __restrict__ out,
+    size_t oplane, int ldo) {
+  const int nbn = (N + 255) / 256, nbm = (M + 255) / 256;
+  int bmi, bni;
+  patch_remap<4, 8>(blockIdx.x, nbm, nbn, bmi, bni);  // one XCD's 32 resident tiles share panels
+  gemm_i8_dma_body<BMODE, KMAX, NW>(M, N, kp, Ares, Bres, aplane, bplane, lpad, cp, out, oplane, ldo,
+                                bmi, bni, blockIdx.y);
+}
+
 template <int BMODE, int BM, int BN, int BK, int WGM, int WGN, int MINB, int KMAX,
           bool STAGE2 = true>
 __global__ __launch_bounds__(WGM* WGN * 64, MINB) void gemm_i8_kernel(
@@ -807,10 +1059,12 @@ static void launch_gemm_i8(int t, int M, int N, int kp, const signed char* ares,
     return e ? atoi(e) : -1;
   }();
   int cfg = forced;
-  // 256 x 256 tiles (128 x 128 per wave, accumulators in AGPRs) once the
-  // product has enough of them to fill the chip; 128 x 128 below that.
-  if (cfg < 0) cfg = (M >= 2048 && N >= 2048) ? 4 : 0;
   auto nblk = [&](int bm, int bn) { return ((M + bm - 1) / bm) * ((N + bn - 1) / bn); };
+  // 256 x 256 direct-to-LDS tiles (one workgroup per CU) once the t planes
+  // have enough of them to fill the chip; 128 x 128 below that.
+  if (cfg < 0) cfg = (nblk(256, 256) * t >= 256) ? 5 : 0;
+  // the direct-to-LDS form addresses a residue plane with 32-bit offsets
+  if (cfg == 5 && ((size_t)M * kp >= (1ull << 32) || (size_t)N * kp >= (1ull << 32))) cfg = 4;
   switch (cfg) {
     case 1:
       hipLaunchKernelGGL((gemm_i8_kernel<BMODE, 128, 256, 128, 2, 2, 1, KMAX>),
@@ -827,6 +1081,11 @@ static void launch_gemm_i8(int t, int M, int N, int kp, const signed char* ares,
                          dim3(nblk(256, 256), t), dim3(256), 0, s, M, N, kp, ares, bres, aplane,
                          bplane, lpad, cp, ores, oplane, ldo);
       break;
+    case 5:
+      hipLaunchKernelGGL((gemm_i8_dma_kernel<BMODE, KMAX, 8>), dim3(nblk(256, 256), t), dim3(512),
+                         0, s, M, N, kp, ares, bres, aplane, bplane, lpad, cp, ores, oplane, ldo);
+      break;
+
     default:
       hipLaunchKernelGGL((gemm_i8_kernel<BMODE, 128, 128, 128, 2, 2, 2, KMAX>),
                          dim3(nblk(128, 128), t), dim3(256), 0, s, M, N, kp, ares, bres, aplane,
@@ -1306,12 +1565,11 @@ __global__ __launch_bounds__(256, 2) void gsplit_gemm_kernel(const SplitProblem*
                                                       q.ldo, r / ntn, r % ntn, l);
 }
 
-// Problems with min(M, N) >= kBigTile and K >= 4096 use 256 x 256 tiles
-// (128 x 128 per wave).
-constexpr int kBigTile = 2048;
+// Large problems use the 256 x 256 direct-to-LDS tiles (512 threads).
+constexpr int kBigTile = 512;  // smallest min(M, N) and K of the large class (measured best)
 
 template <int BMODE>
-__global__ __launch_bounds__(256, 1) void gsplit_gemm_big_kernel(const SplitProblem* __restrict__ P,
+__global__ __launch_bounds__(512, 1) void gsplit_gemm_big_kernel(const SplitProblem* __restrict__ P,
                                                                  const int* __restrict__ ubv,
                                                                  int np, int ph, CrtParams cp) {
   const int u = blockIdx.x;
@@ -1320,10 +1578,8 @@ __global__ __launch_bounds__(256, 1) void gsplit_gemm_big_kernel(const SplitProb
   const int lu = u - ubv[ph * np + pi];
   const int ntn = (q.N + 255) / 256, ntm = (q.M + 255) / 256;
   const int l = lu / (ntm * ntn), r = lu % (ntm * ntn);
-  gemm_i8_body<BMODE, 256, 256, 128, 2, 2, kBatchKmax, false>(q.M, q.N, q.kp, q.ares, q.bres,
-                                                              q.aplane, q.bplane, q.lpad, cp,
-                                                              q.ores, q.oplane, q.ldo, r / ntn,
-                                                              r % ntn, l);
+  gemm_i8_dma_body<BMODE, kBatchKmax, 8>(q.M, q.N, q.kp, q.ares, q.bres, q.aplane, q.bplane,
+                                         q.lpad, cp, q.ores, q.oplane, q.ldo, r / ntn, r % ntn, l);
 }
 
 // A grouped CRT unit is 256 threads of four rows each.
@@ -1405,9 +1661,11 @@ void split_batch_layout(const BatchProblem* probs, int nprob, int t, char* ws, S
     gx[PH_SPLITA] = (q.M + 63) / 64; gy[PH_SPLITA] = q.kp / 64;
     if (dense) { gx[PH_SPLITB] = (q.N + 15) / 16; gy[PH_SPLITB] = q.kp / 256; }
     else { gx[PH_SPLITB] = (q.lpad + 255) / 256; gy[PH_SPLITB] = 1; }
-    // the large class pays off once the contraction is long enough to amortise
-    // its prologue and epilogue (measured: no gain for K <= 2048 down to n = 512)
-    const bool big = std::min(q.M, q.N) >= kBigTile && q.K >= 4096;
+    // the large class: the direct-to-LDS tiles, whose residue planes must be
+    // addressable with 32-bit offsets
+    const bool big = std::min(q.M, q.N) >= kBigTile && q.K >= kBigTile &&
+                     (size_t)q.M * q.kp < (1ull << 32) &&
+                     (q.bmode != BMODE_DENSE || (size_t)q.N * q.kp < (1ull << 32));
     int gph = dense ? PH_GEMM_D : (q.bmode == BMODE_TOE ? PH_GEMM_T : PH_GEMM_H);
     if (big) gph += PH_GEMM_DB - PH_GEMM_D;
     const int tile = big ? 256 : 128;
@@ -1455,17 +1713,17 @@ int split_batch_enqueue(const SplitBatch& sb, const SplitProblem* dtab, hipStrea
     ++n;
   }
   if (sb.total[PH_GEMM_DB]) {
-    hipLaunchKernelGGL(gsplit_gemm_big_kernel<BMODE_DENSE>, dim3(sb.total[PH_GEMM_DB]), dim3(256),
+    hipLaunchKernelGGL(gsplit_gemm_big_kernel<BMODE_DENSE>, dim3(sb.total[PH_GEMM_DB]), dim3(512),
                        0, s, dtab, dub, np, (int)PH_GEMM_DB, sb.cp);
     ++n;
   }
   if (sb.total[PH_GEMM_TB]) {
-    hipLaunchKernelGGL(gsplit_gemm_big_kernel<BMODE_TOE>, dim3(sb.total[PH_GEMM_TB]), dim3(256), 0,
+    hipLaunchKernelGGL(gsplit_gemm_big_kernel<BMODE_TOE>, dim3(sb.total[PH_GEMM_TB]), dim3(512), 0,
                        s, dtab, dub, np, (int)PH_GEMM_TB, sb.cp);
     ++n;
   }
   if (sb.total[PH_GEMM_HB]) {
-    hipLaunchKernelGGL(gsplit_gemm_big_kernel<BMODE_HANK>, dim3(sb.total[PH_GEMM_HB]), dim3(256), 0,
+    hipLaunchKernelGGL(gsplit_gemm_big_kernel<BMODE_HANK>, dim3(sb.total[PH_GEMM_HB]), dim3(512), 0,
                        s, dtab, dub, np, (int)PH_GEMM_HB, sb.cp);
     ++n;
   }
