@@ -369,6 +369,16 @@ __global__ void gen_residue_kernel(const double* __restrict__ g, long long lo, l
 enum { BMODE_DENSE = 0, BMODE_TOE = 1, BMODE_HANK = 2 };
 constexpr int kFastModK = 16384;  // longest padded contraction of the FP32 reduction mod m
 
+// acc mod m in [0, m) for |acc| <= 2^28.  t' = fl(fl(acc) * fl(1/m) - 1/4) (one
+// FMA) is within 3 * 2^-24 * |acc| / m <= 1/4 of acc / m - 1/4, so
+// q = floor(t') is floor(acc / m) or one less (|q| < 2^21: the 24-bit multiply
+// is exact), r = acc - q m lies in [0, 2m) and one unsigned min finishes.
+__device__ __forceinline__ unsigned mod_acc(int x, int m, float rmf) {
+  const int q = (int)floorf(fmaf((float)x, rmf, -0.25f));
+  const unsigned r = (unsigned)(x - __mul24(q, m));
+  return min(r, r - (unsigned)m);
+}
+
 template <int BMODE, int BM, int BN, int BK, int WGM, int WGN, int KMAX, bool STAGE2 = true>
 __device__ __forceinline__ void gemm_i8_body(
     int M, int N, int kp, const signed char* __restrict__ Ares, const signed char* __restrict__ Bres,
@@ -584,19 +594,13 @@ __device__ __forceinline__ void gemm_i8_body(
   }
 
   // Reduction mod m.  Residues lie in [-128, 127], so |acc| <= kp * 2^14; for
-  // kp <= kFastModK that is <= 2^28, where q = floor(acc * fl(1/m)) in FP32 is
-  // within one of the true quotient (relative error <= 3 * 2^-24, quotient
-  // < 2^21) and |q| < 2^23 makes the 24-bit multiply exact: one correction on
-  // each side gives the residue in [0, m).  Longer contractions use %.
+  // kp <= kFastModK that is <= 2^28 (mod_acc).  Longer contractions use %.
   const bool fastmod = kp <= kFastModK;
   const float rmf = 1.0f / (float)m;
   auto modm = [&](int x) -> unsigned {
     int y;
     if (fastmod) {
-      const int q = (int)floorf((float)x * rmf);
-      y = x - __mul24(q, m);
-      y += (y < 0) ? m : 0;
-      y -= (y >= m) ? m : 0;
+      return mod_acc(x, m, rmf);
     } else {
       y = x % m;
       if (y < 0) y += m;
@@ -796,16 +800,22 @@ __device__ __forceinline__ void gemm_i8_dma_body(
   // one k step; the register slot (kt parity) is a compile-time constant
   auto step = [&](int kt, auto slotc) {
     constexpr int slot = decltype(slotc)::value;
-    // step kt+1 landed for this wave (steps kt+2 .. kt+NS-2 may still be in
-    // flight), then for every wave; every wave has also finished reading
-    // stage (kt-1) % NS.
+    // One barrier per two steps.  At an even step, steps kt+1 and kt+2 have
+    // landed for this wave (only kt+3 may be in flight), then for every wave,
+    // and every wave has read its fragments of steps <= kt-1.  The DMA of this
+    // step and of the next (odd) one write stages (kt-1) % NS and kt % NS,
+    // whose fragments were read before this barrier; the prefetches of this
+    // step and the next read stages kt+1 and kt+2.
     // The loop body has no conditions: past the last step, the DMA re-reads
     // step nkt-1 into the free stage and the fragment prefetch reads a landed
     // stage; neither result is used.
-    wait_vmcnt<(NS - 3) * DPS>();
-    asm volatile("" ::: "memory");
-    __builtin_amdgcn_s_barrier();
-    asm volatile("" ::: "memory");
+    static_assert(NS == 5, "the two-step barrier schedule assumes five stages");
+    if constexpr (slot == 0) {
+      wait_vmcnt<DPS>();
+      asm volatile("" ::: "memory");
+      __builtin_amdgcn_s_barrier();
+      asm volatile("" ::: "memory");
+    }
     if constexpr (WIN) window_words(kt);
     issue(min(kt + NS - 1, nkt - 1), (kt + NS - 1) % NS);
     frags(kt + 1, slot ^ 1);
@@ -833,10 +843,7 @@ __device__ __forceinline__ void gemm_i8_dma_body(
   auto modm = [&](int x) -> unsigned {
     int y;
     if (fastmod) {
-      const int q = (int)floorf((float)x * rmf);
-      y = x - __mul24(q, m);
-      y += (y < 0) ? m : 0;
-      y -= (y >= m) ? m : 0;
+      return mod_acc(x, m, rmf);
     } else {
       y = x % m;
       if (y < 0) y += m;
