@@ -172,8 +172,17 @@ __device__ __forceinline__ void row_exp_kernel_body(int M, int K, const double* 
   int k0 = by_ * kchunk;
   if (i >= M) return;
   int k1 = min(K, k0 + kchunk);
-  double mx = 0.0;
-  for (int k = k0; k < k1; ++k) mx = fmax(mx, fabs(A[(size_t)k * lda + i]));
+  // eight independent maxima: eight loads in flight per thread
+  double m8[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  const double* a = A + (size_t)k0 * lda + i;
+  int k = k0;
+  for (; k + 8 <= k1; k += 8, a += (size_t)8 * lda) {
+#pragma unroll
+    for (int q = 0; q < 8; ++q) m8[q] = fmax(m8[q], fabs(a[(size_t)q * lda]));
+  }
+  for (; k < k1; ++k, a += lda) m8[0] = fmax(m8[0], fabs(*a));
+  double mx = fmax(fmax(fmax(m8[0], m8[1]), fmax(m8[2], m8[3])),
+                   fmax(fmax(m8[4], m8[5]), fmax(m8[6], m8[7])));
   if (mx > 0.0) atomicMax(&ea[i], exponent_of(mx));
 }
 
@@ -1505,15 +1514,18 @@ __device__ __forceinline__ int find_split_prob(const int* __restrict__ ub, int n
   return lo;
 }
 
+// One kernel per phase: each carries only its own LDS and registers.
+template <int ph>
 __global__ __launch_bounds__(256) void gsplit_phase_kernel(const SplitProblem* __restrict__ P,
                                                            const int* __restrict__ ubv, int np,
-                                                           int ph, CrtParams cp) {
+                                                           CrtParams cp) {
   const int u = blockIdx.x;
   const int pi = find_split_prob(ubv + ph * np, np, u);
   const SplitProblem& q = P[pi];
   const int lu = u - ubv[ph * np + pi];
   const int bx = lu % q.gx[ph], by = lu / q.gx[ph];
   const bool dense = q.bmode == BMODE_DENSE;
+  (void)dense;
   switch (ph) {
     case PH_FILL: {
       int na = (q.M + 255) / 256, nh = (q.N + 255) / 256;
@@ -1700,7 +1712,13 @@ int split_batch_enqueue(const SplitBatch& sb, const SplitProblem* dtab, hipStrea
   }
   for (int ph : {PH_FILL, PH_ROWEXP, PH_COLEXP, PH_EHFILL, PH_SPLITA, PH_SPLITB}) {
     if (sb.total[ph] == 0) continue;
-    hipLaunchKernelGGL(gsplit_phase_kernel, dim3(sb.total[ph]), dim3(256), 0, s, dtab, dub, np, ph,
+    auto kern = ph == PH_FILL     ? gsplit_phase_kernel<PH_FILL>
+                : ph == PH_ROWEXP ? gsplit_phase_kernel<PH_ROWEXP>
+                : ph == PH_COLEXP ? gsplit_phase_kernel<PH_COLEXP>
+                : ph == PH_EHFILL ? gsplit_phase_kernel<PH_EHFILL>
+                : ph == PH_SPLITA ? gsplit_phase_kernel<PH_SPLITA>
+                                  : gsplit_phase_kernel<PH_SPLITB>;
+    hipLaunchKernelGGL(kern, dim3(sb.total[ph]), dim3(256), 0, s, dtab, dub, np,
                        sb.cp);
     ++n;
   }
