@@ -50,6 +50,46 @@ constexpr int kNoExp = -10000;
 constexpr int kModuli[kMaxModuli] = {256, 255, 253, 251, 247, 241, 239, 233,
                                      229, 227, 223, 217, 211, 199, 197, 193};
 
+// y_l / m_l = wa + wb + wc for t = T moduli (y_l = (M/m_l)^-1 mod m_l): wa on
+// the 2^-41 grid and wb on the 2^-81 grid, so that a residue (< 2^8) times
+// either is exact and sums of up to 16 such products stay exact in FP64; wc is
+// the rounded rest (< 2^-81).  Evaluated at compile time, so the CRT kernels
+// take the constants as literals (t-length arrays in a kernel argument do not
+// fit the scalar registers next to everything else).
+template <int T>
+struct CrtFrac {
+  double wa[T], wb[T], wc[T];
+};
+constexpr long long cx_inv_mod(long long a, long long m) {
+  long long t = 0, nt = 1, r = m, nr = ((a % m) + m) % m;
+  while (nr) {
+    long long q = r / nr, tmp = t - q * nt;
+    t = nt;
+    nt = tmp;
+    tmp = r - q * nr;
+    r = nr;
+    nr = tmp;
+  }
+  return (t % m + m) % m;
+}
+template <int T>
+constexpr CrtFrac<T> make_crt_frac() {
+  CrtFrac<T> f{};
+  for (int l = 0; l < T; ++l) {
+    const long long m = kModuli[l];
+    long long ml = 1;  // (M / m_l) mod m_l
+    for (int j = 0; j < T; ++j)
+      if (j != l) ml = ml * (kModuli[j] % m) % m;
+    const unsigned long long y = (unsigned long long)cx_inv_mod(ml, m);
+    const unsigned long long num = y << 41, wa = num / m, r1 = num % m;
+    const unsigned long long num2 = r1 << 40, wb = num2 / m, r2 = num2 % m;
+    f.wa[l] = (double)wa * 0x1p-41;
+    f.wb[l] = (double)wb * 0x1p-81;
+    f.wc[l] = ((double)r2 / (double)m) * 0x1p-81;
+  }
+  return f;
+}
+
 struct CrtParams {
   int t;
   int m[kMaxModuli];
@@ -58,10 +98,6 @@ struct CrtParams {
   double inv_lo[kMaxModuli];
   double M_hi, M_lo;          // M = prod m_l as a double-double
   double rm[kMaxModuli];      // 1/m_l rounded (for the residue quotient)
-  // y_l / m_l = wa + wb + wc: wa on the 2^-41 grid and wb on the 2^-81 grid, so
-  // that a residue (< 2^8) times either is exact and sums of up to 16 such
-  // products stay exact in FP64; wc is the rounded rest (< 2^-81).
-  double wa[kMaxModuli], wb[kMaxModuli], wc[kMaxModuli];
   unsigned char minv[kMaxModuli][kMaxModuli];  // minv[j][l] = m_j^-1 mod m_l (j < l)
 };
 
@@ -94,15 +130,6 @@ static CrtParams make_crt(int t) {
     p.inv_hi[l] = hi;
     p.inv_lo[l] = lo;
     p.rm[l] = 1.0 / m;
-    {
-      const unsigned long long num = (unsigned long long)p.y[l] << 41;  // < 2^49
-      const unsigned long long wa = num / (unsigned)m, r1 = num % (unsigned)m;
-      const unsigned long long num2 = r1 << 40;  // < 2^48
-      const unsigned long long wb = num2 / (unsigned)m, r2 = num2 % (unsigned)m;
-      p.wa[l] = std::ldexp((double)wa, -41);
-      p.wb[l] = std::ldexp((double)wb, -81);
-      p.wc[l] = std::ldexp((double)r2 / m, -81);
-    }
     for (int j = 0; j < l; ++j) p.minv[j][l] = (unsigned char)inv_mod(kModuli[j] % m, m);
   }
   // M as hi + lo (M < 2^128): split the 128-bit integer exactly.
@@ -362,7 +389,7 @@ __global__ void gen_residue_kernel(const double* __restrict__ g, long long lo, l
 // ---------------------------------------------------------------------------
 // int8 modular GEMM, one modulus per blockIdx.y:
 //   R_l = (Ares_l * B_l) mod m_l, stored as uint8, column-major with leading
-//   dimension ldo = M rounded up to 4 (every lane stores whole dwords).
+//   dimension ldo = M rounded up to 16 (whole dwords / 16-byte chunks).
 // Ares_l: M x kp row-major int8.  B_l is either streamed (BMODE_DENSE: kp x N
 // column-major residues) or assembled from a generator window (BMODE_TOE /
 // BMODE_HANK: residues R_l[u] of the scaled generator, u = k + (N-1-j) or
@@ -664,22 +691,24 @@ __device__ __forceinline__ void wait_vmcnt() {
   __builtin_amdgcn_s_waitcnt((N & 15) | ((N >> 4) << 14) | (7 << 4) | (15 << 8));
 }
 
-// NW = 8: waves of 128 x 64 (two per SIMD); NW = 4: waves of 128 x 128 (one
-// per SIMD, accumulators in AGPRs).
-template <int BMODE, int KMAX, int NW>
+// BN = 256: one workgroup per CU, five stages, eight waves.  BN = 128: 256 x 128
+// tiles, four waves, three stages (72 KB): two workgroups per CU, so one
+// workgroup's prologue and epilogue overlap the other's MFMAs (short
+// contractions of a batch).  Waves are 128 x 64 in both.
+template <int BMODE, int KMAX, int BN>
 __device__ __forceinline__ void gemm_i8_dma_body(
     int M, int N, int kp, const signed char* __restrict__ Ares, const signed char* __restrict__ Bres,
     size_t aplane, size_t bplane, int lpad, const CrtParams& cp, unsigned char* __restrict__ out,
     size_t oplane, int ldo, int bmi, int bni, int l) {
-  static_assert(NW == 4 || NW == 8, "wave count");
+  static_assert(BN == 256 || BN == 128, "tile width");
   constexpr bool WIN = BMODE != BMODE_DENSE;
-  constexpr int BM = 256, BN = 256, BK = 64, NS = kDmaStages;
-  constexpr int WGN = NW == 8 ? 4 : 2, WM = 128, WN = NW == 8 ? 64 : 128;
+  constexpr int BM = 256, BK = 64, NS = BN == 256 ? kDmaStages : 3;
+  constexpr int WGN = BN / 64, NW = 2 * WGN, WM = 128, WN = 64;
   constexpr int TM = WM / 16, TN = WN / 16;
   constexpr int NT = NW * 64;
-  constexpr int GPW = 16 / NW;  // 16-row DMA groups per wave and operand
+  constexpr int GPWA = 16 / NW, GPWB = (BN / 16) / NW;  // 16-row DMA groups per wave
   constexpr int SA = BM * BK, SB = WIN ? 16 : BN * BK;
-  constexpr int DPS = WIN ? GPW : 2 * GPW;  // DMA instructions per wave per k step
+  constexpr int DPS = WIN ? GPWA : GPWA + GPWB;  // DMA instructions per wave per k step
   constexpr int WBYTES = WIN ? KMAX + BN + 32 : 16;
 
   // one LDS allocation: NS stages of A, then NS stages of B or the window
@@ -708,15 +737,19 @@ __device__ __forceinline__ void gemm_i8_dma_body(
     }
   }
 
-  // DMA sources: wave w fills row groups GPW*w .. GPW*w+GPW-1 (16 rows each)
-  // of A and the same column groups of B.  Offsets fit 32 bits (checked on the host).
+  // DMA sources: wave w fills row groups GPWA*w .. GPWA*w+GPWA-1 (16 rows each)
+  // of A and column groups GPWB*w .. of B.  Offsets fit 32 bits (checked on the host).
   const int rg = lane >> 2;                                   // row within a group
   const int gch = (lane & 3) ^ dma_swz(rg);                   // global chunk this lane loads
-  unsigned aoff[GPW], boff[GPW];
+  unsigned aoff[GPWA], boff[GPWB];
 #pragma unroll
-  for (int q = 0; q < GPW; ++q) {
-    const int r = 16 * (GPW * wave + q) + rg;
+  for (int q = 0; q < GPWA; ++q) {
+    const int r = 16 * (GPWA * wave + q) + rg;
     aoff[q] = (unsigned)min(m0 + r, M - 1) * (unsigned)kp + 16u * gch;
+  }
+#pragma unroll
+  for (int q = 0; q < GPWB; ++q) {
+    const int r = 16 * (GPWB * wave + q) + rg;
     boff[q] = WIN ? 0u : (unsigned)min(n0 + r, N - 1) * (unsigned)kp + 16u * gch;
   }
   // DMA of k step kt into stage kt % NS.  Written as inline assembly: the
@@ -735,11 +768,11 @@ __device__ __forceinline__ void gemm_i8_dma_body(
   auto issue = [&](int kt, int st) {
     const signed char* ak = Al + (size_t)kt * BK;
 #pragma unroll
-    for (int q = 0; q < GPW; ++q) dma16(ak, aoff[q], &Ast[st][(GPW * wave + q) * 1024]);
+    for (int q = 0; q < GPWA; ++q) dma16(ak, aoff[q], &Ast[st][(GPWA * wave + q) * 1024]);
     if constexpr (!WIN) {
       const signed char* bk = Bl + (size_t)kt * BK;
 #pragma unroll
-      for (int q = 0; q < GPW; ++q) dma16(bk, boff[q], &Bst[st][(GPW * wave + q) * 1024]);
+      for (int q = 0; q < GPWB; ++q) dma16(bk, boff[q], &Bst[st][(GPWB * wave + q) * 1024]);
     }
   };
 
@@ -796,7 +829,7 @@ __device__ __forceinline__ void gemm_i8_dma_body(
   };
 
   const int nkt = kp / BK;
-  // prologue: steps 0 .. NS-2 in flight (nkt >= 4 = NS - 1), step 0 landed everywhere
+  // prologue: steps 0 .. NS-2 in flight (nkt >= 4 >= NS - 1), step 0 landed everywhere
 #pragma unroll
   for (int q = 0; q < NS - 1; ++q) issue(q, q);
   wait_vmcnt<(NS - 2) * DPS>();
@@ -818,9 +851,18 @@ __device__ __forceinline__ void gemm_i8_dma_body(
     // The loop body has no conditions: past the last step, the DMA re-reads
     // step nkt-1 into the free stage and the fragment prefetch reads a landed
     // stage; neither result is used.
-    static_assert(NS == 5, "the two-step barrier schedule assumes five stages");
-    if constexpr (slot == 0) {
-      wait_vmcnt<DPS>();
+    // With three stages (DMA two steps ahead) the barrier is taken every step,
+    // once step kt+1 has landed (nothing later is in flight at that point).
+    static_assert(NS == 5 || NS == 3, "barrier schedules for five or three stages");
+    if constexpr (NS == 5) {
+      if constexpr (slot == 0) {
+        wait_vmcnt<DPS>();
+        asm volatile("" ::: "memory");
+        __builtin_amdgcn_s_barrier();
+        asm volatile("" ::: "memory");
+      }
+    } else {
+      wait_vmcnt<0>();
       asm volatile("" ::: "memory");
       __builtin_amdgcn_s_barrier();
       asm volatile("" ::: "memory");
@@ -859,29 +901,47 @@ __device__ __forceinline__ void gemm_i8_dma_body(
     }
     return (unsigned)y;
   };
+  // The residue tile goes through LDS (column-major, pitch OP bytes) and
+  // leaves as 16-byte column chunks: 4-byte stores scattered over 16 columns
+  // per instruction made the epilogue store-bound.  OP / 4 = 68 = 4 (mod 32)
+  // spreads a wave's dword writes over all banks.  ldo is a multiple of 16,
+  // so a chunk that starts inside a column's ldo rows ends inside them.
+  constexpr int OP = 272;
+  static_assert(BN * OP <= NS * SA + BYTES_B, "output tile fits the stage memory");
+  asm volatile("" ::: "memory");
+  __builtin_amdgcn_s_barrier();  // every wave is done with the stages
+  asm volatile("" ::: "memory");
 #pragma unroll
   for (int i = 0; i < TM; ++i)
 #pragma unroll
     for (int j = 0; j < TN; ++j) {
-      int gc = n0 + wn0 + j * 16 + lr;
-      int gr = m0 + wm0 + i * 16 + lg * 4;
-      if (gc >= N || gr >= M) continue;
       unsigned word = 0;
 #pragma unroll
       for (int r = 0; r < 4; ++r) word |= modm(acc[i][j][r]) << (8 * r);
-      *(unsigned*)(Ol + (size_t)gc * ldo + gr) = word;
+      *(unsigned*)(lds + (wn0 + j * 16 + lr) * OP + wm0 + i * 16 + lg * 4) = word;
     }
+  __syncthreads();
+  constexpr int CHUNKS = BN * (BM / 16);
+  static_assert(CHUNKS % NT == 0, "chunks per thread");
+#pragma unroll
+  for (int it = 0; it < CHUNKS / NT; ++it) {
+    const int x = it * NT + tid;
+    const int c = x >> 4, q = x & 15;
+    const int gc = n0 + c, gr = m0 + 16 * q;
+    if (gc < N && gr < ldo)
+      *(uint4*)(Ol + (size_t)gc * ldo + gr) = *(const uint4*)(lds + c * OP + 16 * q);
+  }
 }
 
-template <int BMODE, int KMAX, int NW>
-__global__ __launch_bounds__(NW * 64, 1) void gemm_i8_dma_kernel(
+template <int BMODE, int KMAX, int BN>
+__global__ __launch_bounds__(BN / 2 * 4, BN == 256 ? 1 : 2) void gemm_i8_dma_kernel(
     int M, int N, int kp, const signed char* __restrict__ Ares, const signed char* __restrict__ Bres,
     size_t aplane, size_t bplane, int lpad, CrtParams cp, unsigned char* __restrict__ out,
     size_t oplane, int ldo) {
-  const int nbn = (N + 255) / 256, nbm = (M + 255) / 256;
+  const int nbn = (N + BN - 1) / BN, nbm = (M + 255) / 256;
   int bmi, bni;
   patch_remap<4, 8>(blockIdx.x, nbm, nbn, bmi, bni);  // one XCD's 32 resident tiles share panels
-  gemm_i8_dma_body<BMODE, KMAX, NW>(M, N, kp, Ares, Bres, aplane, bplane, lpad, cp, out, oplane, ldo,
+  gemm_i8_dma_body<BMODE, KMAX, BN>(M, N, kp, Ares, Bres, aplane, bplane, lpad, cp, out, oplane, ldo,
                                 bmi, bni, blockIdx.y);
 }
 
@@ -964,6 +1024,7 @@ __device__ __forceinline__ void crt_quad(int M, const unsigned char* __restrict_
   for (int l = 0; l < T; ++l) w[l] = *(const unsigned*)(res + l * oplane + off);
   const int s_h = hbits - eh[j];
   double* cj = C + (size_t)j * ldc;
+  constexpr CrtFrac<T> fr = make_crt_frac<T>();
   unsigned slow = 0;  // rows that need the exact form
 #pragma unroll
   for (int r = 0; r < 4; ++r) {
@@ -973,9 +1034,9 @@ __device__ __forceinline__ void crt_quad(int M, const unsigned char* __restrict_
 #pragma unroll
     for (int l = 0; l < T; ++l) {
       const double x = (double)((w[l] >> (8 * r)) & 255u);
-      sa = fma(x, cp.wa[l], sa);  // exact
-      sb = fma(x, cp.wb[l], sb);  // exact
-      sc = fma(x, cp.wc[l], sc);
+      sa = fma(x, fr.wa[l], sa);  // exact
+      sb = fma(x, fr.wb[l], sb);  // exact
+      sc = fma(x, fr.wc[l], sc);
     }
     double f, fl;
     two_sum(sa - rint(sa), sb, f, fl);
@@ -1080,7 +1141,8 @@ static void launch_gemm_i8(int t, int M, int N, int kp, const signed char* ares,
   // have enough of them to fill the chip; 128 x 128 below that.
   if (cfg < 0) cfg = (nblk(256, 256) * t >= 256) ? 5 : 0;
   // the direct-to-LDS form addresses a residue plane with 32-bit offsets
-  if (cfg == 5 && ((size_t)M * kp >= (1ull << 32) || (size_t)N * kp >= (1ull << 32))) cfg = 4;
+  if ((cfg == 5 || cfg == 6) && ((size_t)M * kp >= (1ull << 32) || (size_t)N * kp >= (1ull << 32)))
+    cfg = 4;
   switch (cfg) {
     case 1:
       hipLaunchKernelGGL((gemm_i8_kernel<BMODE, 128, 256, 128, 2, 2, 1, KMAX>),
@@ -1098,8 +1160,14 @@ static void launch_gemm_i8(int t, int M, int N, int kp, const signed char* ares,
                          bplane, lpad, cp, ores, oplane, ldo);
       break;
     case 5:
-      hipLaunchKernelGGL((gemm_i8_dma_kernel<BMODE, KMAX, 8>), dim3(nblk(256, 256), t), dim3(512),
-                         0, s, M, N, kp, ares, bres, aplane, bplane, lpad, cp, ores, oplane, ldo);
+      hipLaunchKernelGGL((gemm_i8_dma_kernel<BMODE, KMAX, 256>), dim3(nblk(256, 256), t),
+                         dim3(512), 0, s, M, N, kp, ares, bres, aplane, bplane, lpad, cp, ores,
+                         oplane, ldo);
+      break;
+    case 6:
+      hipLaunchKernelGGL((gemm_i8_dma_kernel<BMODE, KMAX, 128>), dim3(nblk(256, 128), t),
+                         dim3(256), 0, s, M, N, kp, ares, bres, aplane, bplane, lpad, cp, ores,
+                         oplane, ldo);
       break;
 
     default:
@@ -1138,7 +1206,7 @@ static int run_split_i8(int t, int M, int N, int K, double alpha, const double* 
   const int abits = choose_abits(t, K);
   if (abits < 8) return -16;
   const int kp = ((K + 255) / 256) * 256;
-  const int ldo = (M + 3) & ~3;
+  const int ldo = (M + 15) & ~15;  // residue column pitch (16-byte chunks)
   const size_t aplane = (size_t)M * kp, hplane = (size_t)kp * N, oplane = (size_t)ldo * N;
   auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
   size_t bytes = al(sizeof(int) * M) + al(sizeof(int) * N) + al(t * aplane) + al(t * hplane) +
@@ -1498,7 +1566,7 @@ struct SplitProblem {
   signed char *ares, *bres;
   unsigned char* ores;
   size_t aplane, bplane, oplane;
-  int ldo;  // residue leading dimension: M rounded up to 4
+  int ldo;  // residue leading dimension: M rounded up to 16
   int gx[NPH];  // x-extent of the phase's unit grid
   int ub[NPH];  // first unit of this problem in the phase
 };
@@ -1597,8 +1665,8 @@ __global__ __launch_bounds__(512, 1) void gsplit_gemm_big_kernel(const SplitProb
   const int lu = u - ubv[ph * np + pi];
   const int ntn = (q.N + 255) / 256, ntm = (q.M + 255) / 256;
   const int l = lu / (ntm * ntn), r = lu % (ntm * ntn);
-  gemm_i8_dma_body<BMODE, kBatchKmax, 8>(q.M, q.N, q.kp, q.ares, q.bres, q.aplane, q.bplane,
-                                         q.lpad, cp, q.ores, q.oplane, q.ldo, r / ntn, r % ntn, l);
+  gemm_i8_dma_body<BMODE, kBatchKmax, 256>(q.M, q.N, q.kp, q.ares, q.bres, q.aplane, q.bplane,
+                                           q.lpad, cp, q.ores, q.oplane, q.ldo, r / ntn, r % ntn, l);
 }
 
 // A grouped CRT unit is 256 threads of four rows each.
@@ -1655,7 +1723,7 @@ void split_batch_layout(const BatchProblem* probs, int nprob, int t, char* ws, S
     q.lpad = (int)((L + 15) / 16 * 16);
     q.aplane = (size_t)q.M * q.kp;
     q.bplane = (q.bmode == BMODE_DENSE) ? (size_t)q.kp * q.N : (size_t)q.lpad;
-    q.ldo = (q.M + 3) & ~3;
+    q.ldo = (q.M + 15) & ~15;
     q.oplane = (size_t)q.ldo * q.N;
     auto take = [&](size_t bytes) {
       char* p = ws ? ws + off : nullptr;
@@ -1838,7 +1906,7 @@ size_t split_workspace_bytes(int prec, int slices, int M, int N, int K) {
   if (prec == 1) {
     const int kp = ((K + 255) / 256) * 256;
     const size_t aplane = (size_t)M * kp, hplane = (size_t)kp * N,
-                 oplane = (size_t)((M + 3) & ~3) * N;
+                 oplane = (size_t)((M + 15) & ~15) * N;
     return al(sizeof(int) * M) + al(sizeof(int) * N) + al(slices * aplane) + al(slices * hplane) +
            al(slices * oplane);
   }
