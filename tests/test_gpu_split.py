@@ -14,7 +14,10 @@ pytestmark = pytest.mark.gpu
 
 
 @pytest.mark.parametrize("kind", ["cauchy", "toeplitz", "hankel", "cauchyeig"])
-@pytest.mark.parametrize("m,n,k", [(1, 1, 1), (100, 130, 77), (513, 257, 1000), (1024, 1024, 1024)])
+# (1100, 1037, 700): 25 tiles of 256 x 256 per modulus, so t >= 13 takes the
+# direct-to-LDS kernel with ragged tiles, a padded residue pitch and K % 64 != 0
+@pytest.mark.parametrize("m,n,k", [(1, 1, 1), (100, 130, 77), (513, 257, 1000), (1024, 1024, 1024),
+                                   (1100, 1037, 700)])
 def test_split_i8_parity(gpu, ora, kind, m, n, k):
     import torch
 
