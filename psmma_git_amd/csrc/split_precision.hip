@@ -631,30 +631,35 @@ __device__ __forceinline__ void gemm_i8_body(
 
   // Reduction mod m.  Residues lie in [-128, 127], so |acc| <= kp * 2^14; for
   // kp <= kFastModK that is <= 2^28 (mod_acc).  Longer contractions use %.
-  const bool fastmod = kp <= kFastModK;
   const float rmf = 1.0f / (float)m;
-  auto modm = [&](int x) -> unsigned {
-    int y;
-    if (fastmod) {
-      return mod_acc(x, m, rmf);
-    } else {
-      y = x % m;
-      if (y < 0) y += m;
-    }
-    return (unsigned)y;
+  // the uniform choice of reduction is taken once, outside the element loops
+  auto reduce_store = [&](auto fastc) {
+    constexpr bool FAST = decltype(fastc)::value;
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+      for (int j = 0; j < TN; ++j) {
+        int gc = n0 + wn0 + j * 16 + lr;
+        int gr = m0 + wm0 + i * 16 + lg * 4;  // a multiple of 4: gr + 3 < ldo when gr < M
+        if (gc >= N || gr >= M) continue;
+        unsigned word = 0;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int x = acc[i][j][r];
+          unsigned y;
+          if constexpr (FAST) {
+            y = mod_acc(x, m, rmf);
+          } else {
+            int z = x % m;
+            y = (unsigned)(z < 0 ? z + m : z);
+          }
+          word |= y << (8 * r);
+        }
+        *(unsigned*)(Ol + (size_t)gc * ldo + gr) = word;
+      }
   };
-#pragma unroll
-  for (int i = 0; i < TM; ++i)
-#pragma unroll
-    for (int j = 0; j < TN; ++j) {
-      int gc = n0 + wn0 + j * 16 + lr;
-      int gr = m0 + wm0 + i * 16 + lg * 4;  // a multiple of 4: gr + 3 < ldo when gr < M
-      if (gc >= N || gr >= M) continue;
-      unsigned word = 0;
-#pragma unroll
-      for (int r = 0; r < 4; ++r) word |= modm(acc[i][j][r]) << (8 * r);
-      *(unsigned*)(Ol + (size_t)gc * ldo + gr) = word;
-    }
+  if (kp <= kFastModK) reduce_store(std::integral_constant<bool, true>{});
+  else reduce_store(std::integral_constant<bool, false>{});
 }
 
 
@@ -889,18 +894,7 @@ __device__ __forceinline__ void gemm_i8_dma_body(
   }
   wait_vmcnt<0>();  // no DMA outlives the loop
 
-  const bool fastmod = kp <= kFastModK;
   const float rmf = 1.0f / (float)m;
-  auto modm = [&](int x) -> unsigned {
-    int y;
-    if (fastmod) {
-      return mod_acc(x, m, rmf);
-    } else {
-      y = x % m;
-      if (y < 0) y += m;
-    }
-    return (unsigned)y;
-  };
   // The residue tile goes through LDS (column-major, pitch OP bytes) and
   // leaves as 16-byte column chunks: 4-byte stores scattered over 16 columns
   // per instruction made the epilogue store-bound.  OP / 4 = 68 = 4 (mod 32)
@@ -911,15 +905,31 @@ __device__ __forceinline__ void gemm_i8_dma_body(
   asm volatile("" ::: "memory");
   __builtin_amdgcn_s_barrier();  // every wave is done with the stages
   asm volatile("" ::: "memory");
+  // the uniform choice of reduction is taken once, outside the element loops
+  auto reduce_store = [&](auto fastc) {
+    constexpr bool FAST = decltype(fastc)::value;
 #pragma unroll
-  for (int i = 0; i < TM; ++i)
+    for (int i = 0; i < TM; ++i)
 #pragma unroll
-    for (int j = 0; j < TN; ++j) {
-      unsigned word = 0;
+      for (int j = 0; j < TN; ++j) {
+        unsigned word = 0;
 #pragma unroll
-      for (int r = 0; r < 4; ++r) word |= modm(acc[i][j][r]) << (8 * r);
-      *(unsigned*)(lds + (wn0 + j * 16 + lr) * OP + wm0 + i * 16 + lg * 4) = word;
-    }
+        for (int r = 0; r < 4; ++r) {
+          const int x = acc[i][j][r];
+          unsigned y;
+          if constexpr (FAST) {
+            y = mod_acc(x, m, rmf);
+          } else {
+            int z = x % m;
+            y = (unsigned)(z < 0 ? z + m : z);
+          }
+          word |= y << (8 * r);
+        }
+        *(unsigned*)(lds + (wn0 + j * 16 + lr) * OP + wm0 + i * 16 + lg * 4) = word;
+      }
+  };
+  if (kp <= kFastModK) reduce_store(std::integral_constant<bool, true>{});
+  else reduce_store(std::integral_constant<bool, false>{});
   __syncthreads();
   constexpr int CHUNKS = BN * (BM / 16);
   static_assert(CHUNKS % NT == 0, "chunks per thread");
