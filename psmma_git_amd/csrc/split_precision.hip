@@ -1679,7 +1679,9 @@ __global__ __launch_bounds__(512, 1) void gsplit_gemm_big_kernel(const SplitProb
                                            q.lpad, cp, q.ores, q.oplane, q.ldo, r / ntn, r % ntn, l);
 }
 
-// A grouped CRT unit is 256 threads of four rows each.
+// A grouped CRT unit is 256 threads x kCrtQuads groups of four rows (the
+// problem lookup of a workgroup is paid once per 1024 kCrtQuads outputs).
+constexpr int kCrtQuads = 4;
 template <int T>
 __global__ __launch_bounds__(256) void gsplit_crt_kernel(const SplitProblem* __restrict__ P,
                                                          const int* __restrict__ ubv, int np,
@@ -1687,9 +1689,11 @@ __global__ __launch_bounds__(256) void gsplit_crt_kernel(const SplitProblem* __r
   const int u = blockIdx.x;
   const int pi = find_split_prob(ubv + PH_CRT * np, np, u);
   const SplitProblem& q = P[pi];
-  const size_t e = (size_t)(u - ubv[PH_CRT * np + pi]) * 256 + threadIdx.x;
-  crt_unit<T>(q.M, q.N, q.ores, q.oplane, q.ldo, cp, q.ea, q.eh, q.abits, q.abits, q.alpha,
-              q.beta, q.C, q.ldc, e);
+  const size_t e0 = (size_t)(u - ubv[PH_CRT * np + pi]) * (256 * kCrtQuads) + threadIdx.x;
+#pragma unroll 1
+  for (int h = 0; h < kCrtQuads; ++h)
+    crt_unit<T>(q.M, q.N, q.ores, q.oplane, q.ldo, cp, q.ea, q.eh, q.abits, q.abits, q.alpha,
+                q.beta, q.C, q.ldc, e0 + (size_t)h * 256);
 }
 
 // Host plan of a grouped split batch (table, unit totals, workspace size).
@@ -1767,7 +1771,7 @@ void split_batch_layout(const BatchProblem* probs, int nprob, int t, char* ws, S
     if (big) gph += PH_GEMM_DB - PH_GEMM_D;
     const int tile = big ? 256 : 128;
     gx[gph] = ((q.M + tile - 1) / tile) * ((q.N + tile - 1) / tile) * t; gy[gph] = 1;
-    gx[PH_CRT] = (int)(((size_t)q.ldo * q.N / 4 + 255) / 256);
+    gx[PH_CRT] = (int)(((size_t)q.ldo * q.N / 4 + 256 * kCrtQuads - 1) / (256 * kCrtQuads));
     gy[PH_CRT] = 1;
     for (int ph = 0; ph < NPH; ++ph) {
       q.gx[ph] = gx[ph];
