@@ -177,6 +177,47 @@ __device__ __forceinline__ int sym_residue(double v, int m, double rm) {
   return ri;  // in [-m/2, m/2) subset of [-128, 127]
 }
 
+typedef unsigned int v4u __attribute__((ext_vector_type(4)));
+
+// Loads and stores through the global address space.  Pointers read from a
+// problem table are generic and would otherwise be accessed with flat
+// instructions.
+template <class T>
+__device__ __forceinline__ T gld(const T* p) {
+  return *(const __attribute__((address_space(1))) T*)p;
+}
+template <class T>
+__device__ __forceinline__ void gst(T* p, T v) {
+  *(__attribute__((address_space(1))) T*)p = v;
+}
+
+// Sixteen residues of integer-valued doubles |v| <= 2^52, packed four per word.
+// q = rint(v * fl(1/m)) has an absolute error of at most |v| / m * 2^-52 <=
+// 1/193, so r = v - q m satisfies |r| <= 0.5052 m: below 128 for every m <= 253
+// (integer r), which then needs no correction into [-m/2, m/2); the moduli 255
+// and 256 take sym_residue's correction.
+template <bool ADJ>
+__device__ __forceinline__ v4u residue_words(const double* v, int m, double rm) {
+  v4u w;
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    unsigned word = 0;
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      int r;
+      if constexpr (ADJ) {
+        r = sym_residue(v[c * 4 + b], m, rm);
+      } else {
+        const double x = v[c * 4 + b];
+        r = (int)fma(-rint(x * rm), (double)m, x);
+      }
+      word |= ((unsigned)(r & 0xff)) << (8 * b);
+    }
+    w[c] = word;
+  }
+  return w;
+}
+
 __device__ __forceinline__ int exponent_of(double x) {
   int e;
   frexp(fabs(x), &e);
@@ -251,33 +292,22 @@ __device__ __forceinline__ void split_a_kernel_body(int M, int K, const double* 
   for (int q = 0; q < 16; ++q) {
     int ii = t % 64, kk = t / 64 + 4 * q;
     int gi = i0 + ii, gk = k0 + kk;
-    tile[kk][ii] = (gi < M && gk < K) ? A[(size_t)gk * lda + gi] : 0.0;
+    tile[kk][ii] = (gi < M && gk < K) ? gld(A + (size_t)gk * lda + gi) : 0.0;
   }
   __syncthreads();
   // thread -> row i0 + t/4, 16 consecutive k
   const int ii = t / 4, kb = (t % 4) * 16;
   const int gi = i0 + ii;
   if (gi >= M) return;
-  const int sh = abits - ea[gi];
+  const int sh = abits - gld(ea + gi);
   double v[16];
 #pragma unroll
   for (int e = 0; e < 16; ++e) v[e] = rint(ldexp(tile[kb + e][ii], sh));
   for (int l = 0; l < cp.t; ++l) {
-    int m = cp.m[l];
-    double rm = cp.rm[l];
-    unsigned w[4];
-#pragma unroll
-    for (int c = 0; c < 4; ++c) {
-      unsigned word = 0;
-#pragma unroll
-      for (int b = 0; b < 4; ++b) {
-        int r = sym_residue(v[c * 4 + b], m, rm);
-        word |= ((unsigned)(r & 0xff)) << (8 * b);
-      }
-      w[c] = word;
-    }
-    uint4* dst = (uint4*)(res + l * plane + (size_t)gi * kp + k0 + kb);
-    *dst = make_uint4(w[0], w[1], w[2], w[3]);
+    const int m = cp.m[l];
+    const double rm = cp.rm[l];
+    v4u* dst = (v4u*)(res + l * plane + (size_t)gi * kp + k0 + kb);
+    gst(dst, m > 253 ? residue_words<true>(v, m, rm) : residue_words<false>(v, m, rm));
   }
 }
 
@@ -308,21 +338,10 @@ __device__ __forceinline__ void split_h_kernel_body(int K, int N, GenPtrs g, int
     v[e] = (k < K) ? rint(ldexp(gen_entry<KIND>(g, r0 + k, c0 + j), sh)) : 0.0;
   }
   for (int l = 0; l < cp.t; ++l) {
-    int m = cp.m[l];
-    double rm = cp.rm[l];
-    unsigned w[4];
-#pragma unroll
-    for (int c = 0; c < 4; ++c) {
-      unsigned word = 0;
-#pragma unroll
-      for (int b = 0; b < 4; ++b) {
-        int r = sym_residue(v[c * 4 + b], m, rm);
-        word |= ((unsigned)(r & 0xff)) << (8 * b);
-      }
-      w[c] = word;
-    }
-    uint4* dst = (uint4*)(res + l * plane + (size_t)j * kp + kb);
-    *dst = make_uint4(w[0], w[1], w[2], w[3]);
+    const int m = cp.m[l];
+    const double rm = cp.rm[l];
+    v4u* dst = (v4u*)(res + l * plane + (size_t)j * kp + kb);
+    gst(dst, m > 253 ? residue_words<true>(v, m, rm) : residue_words<false>(v, m, rm));
   }
 }
 
