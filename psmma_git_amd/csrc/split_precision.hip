@@ -430,7 +430,7 @@ constexpr int kFastModK = 16384;  // longest padded contraction of the FP32 redu
 // is exact), r = acc - q m lies in [0, 2m) and one unsigned min finishes.
 __device__ __forceinline__ unsigned mod_acc(int x, int m, float rmf) {
   const int q = (int)floorf(fmaf((float)x, rmf, -0.25f));
-  const unsigned r = (unsigned)(x - __mul24(q, m));
+  const unsigned r = (unsigned)(x + __mul24(q, -m));  // one 24-bit multiply-add
   return min(r, r - (unsigned)m);
 }
 
