@@ -40,6 +40,29 @@ def test_split_i8_parity(gpu, ora, kind, m, n, k):
     assert errs[14] <= max(errs[10], 1e-15)  # more moduli never lose accuracy beyond rounding
 
 
+@pytest.mark.parametrize("kind", ["cauchy", "toeplitz"])
+def test_split_i8_long_contraction(gpu, ora, kind):
+    """K = 16600 pads to 16640 > 16384: the direct-to-LDS kernel's residue
+    reduction takes its exact (integer remainder) path, and Toeplitz takes the
+    streamed-B form (the generator window holds at most 16384 steps)."""
+    import torch
+
+    m, n, k = 1100, 1037, 16600
+    print(f"case split long kind={kind} m={m} n={n} k={k}", flush=True)
+    rng = np.random.default_rng(case_seed("long", kind))
+    gens, dev = make_gens(kind, k + 5, rng)
+    a = rng.uniform(-1, 1, (m, k))
+    want = ora.structured_gemm_np(ORA_KIND[kind], 1.0, a, gens, 0, 0, n)
+    da = dev_colmajor(a)
+    dc = dev_colmajor(np.zeros((m, n)))
+    # t = 15: 46 -> 50 bits per operand at this K (t = 14 leaves Cauchy at ~8e-13)
+    run_dev(kind, 1.0, da, dev, gens, n, dc, 0.0, 0, 0, prec=api.PREC_SPLIT_I8, slices=15)
+    torch.cuda.synchronize()
+    err = ora.rel_fro(host(dc), want)
+    print(f"err={err:.3e}", flush=True)
+    assert err <= 1e-12
+
+
 def test_split_i8_zero_rows_and_columns(gpu, ora):
     import torch
 
