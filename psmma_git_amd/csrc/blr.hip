@@ -33,7 +33,7 @@ hipError_t launch_rrqr_structured(int kind, const GenPtrs& g, const int* r0, con
                                   int nblk, int m, int n, double tol, int kmax, int* rank,
                                   int* jpvt, long long jstride, double* q, int ldq,
                                   long long qstride, double* rpt, int ldr, long long rstride,
-                                  hipStream_t s);
+                                  double* cache, hipStream_t s);
 }  // namespace psmma
 
 using namespace psmma;
@@ -143,14 +143,17 @@ extern "C" int psmma_cauchyeig_blr_dev(int m, int nglob, double alpha, const dou
   const int nblk = nb * (nb - 1);
   GenPtrs g{fpole, u, v, difl, difr, nglob, nglob, nglob};
 
-  // workspace: block lists, RRQR outputs
+  // workspace: block lists, RRQR outputs, the evaluated blocks (each generated once,
+  // then read by every pass of its RRQR)
   const size_t qbytes = (size_t)nblk * leaf * kKmax * sizeof(double);
+  const size_t cbytes = (size_t)nblk * leaf * leaf * sizeof(double);
   size_t off_r0 = 0, off_c0 = align_up(off_r0 + sizeof(int) * nblk);
   size_t off_rank = align_up(off_c0 + sizeof(int) * nblk);
   size_t off_jp = align_up(off_rank + sizeof(int) * nblk);
   size_t off_q = align_up(off_jp + sizeof(int) * (size_t)nblk * leaf);
   size_t off_r = align_up(off_q + qbytes);
-  size_t fixed = align_up(off_r + qbytes);
+  size_t off_cache = align_up(off_r + qbytes);
+  size_t fixed = align_up(off_cache + cbytes);
   char* ws = (char*)workspace(0, fixed);
   if (!ws) return 1002;
   std::vector<int> hr0(nblk), hc0(nblk);
@@ -174,7 +177,7 @@ extern "C" int psmma_cauchyeig_blr_dev(int m, int nglob, double alpha, const dou
     if (e == hipSuccess)
       e = launch_rrqr_structured(KIND_CAUCHYEIG, g, dr0, dc0, nblk, leaf, leaf, tol, kKmax, drank,
                                  (int*)(ws + off_jp), leaf, dq, leaf, (long long)leaf * kKmax, drp,
-                                 kKmax, (long long)kKmax * leaf, s);
+                                 kKmax, (long long)kKmax * leaf, (double*)(ws + off_cache), s);
   }
   std::vector<int> hrank(nblk);
   if (e == hipSuccess && nblk > 0) {

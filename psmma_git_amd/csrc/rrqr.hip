@@ -19,6 +19,7 @@
 
 #include <cmath>
 #include <cstdio>
+#include <type_traits>
 
 #include "../../include/psmma.h"
 #include "psmma_common.h"
@@ -28,10 +29,20 @@ namespace psmma {
 constexpr int kRrqrKmax = 32;   // rank cap of the batched kernel
 constexpr int kRrqrMaxDim = 256;
 constexpr int kVld = kRrqrMaxDim + 1;  // padded LDS pitch: conflict-free row and column walks
+// Rows per group in the passes over a column: a group's inputs are all fetched
+// before any is used, so one LDS (or memory) latency is paid per group.  Full
+// groups carry no row test (a test per row turns each entry into a branch of
+// its own); the last, partial group clamps its row indices and multiplies the
+// rows past the end by zero.
+constexpr int kGrp = 8;
 
 // Block sources.  bind() stages what the block needs in LDS; col(c) returns
-// the per-column context a thread keeps for its current source column; at(i,
-// ctx) is the entry (i, c).
+// the per-column context a thread keeps for its current source column;
+// fetch(i, ctx) loads what entry (i, c) needs, values<G>(raw, rows, ctx, out)
+// finishes a group of G = kG entries; at(i, ctx) is one entry.  A source with
+// kCaches evaluates the block in the first pass (generate, in groups of kGrp)
+// and keeps a copy.  Sources that only load use groups of 16 rows (more loads
+// in flight per wave).
 struct SrcExplicit {
   const double* a;
   int lda;
@@ -40,6 +51,15 @@ struct SrcExplicit {
   using Ctx = const double*;
   __device__ void bind(int b, double*, int, int) { base = a + (size_t)b * stride; }
   __device__ Ctx col(int c) const { return base + (size_t)c * lda; }
+  static constexpr bool kCaches = false;
+  static constexpr int kG = 2 * kGrp;
+  using Raw = double;
+  __device__ Raw fetch(int i, Ctx p) const { return p[i]; }
+  template <int G>
+  __device__ void values(const Raw* r, const int*, Ctx, double* out) const {
+#pragma unroll
+    for (int j = 0; j < G; ++j) out[j] = r[j];
+  }
   __device__ double at(int i, Ctx p) const { return p[i]; }
 };
 
@@ -55,12 +75,41 @@ struct SrcGen {
     cb = c0[b];
   }
   __device__ Ctx col(int c) const { return cb + c; }
+  static constexpr bool kCaches = false;
+  static constexpr int kG = kGrp;
+  using Raw = double;
+  __device__ Raw fetch(int i, Ctx c) const { return gen_entry<KIND>(g, rb + i, c); }
+  template <int G>
+  __device__ void values(const Raw* r, const int*, Ctx, double* out) const {
+#pragma unroll
+    for (int j = 0; j < G; ++j) out[j] = r[j];
+  }
   __device__ double at(int i, Ctx c) const { return gen_entry<KIND>(g, rb + i, c); }
 };
 
+// n / d by the reciprocal refinement that FP64 division on this target runs
+// (v_rcp_f64, two Newton steps, one residual correction) without its range
+// scaling (v_div_scale / v_div_fmas / v_div_fixup).  Equal to n / d whenever
+// the scaling would not apply: finite, normal operands and quotient, away from
+// the ends of the exponent range.  The scaling steps pass a flag through VCC,
+// which serialises every division in a thread; this form leaves independent
+// quotients free to overlap.
+__device__ __forceinline__ double div_refined(double n, double d) {
+  double r = __builtin_amdgcn_rcp(d);
+  double e = fma(-d, r, 1.0);
+  r = fma(r, e, r);
+  e = fma(-d, r, 1.0);
+  r = fma(r, e, r);
+  const double q = n * r;
+  return fma(fma(-d, q, n), r, q);
+}
+
 // Eigen-Cauchy blocks: the row poles and weights staged in LDS, the column's
-// pole, next pole, DIFL, DIFR and normaliser in registers; the same
-// arithmetic as gen_entry<KIND_CAUCHYEIG>.
+// pole, next pole, DIFL, DIFR and normaliser in registers; the arithmetic of
+// gen_entry<KIND_CAUCHYEIG> with the quotient taken by div_refined (the
+// entries' denominators are pole differences and their numerators products of
+// weights: normal numbers far from the exponent range's ends, where division
+// would scale).
 struct SrcEigStaged {
   GenPtrs g;
   const int* r0;
@@ -85,9 +134,84 @@ struct SrcEigStaged {
     const int cg = cb + c;
     return Ctx{g.p0[cg], g.p0[min(cg + 1, g.korder - 1)], g.p3[cg], g.p4[cg], g.p2[cg], cg};
   }
+  static constexpr bool kCaches = false;
+  static constexpr int kG = kGrp;
+  struct Raw {
+    double f, u;
+  };
+  __device__ Raw fetch(int i, const Ctx&) const { return Raw{s[i], s[kRrqrMaxDim + i]}; }
+  __device__ double den(double f, int i, const Ctx& x) const {
+    return (rb + i <= x.cg) ? ((f - x.fc) - x.dl) : ((f - x.fc1) - x.dr);
+  }
+  // div_refined over the group, written stage by stage so that the G
+  // independent quotients interleave
+  template <int G>
+  __device__ void values(const Raw* r, const int* ii, const Ctx& x, double* out) const {
+    double d[G], nu[G], rc[G], e[G];
+#pragma unroll
+    for (int j = 0; j < G; ++j) {
+      d[j] = den(r[j].f, ii[j], x);
+      nu[j] = r[j].u * x.vc;
+    }
+#pragma unroll
+    for (int j = 0; j < G; ++j) rc[j] = __builtin_amdgcn_rcp(d[j]);
+#pragma unroll
+    for (int it = 0; it < 2; ++it) {
+#pragma unroll
+      for (int j = 0; j < G; ++j) e[j] = fma(-d[j], rc[j], 1.0);
+#pragma unroll
+      for (int j = 0; j < G; ++j) rc[j] = fma(rc[j], e[j], rc[j]);
+    }
+#pragma unroll
+    for (int j = 0; j < G; ++j) e[j] = nu[j] * rc[j];
+#pragma unroll
+    for (int j = 0; j < G; ++j) out[j] = fma(fma(-d[j], e[j], nu[j]), rc[j], e[j]);
+  }
   __device__ double at(int i, const Ctx& x) const {
-    const double den = (rb + i <= x.cg) ? ((s[i] - x.fc) - x.dl) : ((s[i] - x.fc1) - x.dr);
-    return (s[kRrqrMaxDim + i] * x.vc) / den;
+    return div_refined(s[kRrqrMaxDim + i] * x.vc, den(s[i], i, x));
+  }
+};
+
+// Eigen-Cauchy blocks evaluated once: the first pass (the initial column
+// norms) computes each entry as SrcEigStaged does and stores it in a row-major
+// copy of the block in device memory, entry (i, j) at i * n + j, so that the
+// threads of a wave (one column each) read a row in one coalesced access;
+// every later pass reads the copy instead of dividing again.  The copy is
+// written and read by the one workgroup that owns the block, and the first
+// pass ends in a workgroup barrier.
+struct SrcEigCached {
+  SrcEigStaged gen;
+  double* cache;  // nblk blocks of m * n entries
+  int ld;
+  double* base;
+  struct Ctx {
+    SrcEigStaged::Ctx g;
+    int c;
+  };
+  static constexpr bool kCaches = true;
+  static constexpr int kG = 2 * kGrp;
+  __device__ void bind(int b, double* st, int m, int n) {
+    gen.bind(b, st, m, n);
+    ld = n;
+    base = cache + (size_t)b * m * n;
+  }
+  __device__ Ctx col(int c) const { return Ctx{gen.col(c), c}; }
+  using Raw = double;
+  __device__ Raw fetch(int i, const Ctx& x) const { return base[i * ld + x.c]; }
+  template <int G>
+  __device__ void values(const Raw* r, const int*, const Ctx&, double* out) const {
+#pragma unroll
+    for (int j = 0; j < G; ++j) out[j] = r[j];
+  }
+  __device__ double at(int i, const Ctx& x) const { return base[i * ld + x.c]; }
+  __device__ void generate(const int* ii, const Ctx& x, double* out) const {
+    SrcEigStaged::Raw r[kGrp];
+#pragma unroll
+    for (int j = 0; j < kGrp; ++j) r[j] = gen.fetch(ii[j], x.g);
+    __builtin_amdgcn_sched_barrier(0);
+    gen.values<kGrp>(r, ii, x.g, out);
+#pragma unroll
+    for (int j = 0; j < kGrp; ++j) base[ii[j] * ld + x.c] = out[j];
   }
 };
 
@@ -172,23 +296,34 @@ __global__ __launch_bounds__(kThreads, 1) void rrqr_kernel(Src src, int m, int n
   {
     int i0, i1;
     split(0, i0, i1);
-    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+    constexpr int G0 = Src::kCaches ? kGrp : Src::kG;
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    auto group = [&](int i, auto tail) {
+      int ii[G0];
+#pragma unroll
+      for (int j = 0; j < G0; ++j) ii[j] = decltype(tail)::value ? min(i + j, i1 - 1) : i + j;
+      double x[G0];
+      if constexpr (Src::kCaches) {
+        src.generate(ii, cx, x);
+      } else {
+        typename Src::Raw r[G0];
+#pragma unroll
+        for (int j = 0; j < G0; ++j) r[j] = src.fetch(ii[j], cx);
+        __builtin_amdgcn_sched_barrier(0);
+        src.template values<G0>(r, ii, cx, x);
+      }
+#pragma unroll
+      for (int j = 0; j < G0; ++j) {
+        if (decltype(tail)::value) x[j] *= (i + j < i1) ? 1.0 : 0.0;
+        acc[j & 3] += x[j] * x[j];
+      }
+    };
     if (pos < n) {
       int i = i0;
-      for (; i + 3 < i1; i += 4) {
-        const double x0 = src.at(i, cx), x1 = src.at(i + 1, cx), x2 = src.at(i + 2, cx),
-                     x3 = src.at(i + 3, cx);
-        s0 += x0 * x0;
-        s1 += x1 * x1;
-        s2 += x2 * x2;
-        s3 += x3 * x3;
-      }
-      for (; i < i1; ++i) {
-        const double x = src.at(i, cx);
-        s0 += x * x;
-      }
+      for (; i + G0 <= i1; i += G0) group(i, std::false_type{});
+      if (i < i1) group(i, std::true_type{});
     }
-    const double s = combine((s0 + s1) + (s2 + s3));
+    const double s = combine((acc[0] + acc[1]) + (acc[2] + acc[3]));
     if (pos < n) vn1 = vn2 = sqrt(s);
   }
 
@@ -319,17 +454,30 @@ __global__ __launch_bounds__(kThreads, 1) void rrqr_kernel(Src src, int m, int n
     if (live) {
       int i0, i1;
       split(k, i0, i1);
-      double g0 = 0.0, g1 = 0.0, g2 = 0.0, g3 = 0.0;
+      constexpr int G = Src::kG;
+      double acc[4] = {0.0, 0.0, 0.0, 0.0};
       const double* vk = V + k * kVld;
+      auto group = [&](int i, auto tail) {
+        typename Src::Raw r[G];
+        double w[G];
+        int ii[G];
+#pragma unroll
+        for (int j = 0; j < G; ++j) {
+          ii[j] = decltype(tail)::value ? min(i + j, i1 - 1) : i + j;
+          r[j] = src.fetch(ii[j], cx);
+          w[j] = vk[ii[j]];
+          if (decltype(tail)::value) w[j] *= (i + j < i1) ? 1.0 : 0.0;
+        }
+        __builtin_amdgcn_sched_barrier(0);
+        double x[G];
+        src.template values<G>(r, ii, cx, x);
+#pragma unroll
+        for (int j = 0; j < G; ++j) acc[j & 3] += x[j] * w[j];
+      };
       int i = i0;
-      for (; i + 3 < i1; i += 4) {
-        g0 += src.at(i, cx) * vk[i];
-        g1 += src.at(i + 1, cx) * vk[i + 1];
-        g2 += src.at(i + 2, cx) * vk[i + 2];
-        g3 += src.at(i + 3, cx) * vk[i + 3];
-      }
-      for (; i < i1; ++i) g0 += src.at(i, cx) * vk[i];
-      g = (g0 + g1) + (g2 + g3);
+      for (; i + G <= i1; i += G) group(i, std::false_type{});
+      if (i < i1) group(i, std::true_type{});
+      g = (acc[0] + acc[1]) + (acc[2] + acc[3]);
     }
     g = combine(g);
     bool recompute = false;
@@ -357,35 +505,56 @@ __global__ __launch_bounds__(kThreads, 1) void rrqr_kernel(Src src, int m, int n
           vn1 *= sqrt(tt);
       }
     }
-    __syncthreads();  // F(pos, k) visible to both twins
+    // (uniform; the barrier also makes F(pos, k) visible to both twins)
+    if (!__syncthreads_or(recompute)) continue;
     double sr = 0.0;
     if (recompute) {
       int i0, i1;
       split(k + 1, i0, i1);
-      double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
-      int i = i0;
-      for (; i + 3 < i1; i += 4) {
-        double x0 = src.at(i, cx), x1 = src.at(i + 1, cx), x2 = src.at(i + 2, cx),
-               x3 = src.at(i + 3, cx);
-        for (int l = 0; l <= k; ++l) {
-          const double fl = Frow[l];
-          const double* vl = V + l * kVld + i;
-          x0 -= vl[0] * fl;
-          x1 -= vl[1] * fl;
-          x2 -= vl[2] * fl;
-          x3 -= vl[3] * fl;
+      constexpr int G = Src::kG;
+      double acc[4] = {0.0, 0.0, 0.0, 0.0};
+      auto group = [&](int i, auto tail) {
+        typename Src::Raw r[G];
+        int ii[G];
+#pragma unroll
+        for (int j = 0; j < G; ++j) {
+          ii[j] = decltype(tail)::value ? min(i + j, i1 - 1) : i + j;
+          r[j] = src.fetch(ii[j], cx);
         }
-        s0 += x0 * x0;
-        s1 += x1 * x1;
-        s2 += x2 * x2;
-        s3 += x3 * x3;
-      }
-      for (; i < i1; ++i) {
-        double x = src.at(i, cx);
-        for (int l = 0; l <= k; ++l) x -= V[l * kVld + i] * Frow[l];
-        s0 += x * x;
-      }
-      sr = (s0 + s1) + (s2 + s3);
+        __builtin_amdgcn_sched_barrier(0);
+        double x[G];
+        src.template values<G>(r, ii, cx, x);
+        // two reflectors per LDS round trip; each entry still subtracts them in order
+        int l = 0;
+        for (; l + 1 <= k; l += 2) {
+          const double f0 = Frow[l], f1 = Frow[l + 1];
+          const double* v0 = V + l * kVld;
+          const double* v1 = v0 + kVld;
+          double p0[G], p1[G];
+#pragma unroll
+          for (int j = 0; j < G; ++j) {
+            p0[j] = v0[ii[j]];
+            p1[j] = v1[ii[j]];
+          }
+#pragma unroll
+          for (int j = 0; j < G; ++j) x[j] = (x[j] - p0[j] * f0) - p1[j] * f1;
+        }
+        if (l <= k) {
+          const double fl = Frow[l];
+          const double* vl = V + l * kVld;
+#pragma unroll
+          for (int j = 0; j < G; ++j) x[j] -= vl[ii[j]] * fl;
+        }
+#pragma unroll
+        for (int j = 0; j < G; ++j) {
+          if (decltype(tail)::value) x[j] *= (i + j < i1) ? 1.0 : 0.0;
+          acc[j & 3] += x[j] * x[j];
+        }
+      };
+      int i = i0;
+      for (; i + G <= i1; i += G) group(i, std::false_type{});
+      if (i < i1) group(i, std::true_type{});
+      sr = (acc[0] + acc[1]) + (acc[2] + acc[3]);
     }
     sr = combine(sr);
     if (recompute) vn1 = vn2 = sqrt(sr);
@@ -451,10 +620,16 @@ hipError_t launch_rrqr(const Src& src, int nblk, int m, int n, double tol, int k
 }
 
 // Blocks (r0[b], c0[b]) of size m x n of a structured matrix (device block lists).
+// cache: optional workspace of nblk * m * n doubles; eigen-Cauchy blocks are then
+// evaluated once and read back from it.
 hipError_t launch_rrqr_structured(int kind, const GenPtrs& g, const int* r0, const int* c0, int nblk,
                                   int m, int n, double tol, int kmax, int* rank, int* jpvt,
                                   long long jstride, double* q, int ldq, long long qstride,
-                                  double* rpt, int ldr, long long rstride, hipStream_t s) {
+                                  double* rpt, int ldr, long long rstride, double* cache,
+                                  hipStream_t s) {
+  if (kind == KIND_CAUCHYEIG && cache)
+    return launch_rrqr(SrcEigCached{SrcEigStaged{g, r0, c0, 0, 0, nullptr}, cache, 0, nullptr}, nblk,
+                       m, n, tol, kmax, rank, jpvt, jstride, q, ldq, qstride, rpt, ldr, rstride, s);
   switch (kind) {
     case KIND_CAUCHY:
       return launch_rrqr(SrcGen<KIND_CAUCHY>{g, r0, c0, 0, 0}, nblk, m, n, tol, kmax, rank, jpvt,
