@@ -82,6 +82,33 @@ def test_blr_merge_is_orthogonal(gpu, ora):
     assert np.max(np.abs(x.T @ x - np.eye(n))) < 1e-11
 
 
+def test_blr_structured_rrqr_equals_explicit(gpu, ora):
+    """The product's RRQR evaluates each block from the generators once (a quotient
+    equal to IEEE division for these operands) and reads that copy in every pass; run
+    on the same blocks materialised by the oracle's CauchylikeEig formula, the
+    explicit-input RRQR must make the same rank decisions, block for block."""
+    import torch
+
+    n, leaf = 1024, 256
+    gens, gh, rng = _setup(n, 34)
+    c = ora.cauchyeig_matrix(*gh, 0, n, 0, n)
+    nb = n // leaf
+    blocks = [c[i * leaf:(i + 1) * leaf, j * leaf:(j + 1) * leaf]
+              for i in range(nb) for j in range(nb) if i != j]
+    t = torch.from_numpy(np.stack([b.T for b in blocks])).cuda().contiguous()
+    rank, _, _, _ = api.rrqr_batched_dev(t, 1e-12, 32)
+    rank = rank.cpu().numpy()
+    dx = api.colmajor_empty(n, n)
+    st = api.cauchyeig_blr_dev(1.0, _dev(rng.standard_normal((n, n))), *gens, dx, leaf=leaf,
+                               tol=1e-12)
+    torch.cuda.synchronize()
+    low = rank[(rank > 0) & (rank < leaf)]
+    assert st["lowrank"] == len(low)
+    assert st["rank_sum"] == int(low.sum())
+    assert st["zero"] == int(np.sum(rank == 0))
+    assert st["dense"] == int(np.sum((rank < 0) | (rank >= leaf)))
+
+
 def test_blr_rejects_ragged_leaf(gpu):
     import torch
 
