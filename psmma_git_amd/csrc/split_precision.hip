@@ -1600,26 +1600,21 @@ struct SplitProblem {
   int ub[NPH];  // first unit of this problem in the phase
 };
 
-// The problem owning unit u of a phase: binary search over the phase's
-// contiguous array of first units (ubv[ph * np + i] = P[i].ub[ph]).
-__device__ __forceinline__ int find_split_prob(const int* __restrict__ ub, int np, int u) {
-  int lo = 0, hi = np - 1;
-  while (lo < hi) {
-    int mid = (lo + hi + 1) >> 1;
-    if (ub[mid] <= u) lo = mid; else hi = mid - 1;
-  }
-  return lo;
-}
+// The problem owning unit u of a phase and the unit's index within that
+// problem: one table entry per unit, filled on the host when the batch is
+// bound, so a workgroup finds its problem with a single load (a search over
+// the problem list costs a chain of dependent loads per workgroup, comparable
+// to the whole work of a small unit).
+using UnitOwner = int2;
 
 // One kernel per phase: each carries only its own LDS and registers.
 template <int ph>
 __global__ __launch_bounds__(256) void gsplit_phase_kernel(const SplitProblem* __restrict__ P,
-                                                           const int* __restrict__ ubv, int np,
+                                                           const UnitOwner* __restrict__ own,
                                                            CrtParams cp) {
-  const int u = blockIdx.x;
-  const int pi = find_split_prob(ubv + ph * np, np, u);
-  const SplitProblem& q = P[pi];
-  const int lu = u - ubv[ph * np + pi];
+  const UnitOwner o = own[blockIdx.x];
+  const SplitProblem& q = P[o.x];
+  const int lu = o.y;
   const int bx = lu % q.gx[ph], by = lu / q.gx[ph];
   const bool dense = q.bmode == BMODE_DENSE;
   (void)dense;
@@ -1668,12 +1663,11 @@ constexpr int kBatchKmax = 16384;  // longest contraction of a generator-window 
 
 template <int BMODE>
 __global__ __launch_bounds__(256, 2) void gsplit_gemm_kernel(const SplitProblem* __restrict__ P,
-                                                             const int* __restrict__ ubv, int np,
-                                                             int ph, CrtParams cp) {
-  const int u = blockIdx.x;
-  const int pi = find_split_prob(ubv + ph * np, np, u);
-  const SplitProblem& q = P[pi];
-  const int lu = u - ubv[ph * np + pi];
+                                                             const UnitOwner* __restrict__ own,
+                                                             CrtParams cp) {
+  const UnitOwner o = own[blockIdx.x];
+  const SplitProblem& q = P[o.x];
+  const int lu = o.y;
   const int ntn = (q.N + 127) / 128, ntm = (q.M + 127) / 128;
   const int l = lu / (ntm * ntn), r = lu % (ntm * ntn);
   gemm_i8_body<BMODE, 128, 128, 128, 2, 2, kBatchKmax>(q.M, q.N, q.kp, q.ares, q.bres, q.aplane,
@@ -1686,29 +1680,26 @@ constexpr int kBigTile = 512;  // smallest min(M, N) and K of the large class (m
 
 template <int BMODE>
 __global__ __launch_bounds__(512, 1) void gsplit_gemm_big_kernel(const SplitProblem* __restrict__ P,
-                                                                 const int* __restrict__ ubv,
-                                                                 int np, int ph, CrtParams cp) {
-  const int u = blockIdx.x;
-  const int pi = find_split_prob(ubv + ph * np, np, u);
-  const SplitProblem& q = P[pi];
-  const int lu = u - ubv[ph * np + pi];
+                                                                 const UnitOwner* __restrict__ own,
+                                                                 CrtParams cp) {
+  const UnitOwner o = own[blockIdx.x];
+  const SplitProblem& q = P[o.x];
+  const int lu = o.y;
   const int ntn = (q.N + 255) / 256, ntm = (q.M + 255) / 256;
   const int l = lu / (ntm * ntn), r = lu % (ntm * ntn);
   gemm_i8_dma_body<BMODE, kBatchKmax, 256>(q.M, q.N, q.kp, q.ares, q.bres, q.aplane, q.bplane,
                                            q.lpad, cp, q.ores, q.oplane, q.ldo, r / ntn, r % ntn, l);
 }
 
-// A grouped CRT unit is 256 threads x kCrtQuads groups of four rows (the
-// problem lookup of a workgroup is paid once per 1024 kCrtQuads outputs).
+// A grouped CRT unit is 256 threads x kCrtQuads groups of four rows.
 constexpr int kCrtQuads = 4;
 template <int T>
 __global__ __launch_bounds__(256) void gsplit_crt_kernel(const SplitProblem* __restrict__ P,
-                                                         const int* __restrict__ ubv, int np,
+                                                         const UnitOwner* __restrict__ own,
                                                          CrtParams cp) {
-  const int u = blockIdx.x;
-  const int pi = find_split_prob(ubv + PH_CRT * np, np, u);
-  const SplitProblem& q = P[pi];
-  const size_t e0 = (size_t)(u - ubv[PH_CRT * np + pi]) * (256 * kCrtQuads) + threadIdx.x;
+  const UnitOwner o = own[blockIdx.x];
+  const SplitProblem& q = P[o.x];
+  const size_t e0 = (size_t)o.y * (256 * kCrtQuads) + threadIdx.x;
 #pragma unroll 1
   for (int h = 0; h < kCrtQuads; ++h)
     crt_unit<T>(q.M, q.N, q.ores, q.oplane, q.ldo, cp, q.ea, q.eh, q.abits, q.abits, q.alpha,
@@ -1802,10 +1793,24 @@ void split_batch_layout(const BatchProblem* probs, int nprob, int t, char* ws, S
   sb.ws_bytes = off;
 }
 
-// Enqueues the grouped pipeline; dtab is the device copy of sb.tab.
+// Device layout of a bound batch: the problem table, then the unit owners of
+// every phase in phase order (UnitOwner is 8 bytes and SplitProblem a multiple
+// of 8 bytes, so the owners are aligned).
+static_assert(sizeof(SplitProblem) % 8 == 0, "owner table alignment");
+size_t owner_count(const SplitBatch& sb) {
+  size_t n = 0;
+  for (int ph = 0; ph < NPH; ++ph) n += (size_t)sb.total[ph];
+  return n;
+}
+const UnitOwner* phase_owners(const SplitBatch& sb, const SplitProblem* dtab, int ph) {
+  size_t off = 0;
+  for (int p = 0; p < ph; ++p) off += (size_t)sb.total[p];
+  return (const UnitOwner*)(dtab + sb.tab.size()) + off;
+}
+
+// Enqueues the grouped pipeline; dtab is the device copy of the bound layout.
 int split_batch_enqueue(const SplitBatch& sb, const SplitProblem* dtab, hipStream_t s, int* nodes) {
   const int np = (int)sb.tab.size();
-  const int* dub = (const int*)(dtab + np);  // the phases' unit prefixes follow the table
   int n = 0;
   if (np == 0) {
     if (nodes) *nodes = 0;
@@ -1819,44 +1824,45 @@ int split_batch_enqueue(const SplitBatch& sb, const SplitProblem* dtab, hipStrea
                 : ph == PH_EHFILL ? gsplit_phase_kernel<PH_EHFILL>
                 : ph == PH_SPLITA ? gsplit_phase_kernel<PH_SPLITA>
                                   : gsplit_phase_kernel<PH_SPLITB>;
-    hipLaunchKernelGGL(kern, dim3(sb.total[ph]), dim3(256), 0, s, dtab, dub, np,
-                       sb.cp);
+    hipLaunchKernelGGL(kern, dim3(sb.total[ph]), dim3(256), 0, s, dtab,
+                       phase_owners(sb, dtab, ph), sb.cp);
     ++n;
   }
   if (sb.total[PH_GEMM_D]) {
     hipLaunchKernelGGL(gsplit_gemm_kernel<BMODE_DENSE>, dim3(sb.total[PH_GEMM_D]), dim3(256), 0, s,
-                       dtab, dub, np, (int)PH_GEMM_D, sb.cp);
+                       dtab, phase_owners(sb, dtab, PH_GEMM_D), sb.cp);
     ++n;
   }
   if (sb.total[PH_GEMM_T]) {
     hipLaunchKernelGGL(gsplit_gemm_kernel<BMODE_TOE>, dim3(sb.total[PH_GEMM_T]), dim3(256), 0, s,
-                       dtab, dub, np, (int)PH_GEMM_T, sb.cp);
+                       dtab, phase_owners(sb, dtab, PH_GEMM_T), sb.cp);
     ++n;
   }
   if (sb.total[PH_GEMM_H]) {
     hipLaunchKernelGGL(gsplit_gemm_kernel<BMODE_HANK>, dim3(sb.total[PH_GEMM_H]), dim3(256), 0, s,
-                       dtab, dub, np, (int)PH_GEMM_H, sb.cp);
+                       dtab, phase_owners(sb, dtab, PH_GEMM_H), sb.cp);
     ++n;
   }
   if (sb.total[PH_GEMM_DB]) {
     hipLaunchKernelGGL(gsplit_gemm_big_kernel<BMODE_DENSE>, dim3(sb.total[PH_GEMM_DB]), dim3(512),
-                       0, s, dtab, dub, np, (int)PH_GEMM_DB, sb.cp);
+                       0, s, dtab, phase_owners(sb, dtab, PH_GEMM_DB), sb.cp);
     ++n;
   }
   if (sb.total[PH_GEMM_TB]) {
     hipLaunchKernelGGL(gsplit_gemm_big_kernel<BMODE_TOE>, dim3(sb.total[PH_GEMM_TB]), dim3(512), 0,
-                       s, dtab, dub, np, (int)PH_GEMM_TB, sb.cp);
+                       s, dtab, phase_owners(sb, dtab, PH_GEMM_TB), sb.cp);
     ++n;
   }
   if (sb.total[PH_GEMM_HB]) {
     hipLaunchKernelGGL(gsplit_gemm_big_kernel<BMODE_HANK>, dim3(sb.total[PH_GEMM_HB]), dim3(512), 0,
-                       s, dtab, dub, np, (int)PH_GEMM_HB, sb.cp);
+                       s, dtab, phase_owners(sb, dtab, PH_GEMM_HB), sb.cp);
     ++n;
   }
   dim3 cg(sb.total[PH_CRT]);
+  const UnitOwner* crt_own = phase_owners(sb, dtab, PH_CRT);
 #define PSMMA_GCRT_CASE(TT)                                                              \
   case TT:                                                                               \
-    hipLaunchKernelGGL(gsplit_crt_kernel<TT>, cg, dim3(256), 0, s, dtab, dub, np, sb.cp); \
+    hipLaunchKernelGGL(gsplit_crt_kernel<TT>, cg, dim3(256), 0, s, dtab, crt_own, sb.cp); \
     break;
   switch (sb.t) {
     PSMMA_GCRT_CASE(2) PSMMA_GCRT_CASE(3) PSMMA_GCRT_CASE(4) PSMMA_GCRT_CASE(5) PSMMA_GCRT_CASE(6)
@@ -1901,15 +1907,25 @@ int gsplit_bind(void* hv, void* ws) {
   GSplit* h = (GSplit*)hv;
   split_batch_layout(h->probs.data(), (int)h->probs.size(), h->sb.t, (char*)ws, h->sb);
   if (h->sb.tab.empty()) return 0;
-  const size_t np = h->sb.tab.size();
-  size_t bytes = sizeof(SplitProblem) * np;
-  std::vector<int> ubv(NPH * np);
-  for (int ph = 0; ph < NPH; ++ph)
-    for (size_t i = 0; i < np; ++i) ubv[ph * np + i] = h->sb.tab[i].ub[ph];
-  if (hipMalloc(&h->dtab, bytes + sizeof(int) * ubv.size()) != hipSuccess) return 1002;
-  hipError_t e = hipMemcpy(h->dtab, h->sb.tab.data(), bytes, hipMemcpyHostToDevice);
-  if (e == hipSuccess)
-    e = hipMemcpy(h->dtab + np, ubv.data(), sizeof(int) * ubv.size(), hipMemcpyHostToDevice);
+  const SplitBatch& sb = h->sb;
+  const size_t np = sb.tab.size();
+  const size_t bytes = sizeof(SplitProblem) * np;
+  // unit owners, phase by phase: problem i holds units [ub, next problem's ub)
+  std::vector<UnitOwner> own(owner_count(sb));
+  size_t off = 0;
+  for (int ph = 0; ph < NPH; ++ph) {
+    for (size_t i = 0; i < np; ++i) {
+      const int u0 = sb.tab[i].ub[ph];
+      const int u1 = (i + 1 < np) ? sb.tab[i + 1].ub[ph] : sb.total[ph];
+      for (int u = u0; u < u1; ++u) own[off + u] = UnitOwner{(int)i, u - u0};
+    }
+    off += (size_t)sb.total[ph];
+  }
+  if (hipMalloc(&h->dtab, bytes + sizeof(UnitOwner) * own.size()) != hipSuccess) return 1002;
+  hipError_t e = hipMemcpy(h->dtab, sb.tab.data(), bytes, hipMemcpyHostToDevice);
+  if (e == hipSuccess && !own.empty())
+    e = hipMemcpy(h->dtab + np, own.data(), sizeof(UnitOwner) * own.size(),
+                  hipMemcpyHostToDevice);
   return e == hipSuccess ? 0 : 1000 + (int)e;
 }
 
