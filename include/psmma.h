@@ -253,6 +253,33 @@ int psmma_rrqr_batched_dev(int nblk, int m, int n, const double* a, int lda, lon
                            void* stream);
 
 /* ===========================================================================
+ * 5b. Batched generator-only RRLU of eigen-Cauchy blocks (device pointers).
+ *
+ * Compress_cauchyEig2 (SRC/CauchyHssEig.f90:2640-2829) over rrluCauchyEigrow
+ * (:7-212), the compression ConstCauchyEiglowrank applies to off-diagonal
+ * eigen-Cauchy tiles (SRC/Auxil_2p5D.f90:642-748).  Block b has the 1-based
+ * global rows pl[b*m + i] and columns pu[b*n + j] (m, n <= 256) of the matrix
+ * u_i v_j / (fpole_i - lambda_j) of psmma_cauchyeig_dev (order korder, every
+ * difference rebuilt from fpole, difl, difr through the global indices); the
+ * block itself is never formed.  Rook pivoting, complete pivoting at the first
+ * convergence test, a swap pass every 5th step (rho = 1.2); rank[b] = the first
+ * step whose pivot falls below tol times the largest pivot so far (the
+ * reference's TOL is the default-real 1.0E-17), min(m,n) at full rank, -2 when
+ * an index lies outside [1, korder].  For 0 < rank < min(m,n) and rank <= kmax:
+ *   q + b*stride_q (ldq):  Q (m x rank) = P^-1 [I; Z_i W_j / (F_i - F_j)];
+ *   y + b*stride_y (ldy):  Y (rank x n), the chosen rows of the block,
+ * so that block ~= Q * Y; otherwise q and y are left unwritten.  nswap (may be
+ * NULL): swap-pass exchanges per block.  One workgroup per block; results are
+ * bitwise those of the CPU restatement.  Asynchronous on `stream`.  Returns 0
+ * or -(argument number). */
+int psmma_rrlu_cauchyeig_batched_dev(int nblk, int m, int n, const int* pl, const int* pu,
+                                     const double* fpole, const double* u, const double* v,
+                                     const double* difl, const double* difr, int korder,
+                                     double tol, int kmax, int* rank, int* nswap, double* q,
+                                     int ldq, long long stride_q, double* y, int ldy,
+                                     long long stride_y, void* stream);
+
+/* ===========================================================================
  * 6. Compressed eigenvector-merge product (device pointers).
  *
  * X(m x n) = alpha * A(m x n) * C + beta * X with C the n x n eigen-Cauchy

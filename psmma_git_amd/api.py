@@ -68,6 +68,9 @@ _SIGS = {
     "psmma_secular_dev": ([_I, _P, _P, _D, _P, _P, _P, _P, _P], _I),
     "psmma_rrqr_batched_dev": ([_I, _I, _I, _P, _I, ctypes.c_longlong, _D, _I, _P, _P, _P, _I,
                                 ctypes.c_longlong, _P, _I, ctypes.c_longlong, _P], _I),
+    "psmma_rrlu_cauchyeig_batched_dev": ([_I, _I, _I, _P, _P, _P, _P, _P, _P, _P, _I, _D, _I, _P,
+                                          _P, _P, _I, ctypes.c_longlong, _P, _I,
+                                          ctypes.c_longlong, _P], _I),
     "psmma_cauchyeig_blr_dev": ([_I, _I, _D, _P, _I, _P, _P, _P, _P, _P, _I, _D, _D, _P, _I, _P,
                                  _P], _I),
     "psmma_batched_dev": ([_I, _P, _I, _I, _P], _I),
@@ -332,6 +335,35 @@ def cauchyeig_blr_dev(alpha, a, fpole, u, v, difl, difr, c, leaf=256, tol=1e-12,
                                          float(tol), beta, pc, ldc, st, _stream_ptr(stream)),
            "psmma_cauchyeig_blr_dev")
     return dict(zip(("blocks", "lowrank", "dense", "zero", "rank_sum"), list(st)))
+
+
+RRLU_TOL = float(np.float32(1.0e-17))  # the reference's default-real TOL
+
+
+def rrlu_cauchyeig_batched_dev(pl, pu, fpole, u, v, difl, difr, tol=RRLU_TOL, kmax=32,
+                               stream=None):
+    """Batched generator-only RRLU of eigen-Cauchy blocks (Compress_cauchyEig2).  pl
+    (nblk, m) / pu (nblk, n): int32 device tensors of 1-based global rows / columns;
+    generators: float64 device vectors of the matrix order.  Returns (rank, nswap, q,
+    y): rank, nswap (nblk,) int32; q (nblk, kq, m) and y (nblk, n, kq) hold, as
+    column-major views q[b].T and y[b].T, Q (m x rank) and Y (rank x n) in their
+    leading columns / rows when 0 < rank < min(m, n) and rank <= kmax,
+    kq = min(kmax, m, n)."""
+    import torch
+    nblk, m = pl.shape
+    n = pu.shape[1]
+    kq = max(1, min(kmax, m, n))
+    dev = pl.device
+    rank = torch.empty(nblk, dtype=torch.int32, device=dev)
+    nswap = torch.empty(nblk, dtype=torch.int32, device=dev)
+    q = torch.zeros((nblk, kq, m), dtype=torch.float64, device=dev)
+    y = torch.zeros((nblk, n, kq), dtype=torch.float64, device=dev)
+    _check(lib().psmma_rrlu_cauchyeig_batched_dev(
+        nblk, m, n, pl.data_ptr(), pu.data_ptr(), fpole.data_ptr(), u.data_ptr(), v.data_ptr(),
+        difl.data_ptr(), difr.data_ptr(), fpole.numel(), float(tol), kmax, rank.data_ptr(),
+        nswap.data_ptr(), q.data_ptr(), m, kq * m, y.data_ptr(), kq, n * kq,
+        _stream_ptr(stream)), "psmma_rrlu_cauchyeig_batched_dev")
+    return rank, nswap, q, y
 
 
 def rrqr_batched_dev(blocks, tol, kmax=RRQR_KMAX, stream=None):
