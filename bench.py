@@ -229,7 +229,30 @@ def eig_merge_config(seed, torch, api, inputs, n=16384):
         "stats": stats}
     del x, ref, tq
     torch.cuda.empty_cache()
+    out["rrlu_leaf256_blocks"] = rrlu_blocks(td, u, v, difl, difr, torch, api)
     return out
+
+
+def rrlu_blocks(td, u, v, difl, difr, torch, api, leaf=256):
+    """Generator-only RRLU (Compress_cauchyEig2) of every off-diagonal leaf block of the
+    eigen-Cauchy matrix, at the reference's TOL and at 1e-12 (the compressed product's
+    tolerance); outputs preallocated, HIP events around the launches."""
+    n = td.numel()
+    nb = n // leaf
+    pairs = torch.tensor([(i, j) for i in range(nb) for j in range(nb) if i != j],
+                         dtype=torch.int32, device="cuda")
+    ar = torch.arange(1, leaf + 1, dtype=torch.int32, device="cuda")
+    pl = (pairs[:, :1] * leaf + ar).contiguous()
+    pu = (pairs[:, 1:] * leaf + ar).contiguous()
+    res = {"blocks": int(pairs.shape[0]), "leaf": leaf}
+    buf = api.rrlu_cauchyeig_batched_dev(pl, pu, td, u, v, difl, difr, api.RRLU_TOL, 32)
+    for name, tol in (("tol_ref", api.RRLU_TOL), ("tol_1e-12", 1e-12)):
+        ms = ev_time(lambda: api.rrlu_cauchyeig_batched_dev(pl, pu, td, u, v, difl, difr, tol,
+                                                            32, out=buf), 3, torch)
+        rank = buf[0].cpu().numpy()
+        res[name] = {"tol": tol, "ms": round(ms, 3), "rank_sum": int(rank.sum()),
+                     "rank_max": int(rank.max()), "swaps": int(buf[1].sum().item())}
+    return res
 
 
 def batched_config(seed, torch, api, inputs):
@@ -322,6 +345,8 @@ def profile_leg(leg, seed, torch, api, inputs):
     d_np, z_np, _ = inputs.config3_dz(n, seed)
     td, tz = torch.from_numpy(d_np).cuda(), torch.from_numpy(z_np).cuda()
     difl, difr, u, v = api.secular_dev(td, tz, 1.0, loewner=True)
+    if leg == "rrlu":
+        return {"leg": leg, "n": n, **rrlu_blocks(td, u, v, difl, difr, torch, api)}
     gen = torch.Generator(device="cuda")
     gen.manual_seed(seed)
     a = torch.randn((n, n), dtype=torch.float64, device="cuda", generator=gen).t()
@@ -345,10 +370,12 @@ def main():
                     help="skip the other benchmark configurations (reported beside the headline)")
     ap.add_argument("--copy-calib", action="store_true",
                     help="run one 512 MiB device copy first (calibrates HBM byte counters)")
-    ap.add_argument("--profile-leg", choices=["split", "batched", "blr"],
+    ap.add_argument("--profile-leg", choices=["split", "batched", "blr", "rrlu"],
                     help="run only one workload (for rocprofv3): the N=8192 Toeplitz split product "
-                         "(t=13, 3 launches), the 512-problem batch, or the N=16384 compressed "
-                         "merge product (3 launches); prints a short JSON line, not the bench line")
+                         "(t=13, 3 launches), the 512-problem batch, the N=16384 compressed "
+                         "merge product (3 launches), or the generator-only RRLU of the N=16384 "
+                         "merge's 4032 leaf blocks (two tolerances); prints a short JSON line, not "
+                         "the bench line")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:

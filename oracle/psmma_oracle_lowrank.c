@@ -328,9 +328,12 @@ int ora_rrlu_cauchy(double* D, double* F, double* A, double* V, double tol, doub
     for (int j = 0; j < k; ++j) W[j] = W[j] * ((F[k] - D[j]) / (D[k] - D[j]));
     W[k] = (D[k] - F[k]) / A[k];
     for (int j = 0; j < k; ++j) W[k] = W[k] * (F[j] - D[k]) / (D[j] - D[k]);
-    /* the swap pass runs every prd-th step (flgl = mod(k, prd) < 1) */
+    /* the swap pass runs every prd-th step (flgl = mod(k, prd) < 1) and repeats while
+     * it exchanges a row; the reference's loop has no bound, the restatement stops after
+     * 64 (M + N) exchanges (each genuine exchange grows the chosen block's determinant
+     * by more than rho; a longer run is rounding cycling between rows) */
     int again = ((k + 1) % prd) < 1;
-    while (again) {
+    for (long pass = 0; again && pass < 64L * (M + N); ++pass) {
       again = 0;
       /* searchMax2 (SRC/Cauchylowrank.f90:137-163) over the k+1 chosen rows */
       int ii = 0, jjl;
@@ -604,9 +607,12 @@ int ora_rrlu_cauchyeig_row(const double* ff, const double* difl, const double* d
       temp = -FD(k, j);
       W[k] = W[k] * (temp / (F[j] - F[k]));
     }
-    /* the swap pass runs every prd-th step (flgl = mod(k, prd) < 1) */
+    /* the swap pass runs every prd-th step (flgl = mod(k, prd) < 1) and repeats while
+     * it exchanges a row; the reference's loop has no bound, the restatement stops after
+     * 64 (M + N) exchanges (each genuine exchange grows the chosen block's determinant
+     * by more than rho; a longer run is rounding cycling between rows) */
     int again = ((k + 1) % prd) < 1;
-    while (again) {
+    for (long pass = 0; again && pass < 64L * (M + N); ++pass) {
       again = 0;
       /* searchMax2Eig (SRC/CauchyHssEig.f90:1137-1203) over the k+1 chosen rows */
       int ii = 0, jjl;

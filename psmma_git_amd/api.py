@@ -341,23 +341,28 @@ RRLU_TOL = float(np.float32(1.0e-17))  # the reference's default-real TOL
 
 
 def rrlu_cauchyeig_batched_dev(pl, pu, fpole, u, v, difl, difr, tol=RRLU_TOL, kmax=32,
-                               stream=None):
+                               stream=None, out=None):
     """Batched generator-only RRLU of eigen-Cauchy blocks (Compress_cauchyEig2).  pl
     (nblk, m) / pu (nblk, n): int32 device tensors of 1-based global rows / columns;
     generators: float64 device vectors of the matrix order.  Returns (rank, nswap, q,
     y): rank, nswap (nblk,) int32; q (nblk, kq, m) and y (nblk, n, kq) hold, as
     column-major views q[b].T and y[b].T, Q (m x rank) and Y (rank x n) in their
     leading columns / rows when 0 < rank < min(m, n) and rank <= kmax,
-    kq = min(kmax, m, n)."""
+    kq = min(kmax, m, n).  out: a previous call's (rank, nswap, q, y) to reuse."""
     import torch
     nblk, m = pl.shape
     n = pu.shape[1]
     kq = max(1, min(kmax, m, n))
     dev = pl.device
-    rank = torch.empty(nblk, dtype=torch.int32, device=dev)
-    nswap = torch.empty(nblk, dtype=torch.int32, device=dev)
-    q = torch.zeros((nblk, kq, m), dtype=torch.float64, device=dev)
-    y = torch.zeros((nblk, n, kq), dtype=torch.float64, device=dev)
+    if out is not None:
+        rank, nswap, q, y = out
+        if q.shape != (nblk, kq, m) or y.shape != (nblk, n, kq):
+            raise ValueError("out buffers do not match the blocks")
+    else:
+        rank = torch.empty(nblk, dtype=torch.int32, device=dev)
+        nswap = torch.empty(nblk, dtype=torch.int32, device=dev)
+        q = torch.zeros((nblk, kq, m), dtype=torch.float64, device=dev)
+        y = torch.zeros((nblk, n, kq), dtype=torch.float64, device=dev)
     _check(lib().psmma_rrlu_cauchyeig_batched_dev(
         nblk, m, n, pl.data_ptr(), pu.data_ptr(), fpole.data_ptr(), u.data_ptr(), v.data_ptr(),
         difl.data_ptr(), difr.data_ptr(), fpole.numel(), float(tol), kmax, rank.data_ptr(),

@@ -162,6 +162,7 @@ __device__ void pivt(RrluEigState& s, int k, int M, int N) {
   junkL = junkL * fabs(s.V[k]);
   junkU = junkU * fabs(s.U[k]);
   const double piv = fabs(s.U[k] * s.V[k] / fd(s, k, k));
+  __syncthreads();  // every thread has read U(k), V(k) before thread 0 swaps them
   if (junkL <= piv && junkU <= piv) return;
   int flg = (junkL > junkU) ? 1 : 0;
   double junk;

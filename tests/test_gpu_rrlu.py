@@ -71,6 +71,29 @@ def test_rrlu_merge_leaf_blocks(gpu, ora, tol):
     assert (ranks > 0).all() and (ranks < leaf).all()
 
 
+@pytest.mark.parametrize("tol", [api.RRLU_TOL, 1e-12])
+def test_rrlu_blocks_from_gpu_secular_solver(gpu, ora, tol):
+    """The merge's 240 leaf blocks at N = 4096 with the generators of the GPU secular
+    solver (as the compressed product uses them), run twice: both runs bitwise equal
+    to each other and to the oracle."""
+    import torch
+
+    n, leaf = 4096, 256
+    d, z, _ = inputs.config3_dz(n, 2)
+    td, tz = torch.from_numpy(d).cuda(), torch.from_numpy(z).cuda()
+    difl, difr, u, v = api.secular_dev(td, tz, 1.0, loewner=True)
+    torch.cuda.synchronize()
+    gens = tuple(x.cpu().numpy() for x in (td, difl, difr, u, v))
+    nb = n // leaf
+    blocks = [(np.arange(i * leaf, (i + 1) * leaf), np.arange(j * leaf, (j + 1) * leaf))
+              for i in range(nb) for j in range(nb) if i != j]
+    first = _run(gens, blocks, tol, 32)
+    second = _run(gens, blocks, tol, 32)
+    for a, b in zip(first, second):
+        assert np.array_equal(a, b)
+    _check(ora, gens, blocks, tol, kmax=32)
+
+
 @pytest.mark.parametrize("tol", [1e-6, 1e-10, 1e-14, api.RRLU_TOL])
 def test_rrlu_separated_blocks(gpu, ora, tol):
     n = 200
