@@ -227,6 +227,20 @@ def eig_merge_config(seed, torch, api, inputs, n=16384):
         "ms": round(msb, 3), "eff_tflops_dense_equiv": round(flops / msb / 1e9, 2),
         "speedup_vs_fp64": round(ms64 / msb, 3), "rel_err": eb, "orth_max": bm, "orth_fro": bf,
         "stats": stats}
+    # the same product with the generator-only RRLU (Compress_cauchyEig2) as the compressor
+    stats_lu = {}
+
+    def blr_lu():
+        stats_lu.update(api.cauchyeig_blr_dev(1.0, tq, td, u, v, difl, difr, x, leaf=256,
+                                              tol=1e-12, compressor=api.BLR_RRLU))
+
+    msl = ev_time(blr_lu, 3, torch)
+    el = rel(x, ref)
+    lm, lf = ortho(x)
+    out["blr_rrlu_leaf256_tol1e-12"] = {
+        "ms": round(msl, 3), "eff_tflops_dense_equiv": round(flops / msl / 1e9, 2),
+        "speedup_vs_fp64": round(ms64 / msl, 3), "rel_err": el, "orth_max": lm, "orth_fro": lf,
+        "stats": stats_lu}
     del x, ref, tq
     torch.cuda.empty_cache()
     out["rrlu_leaf256_blocks"] = rrlu_blocks(td, u, v, difl, difr, torch, api)
@@ -352,8 +366,10 @@ def profile_leg(leg, seed, torch, api, inputs):
     a = torch.randn((n, n), dtype=torch.float64, device="cuda", generator=gen).t()
     x = api.colmajor_empty(n, n)
     stats = {}
+    comp = api.BLR_RRLU if leg == "blr_rrlu" else None
     ms = ev_time(lambda: stats.update(api.cauchyeig_blr_dev(1.0, a, td, u, v, difl, difr, x,
-                                                            leaf=256, tol=1e-12)), 2, torch)
+                                                            leaf=256, tol=1e-12,
+                                                            compressor=comp)), 2, torch)
     return {"leg": leg, "n": n, "ms": round(ms, 3), "stats": stats}
 
 
@@ -370,10 +386,10 @@ def main():
                     help="skip the other benchmark configurations (reported beside the headline)")
     ap.add_argument("--copy-calib", action="store_true",
                     help="run one 512 MiB device copy first (calibrates HBM byte counters)")
-    ap.add_argument("--profile-leg", choices=["split", "batched", "blr", "rrlu"],
+    ap.add_argument("--profile-leg", choices=["split", "batched", "blr", "blr_rrlu", "rrlu"],
                     help="run only one workload (for rocprofv3): the N=8192 Toeplitz split product "
                          "(t=13, 3 launches), the 512-problem batch, the N=16384 compressed "
-                         "merge product (3 launches), or the generator-only RRLU of the N=16384 "
+                         "merge product with the RRQR or the RRLU compressor (3 launches), or the generator-only RRLU of the N=16384 "
                          "merge's 4032 leaf blocks (two tolerances); prints a short JSON line, not "
                          "the bench line")
     args = ap.parse_args()

@@ -299,6 +299,23 @@ int psmma_cauchyeig_blr_dev(int m, int n, double alpha, const double* a, int lda
                             const double* difl, const double* difr, int leaf, double tol,
                             double beta, double* c, int ldc, long long* stats, void* stream);
 
+/* The same product with a choice of compressor for the off-diagonal blocks:
+ *   PSMMA_BLR_RRQR: the batched truncated RRQR above (Compress_RRQR);
+ *   PSMMA_BLR_RRLU: the generator-only RRLU of psmma_rrlu_cauchyeig_batched_dev
+ *     (Compress_cauchyEig2, the reference's compressor of off-diagonal eigen-Cauchy
+ *     ring tiles, SRC/Auxil_2p5D.f90:642-748), block ~= Q Y with Q and Y the row
+ *     ID; tol is relative to the largest pivot; ranks above 32 keep the block in
+ *     the structured FP64 kernel.
+ * No counterpart in the reference (its compressed merge fixes the compressor).
+ * Returns 0 or -(argument number). */
+#define PSMMA_BLR_RRQR 0
+#define PSMMA_BLR_RRLU 1
+int psmma_cauchyeig_blr_ex_dev(int m, int n, double alpha, const double* a, int lda,
+                               const double* fpole, const double* u, const double* v,
+                               const double* difl, const double* difr, int leaf, double tol,
+                               int compressor, double beta, double* c, int ldc,
+                               long long* stats, void* stream);
+
 /* Library version string. */
 const char* psmma_version(void);
 

@@ -71,6 +71,8 @@ _SIGS = {
     "psmma_rrlu_cauchyeig_batched_dev": ([_I, _I, _I, _P, _P, _P, _P, _P, _P, _P, _I, _D, _I, _P,
                                           _P, _P, _I, ctypes.c_longlong, _P, _I,
                                           ctypes.c_longlong, _P], _I),
+    "psmma_cauchyeig_blr_ex_dev": ([_I, _I, _D, _P, _I, _P, _P, _P, _P, _P, _I, _D, _I, _D, _P,
+                                    _I, _P, _P], _I),
     "psmma_cauchyeig_blr_dev": ([_I, _I, _D, _P, _I, _P, _P, _P, _P, _P, _I, _D, _D, _P, _I, _P,
                                  _P], _I),
     "psmma_batched_dev": ([_I, _P, _I, _I, _P], _I),
@@ -322,18 +324,32 @@ def secular_dev(d, z, rho=1.0, loewner=True, stream=None):
 RRQR_KMAX = 32
 
 
+BLR_RRQR, BLR_RRLU = 0, 1
+
+
 def cauchyeig_blr_dev(alpha, a, fpole, u, v, difl, difr, c, leaf=256, tol=1e-12, beta=0.0,
-                      stream=None):
+                      stream=None, compressor=None):
     """X = alpha * A * C + beta * X with the off-diagonal leaf blocks of the eigen-Cauchy
-    matrix C compressed by the batched RRQR.  a, c: column-major (m x n) device views.
-    Returns the stats dict (blocks, lowrank, dense, zero, rank_sum)."""
+    matrix C compressed by the batched RRQR (default: psmma_cauchyeig_blr_dev) or by the
+    compressor given (BLR_RRQR / BLR_RRLU: psmma_cauchyeig_blr_ex_dev).  a, c:
+    column-major (m x n) device views.  Returns the stats dict (blocks, lowrank, dense,
+    zero, rank_sum)."""
     pa, m, n, lda = _colmajor(a)
     pc, mc, nc, ldc = _colmajor(c)
     st = (ctypes.c_longlong * 5)()
-    _check(lib().psmma_cauchyeig_blr_dev(m, n, alpha, pa, lda, fpole.data_ptr(), u.data_ptr(),
-                                         v.data_ptr(), difl.data_ptr(), difr.data_ptr(), leaf,
-                                         float(tol), beta, pc, ldc, st, _stream_ptr(stream)),
-           "psmma_cauchyeig_blr_dev")
+    if compressor is None:
+        _check(lib().psmma_cauchyeig_blr_dev(m, n, alpha, pa, lda, fpole.data_ptr(),
+                                             u.data_ptr(), v.data_ptr(), difl.data_ptr(),
+                                             difr.data_ptr(), leaf, float(tol), beta, pc, ldc,
+                                             st, _stream_ptr(stream)),
+               "psmma_cauchyeig_blr_dev")
+    else:
+        _check(lib().psmma_cauchyeig_blr_ex_dev(m, n, alpha, pa, lda, fpole.data_ptr(),
+                                                u.data_ptr(), v.data_ptr(), difl.data_ptr(),
+                                                difr.data_ptr(), leaf, float(tol),
+                                                int(compressor), beta, pc, ldc, st,
+                                                _stream_ptr(stream)),
+               "psmma_cauchyeig_blr_ex_dev")
     return dict(zip(("blocks", "lowrank", "dense", "zero", "rank_sum"), list(st)))
 
 

@@ -13,6 +13,10 @@
 // off-diagonal tile with Compress_RRQR and multiplies A by [Q | R P^T] with two
 // DGEMMs) applied on one GPU to a grid of leaf-sized tiles, with the plain
 // GEMMs grouped per block row and column.
+// The compressor is the batched RRQR (Compress_RRQR) or, as an alternative,
+// the generator-only RRLU (Compress_cauchyEig2, SRC/CauchyHssEig.f90:2640-2829,
+// the reference's compressor of off-diagonal eigen-Cauchy ring tiles), whose
+// row ID C(I,J) ~ Q Y fills the same Q and R slots.
 #include <hip/hip_runtime.h>
 #include <rocblas/rocblas.h>
 
@@ -81,6 +85,17 @@ __global__ __launch_bounds__(256) void blr_regroup_kernel(const PackEntry* __res
   }
 }
 
+// Global 1-based rows / columns of block b (leaf consecutive indices from r0[b] / c0[b]).
+__global__ __launch_bounds__(256) void blr_index_kernel(const int* __restrict__ r0,
+                                                        const int* __restrict__ c0, int leaf,
+                                                        int* __restrict__ pl, int* __restrict__ pu) {
+  const int b = blockIdx.x, t = threadIdx.x;
+  if (t < leaf) {
+    pl[(size_t)b * leaf + t] = r0[b] + t + 1;
+    pu[(size_t)b * leaf + t] = c0[b] + t + 1;
+  }
+}
+
 struct DevWs {
   void* p = nullptr;
   size_t bytes = 0;
@@ -125,17 +140,19 @@ size_t align_up(size_t x) { return (x + 255) & ~size_t(255); }
 
 }  // namespace
 
-extern "C" int psmma_cauchyeig_blr_dev(int m, int nglob, double alpha, const double* a, int lda,
-                                       const double* fpole, const double* u, const double* v,
-                                       const double* difl, const double* difr, int leaf,
-                                       double tol, double beta, double* c, int ldc,
-                                       long long* stats, void* stream) {
+extern "C" int psmma_cauchyeig_blr_ex_dev(int m, int nglob, double alpha, const double* a,
+                                          int lda, const double* fpole, const double* u,
+                                          const double* v, const double* difl,
+                                          const double* difr, int leaf, double tol,
+                                          int compressor, double beta, double* c, int ldc,
+                                          long long* stats, void* stream) {
   if (m < 0) return -1;
   if (nglob < 0) return -2;
   if (lda < std::max(1, m)) return -5;
   if (leaf < 1 || leaf > 256 || (nglob % leaf) != 0) return -11;
   if (!(tol >= 0.0)) return -12;
-  if (ldc < std::max(1, m)) return -15;
+  if (compressor != PSMMA_BLR_RRQR && compressor != PSMMA_BLR_RRLU) return -13;
+  if (ldc < std::max(1, m)) return -16;
   if (m == 0 || nglob == 0) return 0;
   if (!a || !c || !fpole || !u || !v || !difl || !difr) return -4;
   hipStream_t s = (hipStream_t)stream;
@@ -143,14 +160,16 @@ extern "C" int psmma_cauchyeig_blr_dev(int m, int nglob, double alpha, const dou
   const int nblk = nb * (nb - 1);
   GenPtrs g{fpole, u, v, difl, difr, nglob, nglob, nglob};
 
-  // workspace: block lists, RRQR outputs, the evaluated blocks (each generated once,
-  // then read by every pass of its RRQR)
+  const bool rrlu = compressor == PSMMA_BLR_RRLU;
+  // workspace: block lists, compressor outputs (Q and R P^T, or Q and Y), and for the
+  // RRQR the evaluated blocks (each generated once, then read by every pass)
   const size_t qbytes = (size_t)nblk * leaf * kKmax * sizeof(double);
-  const size_t cbytes = (size_t)nblk * leaf * leaf * sizeof(double);
+  const size_t cbytes = rrlu ? 0 : (size_t)nblk * leaf * leaf * sizeof(double);
   size_t off_r0 = 0, off_c0 = align_up(off_r0 + sizeof(int) * nblk);
   size_t off_rank = align_up(off_c0 + sizeof(int) * nblk);
-  size_t off_jp = align_up(off_rank + sizeof(int) * nblk);
-  size_t off_q = align_up(off_jp + sizeof(int) * (size_t)nblk * leaf);
+  size_t off_jp = align_up(off_rank + sizeof(int) * nblk);  // RRQR pivots or RRLU rows
+  size_t off_pu = align_up(off_jp + sizeof(int) * (size_t)nblk * leaf);  // RRLU columns
+  size_t off_q = align_up(off_pu + (rrlu ? sizeof(int) * (size_t)nblk * leaf : 0));
   size_t off_r = align_up(off_q + qbytes);
   size_t off_cache = align_up(off_r + qbytes);
   size_t fixed = align_up(off_cache + cbytes);
@@ -174,10 +193,22 @@ extern "C" int psmma_cauchyeig_blr_dev(int m, int nglob, double alpha, const dou
     e = hipMemcpyAsync(dr0, hr0.data(), sizeof(int) * nblk, hipMemcpyHostToDevice, s);
     if (e == hipSuccess)
       e = hipMemcpyAsync(dc0, hc0.data(), sizeof(int) * nblk, hipMemcpyHostToDevice, s);
-    if (e == hipSuccess)
+    if (e == hipSuccess && !rrlu)
       e = launch_rrqr_structured(KIND_CAUCHYEIG, g, dr0, dc0, nblk, leaf, leaf, tol, kKmax, drank,
                                  (int*)(ws + off_jp), leaf, dq, leaf, (long long)leaf * kKmax, drp,
                                  kKmax, (long long)kKmax * leaf, (double*)(ws + off_cache), s);
+    if (e == hipSuccess && rrlu) {
+      int* dpl = (int*)(ws + off_jp);
+      int* dpu = (int*)(ws + off_pu);
+      blr_index_kernel<<<nblk, 256, 0, s>>>(dr0, dc0, leaf, dpl, dpu);
+      e = hipGetLastError();
+      if (e == hipSuccess) {
+        const int st = psmma_rrlu_cauchyeig_batched_dev(
+            nblk, leaf, leaf, dpl, dpu, fpole, u, v, difl, difr, nglob, tol, kKmax, drank,
+            nullptr, dq, leaf, (long long)leaf * kKmax, drp, kKmax, (long long)kKmax * leaf, s);
+        if (st) return st > 0 ? st : 1006;
+      }
+    }
   }
   std::vector<int> hrank(nblk);
   if (e == hipSuccess && nblk > 0) {
@@ -197,7 +228,7 @@ extern "C" int psmma_cauchyeig_blr_dev(int m, int nglob, double alpha, const dou
     const int r = hrank[b];
     if (r == 0) {
       ++nzero;
-    } else if (r < 0 || r >= leaf) {
+    } else if (r < 0 || r >= leaf || r > kKmax) {  // RRLU ranks above kKmax: no outputs
       dense.push_back(b);
     } else {
       rowlist[hr0[b] / leaf].push_back(b);
@@ -308,4 +339,14 @@ extern "C" int psmma_cauchyeig_blr_dev(int m, int nglob, double alpha, const dou
     }
   }
   return 0;
+}
+
+extern "C" int psmma_cauchyeig_blr_dev(int m, int nglob, double alpha, const double* a, int lda,
+                                       const double* fpole, const double* u, const double* v,
+                                       const double* difl, const double* difr, int leaf,
+                                       double tol, double beta, double* c, int ldc,
+                                       long long* stats, void* stream) {
+  const int st = psmma_cauchyeig_blr_ex_dev(m, nglob, alpha, a, lda, fpole, u, v, difl, difr, leaf,
+                                            tol, PSMMA_BLR_RRQR, beta, c, ldc, stats, stream);
+  return st == -16 ? -15 : st;  // argument numbers of this entry point
 }

@@ -69,6 +69,49 @@ def test_blr_product_matches_oracle(gpu, ora, n, leaf, m):
     assert abs(st["rank_sum"] - ranks) <= 0.02 * ranks
 
 
+@pytest.mark.parametrize("n,leaf,m", [(2048, 256, 2048), (2048, 128, 700)])
+def test_blr_rrlu_compressor_matches_oracle(gpu, ora, n, leaf, m):
+    """The same product with the generator-only RRLU (Compress_cauchyEig2) as the
+    compressor: error within the same bar, and, the GPU RRLU being bitwise the
+    oracle's, the same low-rank blocks and rank sum as the oracle's RRLU."""
+    import torch
+
+    print(f"blr rrlu n={n} leaf={leaf} m={m}", flush=True)
+    tol = 1e-12
+    gens, gh, rng = _setup(n, 31)
+    a = rng.standard_normal((m, n))
+    x0 = rng.standard_normal((m, n))
+    da, dx = _dev(a), _dev(x0)
+    st = api.cauchyeig_blr_dev(1.3, da, *gens, dx, leaf=leaf, tol=tol, beta=0.5,
+                               compressor=api.BLR_RRLU)
+    torch.cuda.synchronize()
+    c = ora.cauchyeig_matrix(*gh, 0, n, 0, n)
+    err = ora.rel_fro(dx.cpu().numpy(), 1.3 * (a @ c) + 0.5 * x0)
+    print("rel err", err, st, flush=True)
+    assert err <= 20 * tol
+    f, u, v, dl, dr = gh
+    nb = n // leaf
+    low, ranks = 0, 0
+    for i in range(nb):
+        for j in range(nb):
+            if i != j:
+                r, _, _, _ = ora.compress_cauchyeig2(f, dl, dr, u, v,
+                                                     np.arange(i * leaf, (i + 1) * leaf),
+                                                     np.arange(j * leaf, (j + 1) * leaf), tol)
+                if 0 < r <= 32 and r < leaf:
+                    low += 1
+                    ranks += r
+    assert st["lowrank"] == low and st["rank_sum"] == ranks
+
+
+def test_blr_rejects_unknown_compressor(gpu):
+    n = 256
+    gens, _, _ = _setup(n, 35)
+    a = api.colmajor_empty(n, n)
+    with pytest.raises(api.PsmmaError):
+        api.cauchyeig_blr_dev(1.0, a, *gens, a, leaf=128, compressor=2)
+
+
 def test_blr_merge_is_orthogonal(gpu, ora):
     import torch
 
