@@ -262,7 +262,8 @@ int psmma_rrqr_batched_dev(int nblk, int m, int n, const double* a, int lda, lon
  * u_i v_j / (fpole_i - lambda_j) of psmma_cauchyeig_dev (order korder, every
  * difference rebuilt from fpole, difl, difr through the global indices); the
  * block itself is never formed.  Rook pivoting, complete pivoting at the first
- * convergence test, a swap pass every 5th step (rho = 1.2); rank[b] = the first
+ * convergence test, a swap pass every 5th step (rho = the default-real 1.2E+0);
+ * rank[b] = the first
  * step whose pivot falls below tol times the largest pivot so far (the
  * reference's TOL is the default-real 1.0E-17), min(m,n) at full rank, -2 when
  * an index lies outside [1, korder].  For 0 < rank < min(m,n) and rank <= kmax:
@@ -278,6 +279,25 @@ int psmma_rrlu_cauchyeig_batched_dev(int nblk, int m, int n, const int* pl, cons
                                      double tol, int kmax, int* rank, int* nswap, double* q,
                                      int ldq, long long stride_q, double* y, int ldy,
                                      long long stride_y, void* stream);
+
+/* Plain Cauchy blocks: compress_cauchy('R') over rrluCauchy (SRC/Cauchylowrank.f90:
+ * 387-491, :7-134), the compression ConstCauchylowrank applies to off-diagonal Cauchy
+ * tiles (SRC/Auxil_2p5D.f90:876-963).  Block b has the 1-based global rows pl[b*m + i]
+ * and columns pu[b*n + j] (m, n <= 256) of u_i v_j / (d_i - f_j) with the global
+ * generator vectors d, u (length nrow) and f, v (length ncol), gathered as
+ * ConstCauchylowrank gathers them.  Rook pivoting, complete pivoting at the first
+ * convergence test, a swap pass every 10th step (rho = 1.12); rank[b] as above (-2
+ * for an index out of range).  For 0 < rank < m and rank <= kmax (the reference
+ * writes H exactly when rank < m):
+ *   x + b*stride_x (ldx):  X (m x rank) = P^-1 [I; Z_i W_j / (D_i - D_j)];
+ *   y + b*stride_y (ldy):  Y (rank x n), the chosen rows of the block;
+ * otherwise x and y are left unwritten.  Bitwise the CPU restatement.  Returns 0
+ * or -(argument number). */
+int psmma_rrlu_cauchy_batched_dev(int nblk, int m, int n, const int* pl, const int* pu,
+                                  const double* d, const double* f, const double* u,
+                                  const double* v, int nrow, int ncol, double tol, int kmax,
+                                  int* rank, double* x, int ldx, long long stride_x, double* y,
+                                  int ldy, long long stride_y, void* stream);
 
 /* ===========================================================================
  * 6. Compressed eigenvector-merge product (device pointers).

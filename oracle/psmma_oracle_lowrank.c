@@ -569,7 +569,7 @@ int ora_rrlu_cauchyeig_row(const double* ff, const double* difl, const double* d
   const int mn = rk;
   int swaps = 0;
   double amax = 0.0;
-  const double rho = 1.2;
+  const double rho = (double)1.2f; /* rho = 1.2E+0, a default-real literal (SRC/CauchyHssEig.f90:98) */
   const int prd = 5;
   for (int k = 0; k < mn; ++k) { /* the reference's k = this k + 1 */
     eig_pivt(&e, k);

@@ -71,6 +71,9 @@ _SIGS = {
     "psmma_rrlu_cauchyeig_batched_dev": ([_I, _I, _I, _P, _P, _P, _P, _P, _P, _P, _I, _D, _I, _P,
                                           _P, _P, _I, ctypes.c_longlong, _P, _I,
                                           ctypes.c_longlong, _P], _I),
+    "psmma_rrlu_cauchy_batched_dev": ([_I, _I, _I, _P, _P, _P, _P, _P, _P, _I, _I, _D, _I, _P, _P,
+                                       _I, ctypes.c_longlong, _P, _I, ctypes.c_longlong, _P],
+                                      _I),
     "psmma_cauchyeig_blr_ex_dev": ([_I, _I, _D, _P, _I, _P, _P, _P, _P, _P, _I, _D, _I, _D, _P,
                                     _I, _P, _P], _I),
     "psmma_cauchyeig_blr_dev": ([_I, _I, _D, _P, _I, _P, _P, _P, _P, _P, _I, _D, _D, _P, _I, _P,
@@ -385,6 +388,27 @@ def rrlu_cauchyeig_batched_dev(pl, pu, fpole, u, v, difl, difr, tol=RRLU_TOL, km
         nswap.data_ptr(), q.data_ptr(), m, kq * m, y.data_ptr(), kq, n * kq,
         _stream_ptr(stream)), "psmma_rrlu_cauchyeig_batched_dev")
     return rank, nswap, q, y
+
+
+def rrlu_cauchy_batched_dev(pl, pu, d, f, u, v, tol=RRLU_TOL, kmax=32, stream=None):
+    """Batched generator-only RRLU of plain Cauchy blocks (compress_cauchy('R')).  pl (nblk,
+    m) / pu (nblk, n): int32 device tensors of 1-based global rows / columns; d, u (rows)
+    and f, v (columns): float64 device vectors.  Returns (rank, x, y) with x (nblk, kq, m)
+    and y (nblk, n, kq) holding, as column-major views x[b].T and y[b].T, X (m x rank)
+    and Y (rank x n) when 0 < rank < m and rank <= kmax, kq = min(kmax, m, n)."""
+    import torch
+    nblk, m = pl.shape
+    n = pu.shape[1]
+    kq = max(1, min(kmax, m, n))
+    dev = pl.device
+    rank = torch.empty(nblk, dtype=torch.int32, device=dev)
+    x = torch.zeros((nblk, kq, m), dtype=torch.float64, device=dev)
+    y = torch.zeros((nblk, n, kq), dtype=torch.float64, device=dev)
+    _check(lib().psmma_rrlu_cauchy_batched_dev(
+        nblk, m, n, pl.data_ptr(), pu.data_ptr(), d.data_ptr(), f.data_ptr(), u.data_ptr(),
+        v.data_ptr(), d.numel(), f.numel(), float(tol), kmax, rank.data_ptr(), x.data_ptr(), m,
+        kq * m, y.data_ptr(), kq, n * kq, _stream_ptr(stream)), "psmma_rrlu_cauchy_batched_dev")
+    return rank, x, y
 
 
 def rrqr_batched_dev(blocks, tol, kmax=RRQR_KMAX, stream=None):
