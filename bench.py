@@ -156,6 +156,7 @@ def other_configs(seed, torch, api, inputs):
     out["batched_512"] = batched_config(seed, torch, api, inputs)
     torch.cuda.empty_cache()
     out["eig_merge_n16384"] = eig_merge_config(seed, torch, api, inputs)
+    out["rrlu_cauchy_leaf256_n16384"] = rrlu_cauchy_blocks(seed, torch, api, inputs)
     return out
 
 
@@ -245,6 +246,28 @@ def eig_merge_config(seed, torch, api, inputs, n=16384):
     torch.cuda.empty_cache()
     out["rrlu_leaf256_blocks"] = rrlu_blocks(td, u, v, difl, difr, torch, api)
     return out
+
+
+def rrlu_cauchy_blocks(seed, torch, api, inputs, n=16384, leaf=256):
+    """Generator-only RRLU (compress_cauchy) of every off-diagonal leaf block of an
+    n x n Cauchy-like matrix with the Cauchy benchmark's interlaced generators."""
+    b = inputs.cauchy_generators(n, np.random.default_rng(seed))
+    d, f, u, v = (torch.from_numpy(np.ascontiguousarray(b[:, i])).cuda() for i in range(4))
+    nb = n // leaf
+    pairs = torch.tensor([(i, j) for i in range(nb) for j in range(nb) if i != j],
+                         dtype=torch.int32, device="cuda")
+    ar = torch.arange(1, leaf + 1, dtype=torch.int32, device="cuda")
+    pl = (pairs[:, :1] * leaf + ar).contiguous()
+    pu = (pairs[:, 1:] * leaf + ar).contiguous()
+    res = {"n": n, "blocks": int(pairs.shape[0]), "leaf": leaf}
+    buf = api.rrlu_cauchy_batched_dev(pl, pu, d, f, u, v, api.RRLU_TOL, 32)
+    for name, tol in (("tol_ref", api.RRLU_TOL), ("tol_1e-12", 1e-12)):
+        ms = ev_time(lambda: api.rrlu_cauchy_batched_dev(pl, pu, d, f, u, v, tol, 32, out=buf),
+                     3, torch)
+        rank = buf[0].cpu().numpy()
+        res[name] = {"tol": tol, "ms": round(ms, 3), "rank_sum": int(rank.sum()),
+                     "rank_max": int(rank.max())}
+    return res
 
 
 def rrlu_blocks(td, u, v, difl, difr, torch, api, leaf=256):

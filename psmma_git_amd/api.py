@@ -390,20 +390,26 @@ def rrlu_cauchyeig_batched_dev(pl, pu, fpole, u, v, difl, difr, tol=RRLU_TOL, km
     return rank, nswap, q, y
 
 
-def rrlu_cauchy_batched_dev(pl, pu, d, f, u, v, tol=RRLU_TOL, kmax=32, stream=None):
+def rrlu_cauchy_batched_dev(pl, pu, d, f, u, v, tol=RRLU_TOL, kmax=32, stream=None, out=None):
     """Batched generator-only RRLU of plain Cauchy blocks (compress_cauchy('R')).  pl (nblk,
     m) / pu (nblk, n): int32 device tensors of 1-based global rows / columns; d, u (rows)
     and f, v (columns): float64 device vectors.  Returns (rank, x, y) with x (nblk, kq, m)
     and y (nblk, n, kq) holding, as column-major views x[b].T and y[b].T, X (m x rank)
-    and Y (rank x n) when 0 < rank < m and rank <= kmax, kq = min(kmax, m, n)."""
+    and Y (rank x n) when 0 < rank < m and rank <= kmax, kq = min(kmax, m, n).  out: a
+    previous call's (rank, x, y) to reuse."""
     import torch
     nblk, m = pl.shape
     n = pu.shape[1]
     kq = max(1, min(kmax, m, n))
     dev = pl.device
-    rank = torch.empty(nblk, dtype=torch.int32, device=dev)
-    x = torch.zeros((nblk, kq, m), dtype=torch.float64, device=dev)
-    y = torch.zeros((nblk, n, kq), dtype=torch.float64, device=dev)
+    if out is not None:
+        rank, x, y = out
+        if x.shape != (nblk, kq, m) or y.shape != (nblk, n, kq):
+            raise ValueError("out buffers do not match the blocks")
+    else:
+        rank = torch.empty(nblk, dtype=torch.int32, device=dev)
+        x = torch.zeros((nblk, kq, m), dtype=torch.float64, device=dev)
+        y = torch.zeros((nblk, n, kq), dtype=torch.float64, device=dev)
     _check(lib().psmma_rrlu_cauchy_batched_dev(
         nblk, m, n, pl.data_ptr(), pu.data_ptr(), d.data_ptr(), f.data_ptr(), u.data_ptr(),
         v.data_ptr(), d.numel(), f.numel(), float(tol), kmax, rank.data_ptr(), x.data_ptr(), m,
