@@ -307,8 +307,9 @@ int psmma_rrlu_cauchy_batched_dev(int nblk, int m, int n, const int* pl, const i
  * C is split into leaf x leaf blocks (leaf <= 256, n % leaf == 0); every
  * off-diagonal block is compressed by the batched truncated RRQR at relative
  * tolerance tol (psmma_rrqr_batched_dev semantics, rank cap 32) from the
- * generators, and multiplied in compressed form; diagonal and incompressible
- * blocks use the structured FP64 kernel.  The reference's off-diagonal ring
+ * generators, and multiplied in compressed form; the diagonal blocks are
+ * evaluated and multiplied by one strided-batched DGEMM, incompressible blocks
+ * use the structured FP64 kernel.  The reference's off-diagonal ring
  * step (SRC/pstoeplitz_compute.f90:263-288: Compress_RRQR per tile, then two
  * DGEMMs) on a grid of leaf tiles.  Error: each block within ~tol of its largest
  * R entry.  stats (host, may be NULL, 5 entries): blocks, low-rank blocks,
