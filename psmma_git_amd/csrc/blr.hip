@@ -6,8 +6,10 @@
 //   diagonal blocks:       X(:,J) = alpha A(:,J) C(J,J) + beta X(:,J)  (explicit C(J,J), one
 //                          strided-batched DGEMM)
 //   incompressible blocks: X(:,J) += A(:,I) C(I,J)   (structured FP64 MFMA)
-//   low-rank blocks C(I,J) ~ Q_IJ R_IJ:  T_I = A(:,I) [Q_I1 .. Q_In]       (one GEMM per I)
+//   low-rank blocks C(I,J) ~ Q_IJ R_IJ:  T_I = A(:,I) [Q_I1 .. Q_In]
 //                                        X(:,J) += [T_1J .. T_nJ] [R_1J; ..; R_nJ]
+//   (each stage one strided-batched DGEMM over the block rows / columns, their widths
+//   padded to the widest with zero columns)
 // The T columns are regrouped from (I, J) to (J, I) order between the two
 // GEMM stages, so each stage is one plain GEMM per block row / block column.
 // This is the reference's off-diagonal ring step (PSTOEPLITZ_COMPUTE's
@@ -57,8 +59,8 @@ struct PackEntry {
   int offb;   // column offset in T' and row offset in R (block-column order (J, I))
 };
 
-// U(:, offa + c) = Q_slot(:, c) and R(offb + c, :) = (R P^T)_slot(c, :), one
-// workgroup per block.
+// U(:, offa + c) = Q_slot(:, c) and R(offb + c, :) = (R P^T)_slot(c, :) (R has S
+// rows: its leading dimension), one workgroup per block.
 __global__ __launch_bounds__(256) void blr_pack_kernel(const PackEntry* __restrict__ tab, int leaf,
                                                        const double* __restrict__ qs,
                                                        const double* __restrict__ rs,
@@ -246,28 +248,36 @@ extern "C" int psmma_cauchyeig_blr_ex_dev(int m, int nglob, double alpha, const 
       ranksum += r;
     }
   }
+  // Column slots: block row I owns wA columns of U and T (its blocks' ranks, then
+  // zero padding), block column J owns wB columns of T' and rows of R, with wA / wB
+  // the widest block row / column, so that each GEMM stage is one strided-batched
+  // call of uniform shape (the padding adds zero columns, not error).
   std::vector<PackEntry> tab(nblk);
-  std::vector<int> offa_I(nb + 1, 0), offb_J(nb + 1, 0);
-  long long S = 0;
+  std::vector<int> wrow(nb, 0), wcol(nb, 0);
+  for (int I = 0; I < nb; ++I)
+    for (int b : rowlist[I]) wrow[I] += hrank[b];
+  for (int J = 0; J < nb; ++J)
+    for (int b : collist[J]) wcol[J] += hrank[b];
+  const int wA = *std::max_element(wrow.begin(), wrow.end());
+  const int wB = *std::max_element(wcol.begin(), wcol.end());
   for (int I = 0; I < nb; ++I) {
-    offa_I[I] = (int)S;
+    int o = I * wA;
     for (int b : rowlist[I]) {
       tab[b].slot = b;
       tab[b].r = hrank[b];
-      tab[b].offa = (int)S;
-      S += hrank[b];
+      tab[b].offa = o;
+      o += hrank[b];
     }
   }
-  offa_I[nb] = (int)S;
-  long long Sb = 0;
   for (int J = 0; J < nb; ++J) {
-    offb_J[J] = (int)Sb;
+    int o = J * wB;
     for (int b : collist[J]) {
-      tab[b].offb = (int)Sb;
-      Sb += hrank[b];
+      tab[b].offb = o;
+      o += hrank[b];
     }
   }
-  offb_J[nb] = (int)Sb;
+  const long long S = (long long)nb * wA;   // columns of U and T
+  const long long SR = (long long)nb * wB;  // columns of T', rows of R
   std::vector<PackEntry> packed;
   for (int I = 0; I < nb; ++I)
     for (int b : rowlist[I]) packed.push_back(tab[b]);
@@ -279,14 +289,14 @@ extern "C" int psmma_cauchyeig_blr_ex_dev(int m, int nglob, double alpha, const 
     stats[4] = ranksum;
   }
 
-  // second workspace part: pack table, U (leaf x S), R (S x leaf), T and T' (m x S), the
-  // explicit diagonal blocks (nb x leaf x leaf)
+  // second workspace part: pack table, U (leaf x S), R (SR x leaf), T (m x S), T' (m x SR),
+  // the explicit diagonal blocks (nb x leaf x leaf)
   const size_t off_tab = 0;
   const size_t off_u = align_up(off_tab + sizeof(PackEntry) * std::max<size_t>(1, packed.size()));
   const size_t off_rr = align_up(off_u + sizeof(double) * (size_t)leaf * S);
-  const size_t off_t = align_up(off_rr + sizeof(double) * (size_t)S * leaf);
+  const size_t off_t = align_up(off_rr + sizeof(double) * (size_t)SR * leaf);
   const size_t off_tp = align_up(off_t + sizeof(double) * (size_t)m * S);
-  const size_t off_h = align_up(off_tp + sizeof(double) * (size_t)m * S);
+  const size_t off_h = align_up(off_tp + sizeof(double) * (size_t)m * SR);
   const size_t total = align_up(off_h + sizeof(double) * (size_t)nb * leaf * leaf);
   char* ws2 = (char*)workspace(1, total);
   if (!ws2) return 1002;
@@ -324,8 +334,19 @@ extern "C" int psmma_cauchyeig_blr_ex_dev(int m, int nglob, double alpha, const 
                          lda, g, I * leaf, J * leaf, 1.0, c + (size_t)J * leaf * ldc, ldc, s);
   }
   if (e == hipSuccess && !packed.empty()) {
-    blr_pack_kernel<<<(unsigned)packed.size(), 256, 0, s>>>(dtab, leaf, dq, drp, U, R, S);
-    e = hipGetLastError();
+    // padding of U and R must be zero (the stages multiply through it)
+    e = hipMemsetAsync(U, 0, sizeof(double) * (size_t)leaf * S, s);
+    if (e == hipSuccess) e = hipMemsetAsync(R, 0, sizeof(double) * (size_t)SR * leaf, s);
+    if (e == hipSuccess) {
+      blr_pack_kernel<<<(unsigned)packed.size(), 256, 0, s>>>(dtab, leaf, dq, drp, U, R, SR);
+      e = hipGetLastError();
+    }
+    // T' columns past each block column's ranks: zero (R's matching rows are zero too, but
+    // stale memory could hold non-finite values)
+    for (int J = 0; J < nb && e == hipSuccess; ++J)
+      if (wcol[J] < wB)
+        e = hipMemsetAsync(Tp + ((size_t)J * wB + wcol[J]) * m, 0,
+                           sizeof(double) * (size_t)m * (wB - wcol[J]), s);
   }
   if (e != hipSuccess) {
     fprintf(stderr, "psmma_cauchyeig_blr_dev: %s\n", hipGetErrorString(e));
@@ -333,25 +354,21 @@ extern "C" int psmma_cauchyeig_blr_ex_dev(int m, int nglob, double alpha, const 
   }
   if (!packed.empty()) {
     const double one = 1.0, zero = 0.0;
-    for (int I = 0; I < nb; ++I) {  // T_I = A(:, I) U_I
-      const int w = offa_I[I + 1] - offa_I[I];
-      if (w == 0) continue;
-      if (rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, m, w, leaf, &one,
-                        a + (size_t)I * leaf * lda, lda, U + (size_t)offa_I[I] * leaf, leaf,
-                        &zero, T + (size_t)offa_I[I] * m, m) != rocblas_status_success)
-        return 1005;
-    }
+    // T_I = A(:, I) U_I for every block row I
+    if (rocblas_dgemm_strided_batched(h, rocblas_operation_none, rocblas_operation_none, m, wA,
+                                      leaf, &one, a, lda, (rocblas_stride)leaf * lda, U, leaf,
+                                      (rocblas_stride)leaf * wA, &zero, T, m,
+                                      (rocblas_stride)m * wA, nb) != rocblas_status_success)
+      return 1005;
     dim3 grid((unsigned)packed.size(), (unsigned)std::min(64, (m + 255) / 256));
     blr_regroup_kernel<<<grid, 256, 0, s>>>(dtab, m, T, Tp, m);
     if ((e = hipGetLastError()) != hipSuccess) return 1000 + (int)e;
-    for (int J = 0; J < nb; ++J) {  // X(:, J) += alpha T'_J R_J
-      const int w = offb_J[J + 1] - offb_J[J];
-      if (w == 0) continue;
-      if (rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, m, leaf, w, &alpha,
-                        Tp + (size_t)offb_J[J] * m, m, R + offb_J[J], (int)S, &one,
-                        c + (size_t)J * leaf * ldc, ldc) != rocblas_status_success)
-        return 1005;
-    }
+    // X(:, J) += alpha T'_J R_J for every block column J
+    if (rocblas_dgemm_strided_batched(h, rocblas_operation_none, rocblas_operation_none, m, leaf,
+                                      wB, &alpha, Tp, m, (rocblas_stride)m * wB, R, (int)SR,
+                                      (rocblas_stride)wB, &one, c, ldc, (rocblas_stride)leaf * ldc,
+                                      nb) != rocblas_status_success)
+      return 1005;
   }
   return 0;
 }
