@@ -77,15 +77,26 @@ __global__ __launch_bounds__(256) void blr_pack_kernel(const PackEntry* __restri
   }
 }
 
-// T'(:, offb + c) = T(:, offa + c) for every low-rank block: grid (blocks, column chunks).
+// T'(:, offb + c) = T(:, offa + c) for every low-rank block: grid (blocks, row chunks).
+// With an even leading dimension every column starts 16-byte aligned (the buffers are
+// 256-byte aligned) and moves as pairs of doubles; an odd m leaves one last element.
 __global__ __launch_bounds__(256) void blr_regroup_kernel(const PackEntry* __restrict__ tab, int m,
                                                           const double* __restrict__ T,
                                                           double* __restrict__ Tp, int ldt) {
   const PackEntry e = tab[blockIdx.x];
+  const int step = 256 * gridDim.y, i0 = blockIdx.y * 256 + threadIdx.x;
   for (int c = 0; c < e.r; ++c) {
     const double* src = T + (size_t)(e.offa + c) * ldt;
     double* dst = Tp + (size_t)(e.offb + c) * ldt;
-    for (int i = blockIdx.y * 256 + threadIdx.x; i < m; i += 256 * gridDim.y) dst[i] = src[i];
+    if ((ldt & 1) == 0) {
+      const int m2 = m >> 1;
+      const double2* s2 = (const double2*)src;
+      double2* d2 = (double2*)dst;
+      for (int i = i0; i < m2; i += step) d2[i] = s2[i];
+      if ((m & 1) && i0 == 0) dst[m - 1] = src[m - 1];
+    } else {
+      for (int i = i0; i < m; i += step) dst[i] = src[i];
+    }
   }
 }
 
@@ -360,7 +371,7 @@ extern "C" int psmma_cauchyeig_blr_ex_dev(int m, int nglob, double alpha, const 
                                       (rocblas_stride)leaf * wA, &zero, T, m,
                                       (rocblas_stride)m * wA, nb) != rocblas_status_success)
       return 1005;
-    dim3 grid((unsigned)packed.size(), (unsigned)std::min(64, (m + 255) / 256));
+    dim3 grid((unsigned)packed.size(), (unsigned)std::min(64, (m / 2 + 255) / 256 + 1));
     blr_regroup_kernel<<<grid, 256, 0, s>>>(dtab, m, T, Tp, m);
     if ((e = hipGetLastError()) != hipSuccess) return 1000 + (int)e;
     // X(:, J) += alpha T'_J R_J for every block column J

@@ -34,7 +34,8 @@ def _dev(x):
     return torch.from_numpy(np.asfortranarray(x)).cuda().t().contiguous().t()
 
 
-@pytest.mark.parametrize("n,leaf,m", [(2048, 256, 2048), (2048, 128, 700), (1024, 256, 1024)])
+@pytest.mark.parametrize("n,leaf,m", [(2048, 256, 2048), (2048, 128, 700), (1024, 256, 1024),
+                                      (1024, 256, 333)])
 def test_blr_product_matches_oracle(gpu, ora, n, leaf, m):
     import torch
 
