@@ -555,6 +555,12 @@ static hipError_t launch_right_kind(int cfg, int M, int N, int K, double alpha, 
     case 4:  // 256x128, 8 waves of 64x64: half the generated entries per flop (Cauchy kinds)
       return launch_right_t<KIND, 256, 128, 16, 4, 2, 1>(M, N, K, alpha, A, lda, g, r0, c0, beta,
                                                          C, ldc, s);
+    case 5:  // 128x128, 8 waves of 32x64: two waves per SIMD with one workgroup per CU
+      return launch_right_t<KIND, 128, 128, 16, 4, 2, 1>(M, N, K, alpha, A, lda, g, r0, c0, beta,
+                                                         C, ldc, s);
+    case 6:  // 128x128, 8 waves of 64x32
+      return launch_right_t<KIND, 128, 128, 16, 2, 4, 1>(M, N, K, alpha, A, lda, g, r0, c0, beta,
+                                                         C, ldc, s);
     default:  // small problems: 64x64 workgroup tile, 4 waves of 32x32
       return launch_right_t<KIND, 64, 64, 16, 2, 2>(M, N, K, alpha, A, lda, g, r0, c0, beta, C,
                                                     ldc, s);
@@ -562,7 +568,7 @@ static hipError_t launch_right_kind(int cfg, int M, int N, int K, double alpha, 
 }
 
 // Picks a tile shape: the big tile when it still yields >= 2 workgroups per CU.
-// PSMMA_F64_CFG (0..4) forces one, for tuning sweeps.
+// PSMMA_F64_CFG (0..6) forces one, for tuning sweeps.
 int choose_right_cfg(int kind, int M, int N) {
   static int forced = [] {
     const char* e = getenv("PSMMA_F64_CFG");
@@ -571,11 +577,12 @@ int choose_right_cfg(int kind, int M, int N) {
   if (forced >= 0) return forced;
   long long big = (long long)((M + 127) / 128) * ((N + 127) / 128);
   const bool win = kind == KIND_TOEPLITZ || kind == KIND_HANKEL;
-  // Cauchy kinds: one 8-wave 256x128 workgroup per CU (measured +5% at N = 16384
-  // over 128x128) once there are >= 4 of them per CU
-  if (!win && big >= 2048) return 4;
-  if (big >= 512) return win ? 3 : 0;
-  return 2;
+  // Cauchy kinds (measured on MI355X, N x N x N products): the 8-wave 256x128 tile
+  // from 1024 128x128 tiles up (N = 4096: 2.56 ms against 2.77 for 128x128; +5% at
+  // N = 16384), the 8-wave 128x128 tile from one 128x128 tile per CU (N = 2048:
+  // 0.41 ms against 0.45 for 4 waves and 0.49 for 64x64), 64x64 below
+  if (!win) return big >= 1024 ? 4 : big >= 256 ? 5 : 2;
+  return big >= 512 ? 3 : 2;
 }
 
 hipError_t launch_right_f64(int kind, int cfg, int M, int N, int K, double alpha, const double* A,
