@@ -582,7 +582,9 @@ int choose_right_cfg(int kind, int M, int N) {
   // N = 16384), the 8-wave 128x128 tile from one 128x128 tile per CU (N = 2048:
   // 0.41 ms against 0.45 for 4 waves and 0.49 for 64x64), 64x64 below
   if (!win) return big >= 1024 ? 4 : big >= 256 ? 5 : 2;
-  return big >= 512 ? 3 : 2;
+  // Toeplitz / Hankel: 64x256 from 256 128x128 tiles (Toeplitz N = 2048: 0.32 ms against
+  // 0.35 for 64x64; best of configs 0, 2, 3, 5, 6 at N = 2048, 4096, 8192)
+  return big >= 256 ? 3 : 2;
 }
 
 hipError_t launch_right_f64(int kind, int cfg, int M, int N, int K, double alpha, const double* A,
