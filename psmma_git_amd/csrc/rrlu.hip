@@ -61,8 +61,7 @@ struct RrluEigState {
   int IP[kRrluMaxDim];    // inverse row permutation (output)
   double red_v[kRrluWaves];
   int red_i[kRrluWaves];
-  double bc_v;
-  int bc_i, bc_j;
+  int bc_i;  // a row index published by one thread
 };
 
 // computfd: F(row i) - D(col j) for row / column positions.
