@@ -558,9 +558,6 @@ static hipError_t launch_right_kind(int cfg, int M, int N, int K, double alpha, 
     case 5:  // 128x128, 8 waves of 32x64: two waves per SIMD with one workgroup per CU
       return launch_right_t<KIND, 128, 128, 16, 4, 2, 1>(M, N, K, alpha, A, lda, g, r0, c0, beta,
                                                          C, ldc, s);
-    case 6:  // 128x128, 8 waves of 64x32
-      return launch_right_t<KIND, 128, 128, 16, 2, 4, 1>(M, N, K, alpha, A, lda, g, r0, c0, beta,
-                                                         C, ldc, s);
     default:  // small problems: 64x64 workgroup tile, 4 waves of 32x32
       return launch_right_t<KIND, 64, 64, 16, 2, 2>(M, N, K, alpha, A, lda, g, r0, c0, beta, C,
                                                     ldc, s);
@@ -568,7 +565,7 @@ static hipError_t launch_right_kind(int cfg, int M, int N, int K, double alpha, 
 }
 
 // Picks a tile shape: the big tile when it still yields >= 2 workgroups per CU.
-// PSMMA_F64_CFG (0..6) forces one, for tuning sweeps.
+// PSMMA_F64_CFG (0..5) forces one, for tuning sweeps.
 int choose_right_cfg(int kind, int M, int N) {
   static int forced = [] {
     const char* e = getenv("PSMMA_F64_CFG");
