@@ -580,7 +580,7 @@ int choose_right_cfg(int kind, int M, int N) {
   // 0.41 ms against 0.45 for 4 waves and 0.49 for 64x64), 64x64 below
   if (!win) return big >= 1024 ? 4 : big >= 256 ? 5 : 2;
   // Toeplitz / Hankel: 64x256 from 256 128x128 tiles (Toeplitz N = 2048: 0.32 ms against
-  // 0.35 for 64x64; best of configs 0, 2, 3, 5, 6 at N = 2048, 4096, 8192)
+  // 0.35 for 64x64; best of 0, 2, 3 and two 8-wave 128x128 layouts at N = 2048, 4096, 8192)
   return big >= 256 ? 3 : 2;
 }
 
