@@ -337,6 +337,18 @@ int psmma_cauchyeig_blr_ex_dev(int m, int n, double alpha, const double* a, int 
                                int compressor, double beta, double* c, int ldc,
                                long long* stats, void* stream);
 
+/* The compressed form of the plain Cauchy product X = alpha * A * C + beta * X,
+ * C(i,j) = u_i v_j / (d_i - f_j) (n x n, generators d, f, u, v of length n, the
+ * columns of psmma_cauchy's B): off-diagonal leaf blocks compressed by the batched
+ * RRQR or by the generator-only RRLU (compress_cauchy, the reference's compressor of
+ * off-diagonal Cauchy ring tiles, SRC/pscauchy_compute.f90:355-424), the rest as in
+ * psmma_cauchyeig_blr_ex_dev.  No counterpart in the reference as a single call.
+ * Returns 0 or -(argument number). */
+int psmma_cauchy_blr_dev(int m, int n, double alpha, const double* a, int lda, const double* d,
+                         const double* f, const double* u, const double* v, int leaf, double tol,
+                         int compressor, double beta, double* c, int ldc, long long* stats,
+                         void* stream);
+
 /* Library version string. */
 const char* psmma_version(void);
 

@@ -74,6 +74,8 @@ _SIGS = {
     "psmma_rrlu_cauchy_batched_dev": ([_I, _I, _I, _P, _P, _P, _P, _P, _P, _I, _I, _D, _I, _P, _P,
                                        _I, ctypes.c_longlong, _P, _I, ctypes.c_longlong, _P],
                                       _I),
+    "psmma_cauchy_blr_dev": ([_I, _I, _D, _P, _I, _P, _P, _P, _P, _I, _D, _I, _D, _P, _I, _P, _P],
+                             _I),
     "psmma_cauchyeig_blr_ex_dev": ([_I, _I, _D, _P, _I, _P, _P, _P, _P, _P, _I, _D, _I, _D, _P,
                                     _I, _P, _P], _I),
     "psmma_cauchyeig_blr_dev": ([_I, _I, _D, _P, _I, _P, _P, _P, _P, _P, _I, _D, _D, _P, _I, _P,
@@ -357,6 +359,22 @@ def cauchyeig_blr_dev(alpha, a, fpole, u, v, difl, difr, c, leaf=256, tol=1e-12,
 
 
 RRLU_TOL = float(np.float32(1.0e-17))  # the reference's default-real TOL
+
+
+def cauchy_blr_dev(alpha, a, d, f, u, v, c, leaf=256, tol=1e-12, beta=0.0, stream=None,
+                   compressor=1):
+    """X = alpha * A * C + beta * X, C(i,j) = u_i v_j / (d_i - f_j), with the off-diagonal leaf
+    blocks compressed by the RRLU (BLR_RRLU, default) or the RRQR (BLR_RRQR).  a, c:
+    column-major (m x n) device views; d, f, u, v: float64 device vectors of length n.
+    Returns the stats dict (blocks, lowrank, dense, zero, rank_sum)."""
+    pa, m, n, lda = _colmajor(a)
+    pc, mc, nc, ldc = _colmajor(c)
+    st = (ctypes.c_longlong * 5)()
+    _check(lib().psmma_cauchy_blr_dev(m, n, alpha, pa, lda, d.data_ptr(), f.data_ptr(),
+                                      u.data_ptr(), v.data_ptr(), leaf, float(tol),
+                                      int(compressor), beta, pc, ldc, st, _stream_ptr(stream)),
+           "psmma_cauchy_blr_dev")
+    return dict(zip(("blocks", "lowrank", "dense", "zero", "rank_sum"), list(st)))
 
 
 def rrlu_cauchyeig_batched_dev(pl, pu, fpole, u, v, difl, difr, tol=RRLU_TOL, kmax=32,

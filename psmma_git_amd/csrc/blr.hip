@@ -1,5 +1,6 @@
 // Compressed form of the eigenvector-merge product X = alpha * A * C + beta * X,
-// C the eigen-Cauchy matrix of psmma_cauchyeig: C is split into leaf x leaf
+// C the eigen-Cauchy matrix of psmma_cauchyeig (and of the plain Cauchy product,
+// C(i,j) = U_i V_j / (D_i - F_j), the same way): C is split into leaf x leaf
 // blocks; every off-diagonal block is compressed by the batched truncated RRQR
 // (Compress_RRQR semantics, tolerance tol) straight from the generators, and
 // the product runs as
@@ -113,10 +114,11 @@ __global__ __launch_bounds__(256) void blr_index_kernel(const int* __restrict__ 
 
 // The diagonal blocks C(J,J) explicitly, column-major leaf x leaf each: grid (nb, leaf),
 // thread i of workgroup (J, j) writes entry (i, j) of block J.
+template <int KIND>
 __global__ __launch_bounds__(256) void blr_diag_kernel(GenPtrs g, int leaf, double* __restrict__ H) {
   const int J = blockIdx.x, j = blockIdx.y, i = threadIdx.x;
   if (i < leaf)
-    H[((size_t)J * leaf + j) * leaf + i] = gen_entry<KIND_CAUCHYEIG>(g, J * leaf + i, J * leaf + j);
+    H[((size_t)J * leaf + j) * leaf + i] = gen_entry<KIND>(g, J * leaf + i, J * leaf + j);
 }
 
 struct DevWs {
@@ -163,31 +165,22 @@ size_t align_up(size_t x) { return (x + 255) & ~size_t(255); }
 
 }  // namespace
 
-extern "C" int psmma_cauchyeig_blr_ex_dev(int m, int nglob, double alpha, const double* a,
-                                          int lda, const double* fpole, const double* u,
-                                          const double* v, const double* difl,
-                                          const double* difr, int leaf, double tol,
-                                          int compressor, double beta, double* c, int ldc,
-                                          long long* stats, void* stream) {
-  if (m < 0) return -1;
-  if (nglob < 0) return -2;
-  if (lda < std::max(1, m)) return -5;
-  if (leaf < 1 || leaf > 256 || (nglob % leaf) != 0) return -11;
-  if (!(tol >= 0.0)) return -12;
-  if (compressor != PSMMA_BLR_RRQR && compressor != PSMMA_BLR_RRLU) return -13;
-  if (ldc < std::max(1, m)) return -16;
-  if (m == 0 || nglob == 0) return 0;
-  if (!a || !c || !fpole || !u || !v || !difl || !difr) return -4;
-  hipStream_t s = (hipStream_t)stream;
+namespace {
+
+// The compressed product for one structured kind (KIND_CAUCHYEIG or KIND_CAUCHY);
+// arguments already checked, m > 0, nglob > 0.
+int blr_product(int kind, const GenPtrs& g, int m, int nglob, double alpha, const double* a,
+                int lda, int leaf, double tol, int compressor, double beta, double* c, int ldc,
+                long long* stats, hipStream_t s) {
   const int nb = nglob / leaf;
   const int nblk = nb * (nb - 1);
-  GenPtrs g{fpole, u, v, difl, difr, nglob, nglob, nglob};
-
+  const bool eig = kind == KIND_CAUCHYEIG;
   const bool rrlu = compressor == PSMMA_BLR_RRLU;
   // workspace: block lists, compressor outputs (Q and R P^T, or Q and Y), and for the
-  // RRQR the evaluated blocks (each generated once, then read by every pass)
+  // RRQR of eigen-Cauchy blocks the evaluated blocks (each generated once, then read by
+  // every pass)
   const size_t qbytes = (size_t)nblk * leaf * kKmax * sizeof(double);
-  const size_t cbytes = rrlu ? 0 : (size_t)nblk * leaf * leaf * sizeof(double);
+  const size_t cbytes = (rrlu || !eig) ? 0 : (size_t)nblk * leaf * leaf * sizeof(double);
   size_t off_r0 = 0, off_c0 = align_up(off_r0 + sizeof(int) * nblk);
   size_t off_rank = align_up(off_c0 + sizeof(int) * nblk);
   size_t off_jp = align_up(off_rank + sizeof(int) * nblk);  // RRQR pivots or RRLU rows
@@ -217,18 +210,26 @@ extern "C" int psmma_cauchyeig_blr_ex_dev(int m, int nglob, double alpha, const 
     if (e == hipSuccess)
       e = hipMemcpyAsync(dc0, hc0.data(), sizeof(int) * nblk, hipMemcpyHostToDevice, s);
     if (e == hipSuccess && !rrlu)
-      e = launch_rrqr_structured(KIND_CAUCHYEIG, g, dr0, dc0, nblk, leaf, leaf, tol, kKmax, drank,
+      e = launch_rrqr_structured(kind, g, dr0, dc0, nblk, leaf, leaf, tol, kKmax, drank,
                                  (int*)(ws + off_jp), leaf, dq, leaf, (long long)leaf * kKmax, drp,
-                                 kKmax, (long long)kKmax * leaf, (double*)(ws + off_cache), s);
+                                 kKmax, (long long)kKmax * leaf,
+                                 eig ? (double*)(ws + off_cache) : nullptr, s);
     if (e == hipSuccess && rrlu) {
       int* dpl = (int*)(ws + off_jp);
       int* dpu = (int*)(ws + off_pu);
       blr_index_kernel<<<nblk, 256, 0, s>>>(dr0, dc0, leaf, dpl, dpu);
       e = hipGetLastError();
       if (e == hipSuccess) {
-        const int st = psmma_rrlu_cauchyeig_batched_dev(
-            nblk, leaf, leaf, dpl, dpu, fpole, u, v, difl, difr, nglob, tol, kKmax, drank,
-            nullptr, dq, leaf, (long long)leaf * kKmax, drp, kKmax, (long long)kKmax * leaf, s);
+        // generators: eigen-Cauchy p0..p4 = F, U, V, DIFL, DIFR; Cauchy p0..p3 = D, F, U, V
+        const int st =
+            eig ? psmma_rrlu_cauchyeig_batched_dev(nblk, leaf, leaf, dpl, dpu, g.p0, g.p1, g.p2,
+                                                   g.p3, g.p4, nglob, tol, kKmax, drank, nullptr,
+                                                   dq, leaf, (long long)leaf * kKmax, drp, kKmax,
+                                                   (long long)kKmax * leaf, s)
+                : psmma_rrlu_cauchy_batched_dev(nblk, leaf, leaf, dpl, dpu, g.p0, g.p1, g.p2,
+                                                g.p3, nglob, nglob, tol, kKmax, drank, dq, leaf,
+                                                (long long)leaf * kKmax, drp, kKmax,
+                                                (long long)kKmax * leaf, s);
         if (st) return st > 0 ? st : 1006;
       }
     }
@@ -239,7 +240,7 @@ extern "C" int psmma_cauchyeig_blr_ex_dev(int m, int nglob, double alpha, const 
     if (e == hipSuccess) e = hipStreamSynchronize(s);
   }
   if (e != hipSuccess) {
-    fprintf(stderr, "psmma_cauchyeig_blr_dev: %s\n", hipGetErrorString(e));
+    fprintf(stderr, "compressed structured product: %s\n", hipGetErrorString(e));
     return 1000 + (int)e;
   }
 
@@ -331,7 +332,10 @@ extern "C" int psmma_cauchyeig_blr_ex_dev(int m, int nglob, double alpha, const 
   // kernel at ~37); then incompressible blocks one by one (they accumulate)
   {
     double* H = (double*)(ws2 + off_h);
-    blr_diag_kernel<<<dim3(nb, leaf), 256, 0, s>>>(g, leaf, H);
+    if (eig)
+      blr_diag_kernel<KIND_CAUCHYEIG><<<dim3(nb, leaf), 256, 0, s>>>(g, leaf, H);
+    else
+      blr_diag_kernel<KIND_CAUCHY><<<dim3(nb, leaf), 256, 0, s>>>(g, leaf, H);
     if ((e = hipGetLastError()) != hipSuccess) return 1000 + (int)e;
     if (rocblas_dgemm_strided_batched(h, rocblas_operation_none, rocblas_operation_none, m, leaf,
                                       leaf, &alpha, a, lda, (rocblas_stride)leaf * lda, H, leaf,
@@ -341,7 +345,7 @@ extern "C" int psmma_cauchyeig_blr_ex_dev(int m, int nglob, double alpha, const 
   }
   for (size_t x = 0; x < dense.size() && e == hipSuccess; ++x) {
     const int b = dense[x], I = hr0[b] / leaf, J = hc0[b] / leaf;
-    e = launch_right_f64(KIND_CAUCHYEIG, -1, m, leaf, leaf, alpha, a + (size_t)I * leaf * lda,
+    e = launch_right_f64(kind, -1, m, leaf, leaf, alpha, a + (size_t)I * leaf * lda,
                          lda, g, I * leaf, J * leaf, 1.0, c + (size_t)J * leaf * ldc, ldc, s);
   }
   if (e == hipSuccess && !packed.empty()) {
@@ -360,7 +364,7 @@ extern "C" int psmma_cauchyeig_blr_ex_dev(int m, int nglob, double alpha, const 
                            sizeof(double) * (size_t)m * (wB - wcol[J]), s);
   }
   if (e != hipSuccess) {
-    fprintf(stderr, "psmma_cauchyeig_blr_dev: %s\n", hipGetErrorString(e));
+    fprintf(stderr, "compressed structured product: %s\n", hipGetErrorString(e));
     return 1000 + (int)e;
   }
   if (!packed.empty()) {
@@ -382,6 +386,47 @@ extern "C" int psmma_cauchyeig_blr_ex_dev(int m, int nglob, double alpha, const 
       return 1005;
   }
   return 0;
+}
+
+}  // namespace
+
+extern "C" int psmma_cauchyeig_blr_ex_dev(int m, int nglob, double alpha, const double* a,
+                                          int lda, const double* fpole, const double* u,
+                                          const double* v, const double* difl,
+                                          const double* difr, int leaf, double tol,
+                                          int compressor, double beta, double* c, int ldc,
+                                          long long* stats, void* stream) {
+  if (m < 0) return -1;
+  if (nglob < 0) return -2;
+  if (lda < std::max(1, m)) return -5;
+  if (leaf < 1 || leaf > 256 || (nglob % leaf) != 0) return -11;
+  if (!(tol >= 0.0)) return -12;
+  if (compressor != PSMMA_BLR_RRQR && compressor != PSMMA_BLR_RRLU) return -13;
+  if (ldc < std::max(1, m)) return -16;
+  if (m == 0 || nglob == 0) return 0;
+  if (!a || !c || !fpole || !u || !v || !difl || !difr) return -4;
+  GenPtrs g{fpole, u, v, difl, difr, nglob, nglob, nglob};
+  return blr_product(KIND_CAUCHYEIG, g, m, nglob, alpha, a, lda, leaf, tol, compressor, beta, c,
+                     ldc, stats, (hipStream_t)stream);
+}
+
+extern "C" int psmma_cauchy_blr_dev(int m, int nglob, double alpha, const double* a, int lda,
+                                    const double* d, const double* f, const double* u,
+                                    const double* v, int leaf, double tol, int compressor,
+                                    double beta, double* c, int ldc, long long* stats,
+                                    void* stream) {
+  if (m < 0) return -1;
+  if (nglob < 0) return -2;
+  if (lda < std::max(1, m)) return -5;
+  if (leaf < 1 || leaf > 256 || (nglob % leaf) != 0) return -10;
+  if (!(tol >= 0.0)) return -11;
+  if (compressor != PSMMA_BLR_RRQR && compressor != PSMMA_BLR_RRLU) return -12;
+  if (ldc < std::max(1, m)) return -15;
+  if (m == 0 || nglob == 0) return 0;
+  if (!a || !c || !d || !f || !u || !v) return -4;
+  GenPtrs g{d, f, u, v, nullptr, nglob, nglob, nglob};
+  return blr_product(KIND_CAUCHY, g, m, nglob, alpha, a, lda, leaf, tol, compressor, beta, c, ldc,
+                     stats, (hipStream_t)stream);
 }
 
 extern "C" int psmma_cauchyeig_blr_dev(int m, int nglob, double alpha, const double* a, int lda,

@@ -105,6 +105,48 @@ def test_blr_rrlu_compressor_matches_oracle(gpu, ora, n, leaf, m):
     assert st["lowrank"] == low and st["rank_sum"] == ranks
 
 
+@pytest.mark.parametrize("compressor", [1, 0])
+def test_cauchy_blr_product_matches_oracle(gpu, ora, compressor):
+    """The compressed form of the plain Cauchy product (the Cauchy benchmark's
+    interlaced generators): error within 20 tol; with the RRLU the same low-rank blocks
+    and rank sum as the oracle's compress_cauchy, with the RRQR the rank sum within 2%
+    of the oracle's Compress_RRQR."""
+    import torch
+
+    n, leaf, m, tol = 2048, 256, 700, 1e-12
+    rng = np.random.default_rng(8)
+    b = inputs.cauchy_generators(n, rng)
+    d, f, u, v = (b[:, i].copy() for i in range(4))
+    a = rng.standard_normal((m, n))
+    x0 = rng.standard_normal((m, n))
+    da, dx = _dev(a), _dev(x0)
+    gd = [torch.from_numpy(np.ascontiguousarray(x)).cuda() for x in (d, f, u, v)]
+    st = api.cauchy_blr_dev(1.3, da, *gd, dx, leaf=leaf, tol=tol, beta=0.5, compressor=compressor)
+    torch.cuda.synchronize()
+    c = ora.cauchy_matrix(d, f, u, v, 0, n, 0, n)
+    err = ora.rel_fro(dx.cpu().numpy(), 1.3 * (a @ c) + 0.5 * x0)
+    print("cauchy blr", compressor, "rel err", err, st, flush=True)
+    assert err <= 20 * tol
+    nb = n // leaf
+    low, ranks = 0, 0
+    for i in range(nb):
+        for j in range(nb):
+            if i == j:
+                continue
+            r, cc = slice(i * leaf, (i + 1) * leaf), slice(j * leaf, (j + 1) * leaf)
+            if compressor == 1:
+                rk, _, _, _ = ora.compress_cauchy(d[r], f[cc], u[r], v[cc], tol)
+            else:
+                rk, _, _ = ora.compress_rrqr(np.ascontiguousarray(c[r, cc]), tol)
+            if 0 < rk <= 32 and rk < leaf:
+                low += 1
+                ranks += rk
+    if compressor == 1:
+        assert st["lowrank"] == low and st["rank_sum"] == ranks
+    else:
+        assert abs(st["rank_sum"] - ranks) <= 0.02 * ranks
+
+
 def test_blr_rejects_unknown_compressor(gpu):
     n = 256
     gens, _, _ = _setup(n, 35)
