@@ -157,6 +157,32 @@ def other_configs(seed, torch, api, inputs):
     torch.cuda.empty_cache()
     out["eig_merge_n16384"] = eig_merge_config(seed, torch, api, inputs)
     out["rrlu_cauchy_leaf256_n16384"] = rrlu_cauchy_blocks(seed, torch, api, inputs)
+    out["cauchy_n16384_compressed"] = cauchy_compressed(seed, torch, api, inputs)
+    return out
+
+
+def cauchy_compressed(seed, torch, api, inputs, n=16384, leaf=256):
+    """Plain Cauchy x dense at N = 16384 (the Cauchy benchmark's interlaced generators): the
+    plain FP64 product and the compressed forms (RRLU and RRQR compressors, tol 1e-12), error
+    of the compressed forms against the FP64 product."""
+    b = inputs.cauchy_generators(n, np.random.default_rng(seed))
+    d, f, u, v = (torch.from_numpy(np.ascontiguousarray(b[:, i])).cuda() for i in range(4))
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(seed)
+    a = torch.randn((n, n), dtype=torch.float64, device="cuda", generator=gen).t()
+    ref = api.colmajor_empty(n, n)
+    x = api.colmajor_empty(n, n)
+    ms64 = ev_time(lambda: api.cauchy_dev(1.0, a, d, f, u, v, n, ref), 3, torch)
+    out = {"n": n, "fp64_ms": round(ms64, 3), "fp64_tflops": round(2.0 * n**3 / ms64 / 1e9, 2)}
+    for name, comp in (("rrlu", api.BLR_RRLU), ("rrqr", api.BLR_RRQR)):
+        stats = {}
+        ms = ev_time(lambda: stats.update(api.cauchy_blr_dev(1.0, a, d, f, u, v, x, leaf=leaf,
+                                                             tol=1e-12, compressor=comp)),
+                     3, torch)
+        out[name] = {"ms": round(ms, 3), "speedup_vs_fp64": round(ms64 / ms, 3),
+                     "rel_err_vs_fp64": rel(x, ref), "stats": stats}
+    del a, ref, x
+    torch.cuda.empty_cache()
     return out
 
 
