@@ -6,6 +6,7 @@ module psmma_hip
   implicit none
   integer(c_int), parameter :: PSMMA_PREC_FP64 = 0, PSMMA_PREC_SPLIT_I8 = 1, &
                                PSMMA_PREC_SPLIT_BF16 = 2
+  integer(c_int), parameter :: PSMMA_BLR_RRQR = 0, PSMMA_BLR_RRLU = 1
   interface
     integer(c_int) function psmma_toeplitz_dev(m, n, k, alpha, a, lda, g, lg, korder, r0, c0, &
                                                beta, c, ldc, prec, slices, stream) &
@@ -50,6 +51,73 @@ module psmma_hip
     end function
     integer(c_int) function psmma_last_status() bind(C, name="psmma_last_status")
       import :: c_int
+    end function
+    ! the merge step's generators: secular roots as DIFL / DIFR, Loewner's z, normalisers
+    integer(c_int) function psmma_secular_dev(n, d, z, rho, difl, difr, zhat, v, stream) &
+        bind(C, name="psmma_secular_dev")
+      import :: c_int, c_double, c_ptr
+      integer(c_int), value :: n
+      real(c_double), value :: rho
+      type(c_ptr), value :: d, z, difl, difr, zhat, v, stream
+    end function
+    ! Compress_RRQR on a batch of explicit blocks
+    integer(c_int) function psmma_rrqr_batched_dev(nblk, m, n, a, lda, stride_a, tol, kmax, &
+                                                   rank, jpvt, q, ldq, stride_q, rpt, ldr, &
+                                                   stride_r, stream) &
+        bind(C, name="psmma_rrqr_batched_dev")
+      import :: c_int, c_long_long, c_double, c_ptr
+      integer(c_int), value :: nblk, m, n, lda, kmax, ldq, ldr
+      integer(c_long_long), value :: stride_a, stride_q, stride_r
+      real(c_double), value :: tol
+      type(c_ptr), value :: a, rank, jpvt, q, rpt, stream
+    end function
+    ! Compress_cauchyEig2 / compress_cauchy on batches of blocks given by global rows / columns
+    integer(c_int) function psmma_rrlu_cauchyeig_batched_dev(nblk, m, n, pl, pu, fpole, u, v, &
+                                                             difl, difr, korder, tol, kmax, rank, &
+                                                             nswap, q, ldq, stride_q, y, ldy, &
+                                                             stride_y, stream) &
+        bind(C, name="psmma_rrlu_cauchyeig_batched_dev")
+      import :: c_int, c_long_long, c_double, c_ptr
+      integer(c_int), value :: nblk, m, n, korder, kmax, ldq, ldy
+      integer(c_long_long), value :: stride_q, stride_y
+      real(c_double), value :: tol
+      type(c_ptr), value :: pl, pu, fpole, u, v, difl, difr, rank, nswap, q, y, stream
+    end function
+    integer(c_int) function psmma_rrlu_cauchy_batched_dev(nblk, m, n, pl, pu, d, f, u, v, nrow, &
+                                                          ncol, tol, kmax, rank, x, ldx, stride_x, &
+                                                          y, ldy, stride_y, stream) &
+        bind(C, name="psmma_rrlu_cauchy_batched_dev")
+      import :: c_int, c_long_long, c_double, c_ptr
+      integer(c_int), value :: nblk, m, n, nrow, ncol, kmax, ldx, ldy
+      integer(c_long_long), value :: stride_x, stride_y
+      real(c_double), value :: tol
+      type(c_ptr), value :: pl, pu, d, f, u, v, rank, x, y, stream
+    end function
+    ! compressed products (stats: host array of 5 integer(c_long_long), may be c_null_ptr)
+    integer(c_int) function psmma_cauchyeig_blr_dev(m, n, alpha, a, lda, fpole, u, v, difl, difr, &
+                                                    leaf, tol, beta, c, ldc, stats, stream) &
+        bind(C, name="psmma_cauchyeig_blr_dev")
+      import :: c_int, c_double, c_ptr
+      integer(c_int), value :: m, n, lda, leaf, ldc
+      real(c_double), value :: alpha, tol, beta
+      type(c_ptr), value :: a, fpole, u, v, difl, difr, c, stats, stream
+    end function
+    integer(c_int) function psmma_cauchyeig_blr_ex_dev(m, n, alpha, a, lda, fpole, u, v, difl, &
+                                                       difr, leaf, tol, compressor, beta, c, ldc, &
+                                                       stats, stream) &
+        bind(C, name="psmma_cauchyeig_blr_ex_dev")
+      import :: c_int, c_double, c_ptr
+      integer(c_int), value :: m, n, lda, leaf, compressor, ldc
+      real(c_double), value :: alpha, tol, beta
+      type(c_ptr), value :: a, fpole, u, v, difl, difr, c, stats, stream
+    end function
+    integer(c_int) function psmma_cauchy_blr_dev(m, n, alpha, a, lda, d, f, u, v, leaf, tol, &
+                                                 compressor, beta, c, ldc, stats, stream) &
+        bind(C, name="psmma_cauchy_blr_dev")
+      import :: c_int, c_double, c_ptr
+      integer(c_int), value :: m, n, lda, leaf, compressor, ldc
+      real(c_double), value :: alpha, tol, beta
+      type(c_ptr), value :: a, d, f, u, v, c, stats, stream
     end function
   end interface
 end module psmma_hip

@@ -31,6 +31,26 @@ def test_python_mirror_binds_every_declared_symbol():
     assert set(header_functions()) <= set(api.declared_symbols())
 
 
+FORTRAN_MODULE = Path(__file__).resolve().parents[1] / "psmma_git_amd" / "fortran" / "psmma_hip.f90"
+FLANG = Path("/opt/rocm/llvm/bin/amdflang")
+
+
+@pytest.mark.skipif(not FLANG.exists(), reason="amdflang not installed")
+def test_fortran_interface_module_compiles_and_binds_declared_symbols(tmp_path):
+    """The ISO_C_BINDING module compiles, and every C name it binds is declared in
+    include/psmma.h and exported by the library."""
+    import subprocess
+
+    r = subprocess.run([str(FLANG), "-c", "-o", str(tmp_path / "m.o"), "-J", str(tmp_path),
+                        str(FORTRAN_MODULE)], capture_output=True, text=True, cwd=tmp_path)
+    assert r.returncode == 0, r.stderr
+    bound = re.findall(r'bind\(C, name="([a-z0-9_]+)"\)', FORTRAN_MODULE.read_text())
+    assert len(bound) >= 14
+    assert set(bound) <= set(header_functions())
+    L = api.lib()
+    assert all(hasattr(L, n) for n in bound)
+
+
 def test_version_string():
     assert "gfx950" in api.lib().psmma_version().decode()
 
