@@ -69,7 +69,7 @@ def cpu_baseline(seed: int):
     p = 4 if cores >= 16 else 2
     q = max(1, cores // p)
     n = 8192
-    g, a = inputs.config1(n, seed=seed)
+    g, a = inputs.toeplitz_hankel_inputs(n, seed=seed)
     t0 = time.perf_counter()
     oracle.ps_compute(oracle.TOEPLITZ, p, q, 64, 1.0, a, g, mode=oracle.MODE_DENSE,
                       nthreads=p * q)
@@ -107,7 +107,7 @@ def other_configs(seed, torch, api, inputs):
     out = {}
     # Cauchy-like x dense, N = 2048
     n = 2048
-    b, a_np = inputs.config0(n, seed=seed)
+    b, a_np = inputs.cauchy_dense_inputs(n, seed=seed)
     a = colmajor_from_numpy(a_np, torch)
     d, f, u, v = (torch.from_numpy(np.ascontiguousarray(b[:, i])).cuda() for i in range(4))
     c = api.colmajor_empty(n, n)
@@ -124,7 +124,7 @@ def other_configs(seed, torch, api, inputs):
     del a, c, h, ref
     # Hankel x dense, N = 8192, and the 3-slice bf16 error of the Toeplitz product
     n = 8192
-    g_np, a_np = inputs.config1(n, seed=seed + 1)
+    g_np, a_np = inputs.toeplitz_hankel_inputs(n, seed=seed + 1)
     a = colmajor_from_numpy(a_np, torch)
     del a_np
     g = torch.from_numpy(g_np).cuda()
@@ -142,7 +142,7 @@ def other_configs(seed, torch, api, inputs):
     del a, c, tref
     # DFT x complex dense, 4096 x 512
     n, k = 4096, 512
-    bb = inputs.config2(n, k, seed=seed)
+    bb = inputs.dft_inputs(n, k, seed=seed)
     db = torch.from_numpy(np.ascontiguousarray(bb.T)).cuda().t()
     dc = api.colmajor_empty(n, k, dtype=torch.complex128)
     msd = ev_time(lambda: api.dft_dev(1.0 / np.sqrt(n), db, n, n, dc), 10, torch)
@@ -191,7 +191,7 @@ def eig_merge_config(seed, torch, api, inputs, n=16384):
     generators from the GPU secular solver (roots, Loewner z, column normalisers), Q from the QR
     of a Gaussian matrix; plain FP64 and split forms, error against the FP64 oracle (explicit C
     and rocBLAS DGEMM) and orthogonality ||I - X^T X||."""
-    d_np, z_np, _ = inputs.config3_dz(n, seed)
+    d_np, z_np, _ = inputs.merge_poles_and_z(n, seed)
     td = torch.from_numpy(d_np).cuda()
     tz = torch.from_numpy(z_np).cuda()
     gens = api.secular_dev(td, tz, 1.0, loewner=True)
@@ -321,7 +321,7 @@ def rrlu_blocks(td, u, v, difl, difr, torch, api, leaf=256):
 def batched_config(seed, torch, api, inputs):
     """512 uneven problems (n log-uniform in [64, 2048], half Cauchy, a quarter Toeplitz and
     Hankel), one hipGraph-captured grouped FP64 launch vs the split-precision plan."""
-    probs_np = inputs.config4(512, seed=seed)
+    probs_np = inputs.batch_inputs(512, seed=seed)
     probs, keep = [], []
     flops = 0.0
     for pr in probs_np:
@@ -397,7 +397,7 @@ def profile_leg(leg, seed, torch, api, inputs):
         return batched_config(seed, torch, api, inputs)
     if leg == "split":
         n = 8192
-        g_np, a_np = inputs.config1(n, seed=seed)
+        g_np, a_np = inputs.toeplitz_hankel_inputs(n, seed=seed)
         g = torch.from_numpy(g_np).cuda()
         a = colmajor_from_numpy(a_np, torch)
         c = api.colmajor_empty(n, n)
@@ -405,7 +405,7 @@ def profile_leg(leg, seed, torch, api, inputs):
                      2, torch)
         return {"leg": leg, "n": n, "slices": 13, "ms": round(ms, 3)}
     n = 16384
-    d_np, z_np, _ = inputs.config3_dz(n, seed)
+    d_np, z_np, _ = inputs.merge_poles_and_z(n, seed)
     td, tz = torch.from_numpy(d_np).cuda(), torch.from_numpy(z_np).cuda()
     difl, difr, u, v = api.secular_dev(td, tz, 1.0, loewner=True)
     if leg == "rrlu":
@@ -460,7 +460,7 @@ def main():
         return
     n = args.n
     log(f"[bench] generating Toeplitz x dense N={n} seed={args.seed}")
-    g_np, a_np = inputs.config1(n, seed=args.seed)
+    g_np, a_np = inputs.toeplitz_hankel_inputs(n, seed=args.seed)
     g = torch.from_numpy(g_np).cuda()
     a = colmajor_from_numpy(a_np, torch)
     del a_np

@@ -27,21 +27,21 @@ def dense_uniform(m: int, n: int, rng: np.random.Generator) -> np.ndarray:
     return np.asfortranarray(rng.uniform(-1.0, 1.0, size=m * n).reshape((m, n), order="F"))
 
 
-def config0(n: int = 2048, seed: int = 0):
+def cauchy_dense_inputs(n: int = 2048, seed: int = 0):
     """Plain Cauchy-like x dense.  Returns (gens B(n,4), dense (n x n))."""
     rng = np.random.default_rng(seed)
     b = cauchy_generators(n, rng)
     return b, dense_uniform(n, n, rng)
 
 
-def config1(n: int = 8192, seed: int = 0, kind: str = "toeplitz"):
+def toeplitz_hankel_inputs(n: int = 8192, seed: int = 0, kind: str = "toeplitz"):
     """Toeplitz (or Hankel) x dense: generator of length 2n-1 ~ N(0,1), dense U[-1,1]."""
     rng = np.random.default_rng(seed)
     g = rng.standard_normal(2 * n - 1)
     return g, dense_uniform(n, n, rng)
 
 
-def config2(n: int = 4096, ncols: int = 512, seed: int = 0) -> np.ndarray:
+def dft_inputs(n: int = 4096, ncols: int = 512, seed: int = 0) -> np.ndarray:
     """Complex dense operand of the DFT product: re and im i.i.d. N(0,1)."""
     rng = np.random.default_rng(seed)
     re = rng.standard_normal(n * ncols).reshape((n, ncols), order="F")
@@ -131,8 +131,8 @@ def column_normalisers(d, u, difl, difr, chunk: int = 1024):
     return v
 
 
-def config3_dz(n: int = 16384, seed: int = 0):
-    """The first draws of config3 (same seed, same order): sorted poles d and unit z."""
+def merge_poles_and_z(n: int = 16384, seed: int = 0):
+    """The first draws of eig_merge_inputs (same seed, same order): sorted poles d and unit z."""
     rng = np.random.default_rng(seed)
     d = np.sort(rng.random(n))
     z = rng.standard_normal(n)
@@ -140,12 +140,12 @@ def config3_dz(n: int = 16384, seed: int = 0):
     return d, z, rng
 
 
-def config3(n: int = 16384, seed: int = 0, with_q: bool = True):
+def eig_merge_inputs(n: int = 16384, seed: int = 0, with_q: bool = True):
     """Eigenvector merge: d sorted U[0,1), z ~ N(0,1) normalised, rho = 1.
 
     Returns dict with f (= d, the poles), u (= z), v (column normalisers),
     difl, difr and optionally the orthogonal q (QR of an n x n Gaussian)."""
-    d, z, rng = config3_dz(n, seed)
+    d, z, rng = merge_poles_and_z(n, seed)
     difl, difr = secular_roots(d, z, 1.0)
     # column normaliser: v_j = 1 / || z / (d - lam_j) ||, with d_i - lam_j formed
     # exactly as the tile generator does
@@ -158,7 +158,7 @@ def config3(n: int = 16384, seed: int = 0, with_q: bool = True):
     return out
 
 
-def config4(nprob: int = 512, seed: int = 0, nmin: int = 64, nmax: int = 2048):
+def batch_inputs(nprob: int = 512, seed: int = 0, nmin: int = 64, nmax: int = 2048):
     """Batch of uneven structured products: sizes log-uniform in [nmin, nmax], k = n;
     kinds in a seeded shuffle, half Cauchy, a quarter each Toeplitz and Hankel.
     Each problem: dict(kind, n, gens, dense)."""

@@ -28,7 +28,7 @@ def make_batch(rng, nprob, nmax, beta_zero=True, nmin=1):
             gens = dict(g=rng.standard_normal(2 * nglob - 1), korder=nglob)
             order = ["g"]
         else:
-            e = inputs.config3(nglob, seed=int(rng.integers(1 << 30)), with_q=False)
+            e = inputs.eig_merge_inputs(nglob, seed=int(rng.integers(1 << 30)), with_q=False)
             gens = dict(f=e["f"], u=e["u"], v=e["v"], difl=e["difl"], difr=e["difr"])
             order = ["f", "u", "v", "difl", "difr"]
         dgens = [torch.from_numpy(np.ascontiguousarray(gens[k])).cuda() for k in order]

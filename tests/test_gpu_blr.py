@@ -20,7 +20,7 @@ pytestmark = pytest.mark.gpu
 def _setup(n, seed):
     import torch
 
-    d, z, rng = inputs.config3_dz(n, seed)
+    d, z, rng = inputs.merge_poles_and_z(n, seed)
     td, tz = torch.from_numpy(d).cuda(), torch.from_numpy(z).cuda()
     difl, difr, u, v = api.secular_dev(td, tz, 1.0, loewner=True)
     torch.cuda.synchronize()

@@ -49,7 +49,7 @@ def make_gens(kind, nglob, rng):
     elif kind in ("toeplitz", "hankel"):
         gens = dict(g=rng.standard_normal(2 * nglob - 1), korder=nglob)
     else:
-        e = inputs.config3(nglob, seed=int(rng.integers(1 << 30)), with_q=False)
+        e = inputs.eig_merge_inputs(nglob, seed=int(rng.integers(1 << 30)), with_q=False)
         gens = dict(f=e["f"], u=e["u"], v=e["v"], difl=e["difl"], difr=e["difr"])
     dev = {k: torch.from_numpy(np.ascontiguousarray(v)).cuda() for k, v in gens.items()
            if isinstance(v, np.ndarray)}
@@ -173,7 +173,7 @@ def test_dft_config_shape_vs_fft(gpu, ora):
     import torch
 
     n, ncols = 4096, 512
-    b = inputs.config2(n, ncols, seed=0)
+    b = inputs.dft_inputs(n, ncols, seed=0)
     db = dev_colmajor(b)
     dc = dev_colmajor(np.zeros((n, ncols), complex))
     api.dft_dev(1.0 / np.sqrt(n), db, n, n, dc)
@@ -255,7 +255,7 @@ def test_fortran_cauchy_entry_symmetric_toeplitz_and_split_lda(gpu, ora):
 
 def test_fortran_cauchyeig_entry(gpu, ora):
     n = 160
-    e = inputs.config3(n, seed=3, with_q=True)
+    e = inputs.eig_merge_inputs(n, seed=3, with_q=True)
     q = np.zeros((n + 4, n + 2), order="F")
     d = api.descinit(n, n, 32, 32, n)
     dq = api.descinit(n + 4, n + 2, 32, 32, n + 4)

@@ -20,7 +20,7 @@ pytestmark = pytest.mark.gpu
 
 
 def _eig(n, seed):
-    e = inputs.config3(n, seed=seed, with_q=False)
+    e = inputs.eig_merge_inputs(n, seed=seed, with_q=False)
     f, dl, dr = e["f"], e["difl"], e["difr"]
     zh = inputs.loewner_z(f, e["u"], dl, dr)
     v = inputs.column_normalisers(f, zh, dl, dr)
@@ -79,7 +79,7 @@ def test_rrlu_blocks_from_gpu_secular_solver(gpu, ora, tol):
     import torch
 
     n, leaf = 4096, 256
-    d, z, _ = inputs.config3_dz(n, 2)
+    d, z, _ = inputs.merge_poles_and_z(n, 2)
     td, tz = torch.from_numpy(d).cuda(), torch.from_numpy(z).cuda()
     difl, difr, u, v = api.secular_dev(td, tz, 1.0, loewner=True)
     torch.cuda.synchronize()

@@ -120,7 +120,7 @@ def test_rrqr_eig_cauchy_off_diagonal_blocks(gpu, ora):
     """The compression the eigenvector-merge variant relies on: off-diagonal blocks of
     the eigen-Cauchy matrix at tolerance 1e-12."""
     n, leaf = 1024, 256
-    e = inputs.config3(n, seed=3, with_q=False)
+    e = inputs.eig_merge_inputs(n, seed=3, with_q=False)
     zh = inputs.loewner_z(e["f"], e["u"], e["difl"], e["difr"])
     v = inputs.column_normalisers(e["f"], zh, e["difl"], e["difr"])
     c = ora.cauchyeig_matrix(e["f"], zh, v, e["difl"], e["difr"], 0, n, 0, n)

@@ -33,7 +33,7 @@ def _rel(a, b):
 @pytest.mark.parametrize("n,seed", [(1, 0), (2, 1), (7, 2), (300, 3), (1000, 4)])
 def test_secular_roots_match_oracle(gpu, n, seed):
     print(f"secular roots n={n} seed={seed}", flush=True)
-    d, z, _ = inputs.config3_dz(n, seed)
+    d, z, _ = inputs.merge_poles_and_z(n, seed)
     wl, wr = inputs.secular_roots(d, z, 1.0)
     _, difl, difr, u, v = _gpu_gens(d, z, loewner=True)
     gl, gr = difl.cpu().numpy(), difr.cpu().numpy()
@@ -48,7 +48,7 @@ def test_secular_roots_match_oracle(gpu, n, seed):
 
 def test_secular_without_loewner_normalises_z(gpu):
     n = 400
-    d, z, _ = inputs.config3_dz(n, 11)
+    d, z, _ = inputs.merge_poles_and_z(n, 11)
     _, difl, difr, u, v = _gpu_gens(d, z, loewner=False)
     assert np.array_equal(u.cpu().numpy(), z)
     vw = inputs.column_normalisers(d, z, difl.cpu().numpy(), difr.cpu().numpy())
@@ -75,7 +75,7 @@ def test_merge_product_orthonormal(gpu, ora, prec, slices, tol):
 
     n = 768
     print(f"merge product n={n} prec={prec}", flush=True)
-    d, z, rng = inputs.config3_dz(n, 21)
+    d, z, rng = inputs.merge_poles_and_z(n, 21)
     td, difl, difr, u, v = _gpu_gens(d, z, loewner=True)
     g = rng.standard_normal((n, n))
     q, _ = np.linalg.qr(g)

@@ -90,7 +90,7 @@ def test_split_i8_wide_dynamic_range_cauchy(gpu, ora):
     from oracle import inputs
 
     n = 2048
-    b, a = inputs.config0(n, seed=1)
+    b, a = inputs.cauchy_dense_inputs(n, seed=1)
     d, f, u, v = (torch.from_numpy(np.ascontiguousarray(b[:, i])).cuda() for i in range(4))
     da = dev_colmajor(a)
     dc = dev_colmajor(np.zeros((n, n)))
@@ -98,7 +98,7 @@ def test_split_i8_wide_dynamic_range_cauchy(gpu, ora):
     torch.cuda.synchronize()
     gens = dict(d=b[:, 0], f=b[:, 1], u=b[:, 2], v=b[:, 3])
     err = ora.rel_fro(host(dc), ora.structured_gemm_np(0, 1.0, a, gens, 0, 0, n))
-    print(f"config0 split err {err:.3e}")
+    print(f"cauchy_dense_inputs split err {err:.3e}")
     assert err <= 1e-12
 
 

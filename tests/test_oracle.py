@@ -187,7 +187,7 @@ def test_ring_emulation_dense_equals_product(ora, grid, nb):
 
 def test_ring_emulation_cauchyeig(ora):
     n = 48
-    e = inputs.config3(n, seed=2, with_q=True)
+    e = inputs.eig_merge_inputs(n, seed=2, with_q=True)
     gen = np.asfortranarray(np.stack([e["f"], e["f"] + e["difl"], e["u"], e["v"], e["difl"],
                                       e["difr"]], axis=1))
     want = ora.structured_gemm_np(ora.CAUCHYEIG, 1.0, e["q"], e, 0, 0, n)
@@ -196,11 +196,11 @@ def test_ring_emulation_cauchyeig(ora):
 
 
 # ---------------------------------------------------------------------------
-# Eigenvector-Cauchy generators (config 3): secular roots and orthogonality.
+# Eigenvector-Cauchy generators of the merge step: secular roots and orthogonality.
 
 def test_secular_roots_match_dense_eigensolver():
     n = 120
-    e = inputs.config3(n, seed=4, with_q=False)
+    e = inputs.eig_merge_inputs(n, seed=4, with_q=False)
     d, z = e["f"], e["u"]
     lam = d + e["difl"]
     want = np.linalg.eigvalsh(np.diag(d) + np.outer(z, z))
@@ -212,7 +212,7 @@ def test_secular_roots_match_dense_eigensolver():
 
 def test_cauchyeig_columns_are_orthonormal_eigenvectors(ora):
     n = 150
-    e = inputs.config3(n, seed=5, with_q=False)
+    e = inputs.eig_merge_inputs(n, seed=5, with_q=False)
     c = ora.cauchyeig_matrix(e["f"], e["u"], e["v"], e["difl"], e["difr"], 0, n, 0, n)
     assert np.max(np.abs(np.linalg.norm(c, axis=0) - 1.0)) < 1e-14
     # eigenvector check: (D + z z') c_j = lam_j c_j
@@ -226,7 +226,7 @@ def test_loewner_z_makes_roots_exact_and_vectors_orthonormal(ora):
     """Loewner's theorem: D + zhat zhat^T has exactly the computed roots as eigenvalues,
     and the eigen-Cauchy columns built from zhat are orthonormal to rounding level."""
     n = 160
-    e = inputs.config3(n, seed=6, with_q=False)
+    e = inputs.eig_merge_inputs(n, seed=6, with_q=False)
     d, z, difl, difr = e["f"], e["u"], e["difl"], e["difr"]
     zh = inputs.loewner_z(d, z, difl, difr)
     assert np.all(np.sign(zh) == np.sign(z))
@@ -245,6 +245,6 @@ def test_loewner_z_makes_roots_exact_and_vectors_orthonormal(ora):
 
 def test_column_normalisers_match_explicit_norms(ora):
     n = 90
-    e = inputs.config3(n, seed=7, with_q=False)
+    e = inputs.eig_merge_inputs(n, seed=7, with_q=False)
     c = ora.cauchyeig_matrix(e["f"], e["u"], np.ones(n), e["difl"], e["difr"], 0, n, 0, n)
     assert np.max(np.abs(e["v"] * np.linalg.norm(c, axis=0) - 1.0)) < 1e-14

@@ -136,7 +136,7 @@ def test_faithful_cauchy_ring_low_rank_branch(ora):
 # Eigen-Cauchy blocks (rrluCauchyEigrow / Compress_cauchyEig2).
 
 def _eig(n, seed):
-    e = inputs.config3(n, seed=seed, with_q=False)
+    e = inputs.eig_merge_inputs(n, seed=seed, with_q=False)
     f, dl, dr = e["f"], e["difl"], e["difr"]
     zh = inputs.loewner_z(f, e["u"], dl, dr)
     v = inputs.column_normalisers(f, zh, dl, dr)

@@ -89,7 +89,7 @@ def test_compress_rrqr_zero_matrix(ora):
 def test_compress_rrqr_ranks_of_eig_cauchy_blocks(ora):
     """Off-diagonal blocks of the eigenvector matrix: ranks as LAPACK's R gives them."""
     n, leaf = 512, 128
-    e = inputs.config3(n, seed=3, with_q=False)
+    e = inputs.eig_merge_inputs(n, seed=3, with_q=False)
     zh = inputs.loewner_z(e["f"], e["u"], e["difl"], e["difr"])
     v = inputs.column_normalisers(e["f"], zh, e["difl"], e["difr"])
     c = ora.cauchyeig_matrix(e["f"], zh, v, e["difl"], e["difr"], 0, n, 0, n)
