@@ -138,6 +138,22 @@ def test_rrlu_full_rank_and_shapes(gpu, ora):
     assert rank[0] == rk > 4
 
 
+def test_rrlu_tiny_and_degenerate_blocks(gpu, ora):
+    """1 x 1, 1 x n, m x 1 and 2 x 2 blocks, and an all-zero block: with every pivot zero the
+    reference's strict test |u_kk| < tol * max |u| never fires, so the zero block is reported
+    at full rank (its explicit block is exact); the quotients it divides by zero run on both
+    sides alike."""
+    gens = _eig(64, 9)
+    for rows, cols in [([3], [40]), ([3], [30, 31, 32, 33, 34]), ([0, 1, 2, 3, 4], [40]),
+                       ([0, 1], [50, 51])]:
+        _check(ora, gens, [(np.array(rows), np.array(cols))], api.RRLU_TOL)
+    f, dl, dr, zh, v = gens
+    u0 = zh.copy()
+    u0[:8] = 0.0
+    ranks, _ = _check(ora, (f, dl, dr, u0, v), [(np.arange(8), np.arange(40, 48))], api.RRLU_TOL)
+    assert ranks[0] == 8
+
+
 def test_rrlu_rejects_bad_arguments(gpu):
     import torch
 

@@ -97,6 +97,16 @@ def test_rrlu_cauchy_block_cyclic_ring_tiles(gpu, ora):
     _check(ora, gens, blocks, 1e-10)
 
 
+def test_rrlu_cauchy_tiny_and_degenerate_blocks(gpu, ora):
+    b = inputs.cauchy_generators(64, np.random.default_rng(3))
+    d, f, u, v = b[:, 0], b[:, 1], b[:, 2].copy(), b[:, 3]
+    for rows, cols in [([3], [40]), ([3], [30, 31, 32, 33, 34]), ([0, 1, 2, 3, 4], [40]),
+                       ([0, 1], [50, 51])]:
+        _check(ora, (d, f, u, v), [(np.array(rows), np.array(cols))], api.RRLU_TOL)
+    u[:8] = 0.0  # an all-zero block: full rank, as the reference reports it
+    assert _check(ora, (d, f, u, v), [(np.arange(8), np.arange(40, 48))], api.RRLU_TOL)[0] == 8
+
+
 def test_rrlu_cauchy_rejects_bad_arguments(gpu):
     L = api.lib()
     assert L.psmma_rrlu_cauchy_batched_dev(1, 300, 4, None, None, None, None, None, None, 8, 8,
