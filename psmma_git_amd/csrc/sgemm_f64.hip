@@ -556,13 +556,16 @@ struct RightTile {
   int bm, bn;
 };
 
+// pad_lds: bytes of dynamic LDS the kernel does not use, requested only to cap the
+// number of resident workgroups per CU.
 template <int KIND, int BM, int BN, int BK, int WGM, int WGN, int MINB = 2>
 static hipError_t launch_right_t(int M, int N, int K, double alpha, const double* A, int lda,
                                  const GenPtrs& g, int r0, int c0, double beta, double* C, int ldc,
-                                 hipStream_t s) {
+                                 hipStream_t s, unsigned pad_lds = 0) {
   int nb = ((M + BM - 1) / BM) * ((N + BN - 1) / BN);
   hipLaunchKernelGGL((sgemm_f64_right_kernel<KIND, BM, BN, BK, WGM, WGN, MINB>), dim3(nb),
-                     dim3(WGM * WGN * 64), 0, s, M, N, K, alpha, A, lda, g, r0, c0, beta, C, ldc);
+                     dim3(WGM * WGN * 64), pad_lds, s, M, N, K, alpha, A, lda, g, r0, c0, beta, C,
+                     ldc);
   return hipGetLastError();
 }
 
@@ -583,9 +586,19 @@ static hipError_t launch_right_kind(int cfg, int M, int N, int K, double alpha, 
     case 4:  // 256x128, 8 waves of 64x64: half the generated entries per flop (Cauchy kinds)
       return launch_right_t<KIND, 256, 128, 16, 4, 2, 1>(M, N, K, alpha, A, lda, g, r0, c0, beta,
                                                          C, ldc, s);
-    case 5:  // 128x128, 8 waves of 32x64: two waves per SIMD with one workgroup per CU
+    case 5: {  // 128x128, 8 waves of 32x64: two waves per SIMD with one workgroup per CU
+      // 74 KB of static LDS would let two workgroups share a CU while others idle
+      // (chosen from one tile per CU up); 16 KB of padding keeps it to one per CU:
+      // Cauchy N = 2048 0.41 -> 0.38 ms on MI355X, N = 4096 unchanged.
+      constexpr unsigned kPad = 16 * 1024;
+      static_assert(2 * (sizeof(double) * 2 * (RightLds<KIND, 128, 128, 16>::ASZ +
+                                               RightLds<KIND, 128, 128, 16>::HSZ)) +
+                            2 * kPad > 160 * 1024 ||
+                        KIND == KIND_TOEPLITZ || KIND == KIND_HANKEL,
+                    "padding must exclude a second workgroup");
       return launch_right_t<KIND, 128, 128, 16, 4, 2, 1>(M, N, K, alpha, A, lda, g, r0, c0, beta,
-                                                         C, ldc, s);
+                                                         C, ldc, s, kPad);
+    }
     default:  // small problems: 64x64 workgroup tile, 4 waves of 32x32
       return launch_right_t<KIND, 64, 64, 16, 2, 2>(M, N, K, alpha, A, lda, g, r0, c0, beta, C,
                                                     ldc, s);
