@@ -581,8 +581,15 @@ static hipError_t launch_right_kind(int cfg, int M, int N, int K, double alpha, 
       return launch_right_t<KIND, 128, 128, 32, 2, 2>(M, N, K, alpha, A, lda, g, r0, c0, beta, C,
                                                       ldc, s);
     case 3:  // 64x256: fewer dense-operand re-reads per flop
-      return launch_right_t<KIND, 64, 256, 16, 1, 4>(M, N, K, alpha, A, lda, g, r0, c0, beta, C,
-                                                     ldc, s);
+      // Window kinds with BK = 32: half the barriers per flop, still two workgroups per
+      // CU (45 KB of LDS, 217 registers): Toeplitz N = 8192 68.3 -> 69.6 TF/s on MI355X
+      // (BK = 48 the same, BK = 64 62 TF/s at one workgroup per CU).
+      if constexpr (KIND == KIND_TOEPLITZ || KIND == KIND_HANKEL)
+        return launch_right_t<KIND, 64, 256, 32, 1, 4>(M, N, K, alpha, A, lda, g, r0, c0, beta, C,
+                                                       ldc, s);
+      else
+        return launch_right_t<KIND, 64, 256, 16, 1, 4>(M, N, K, alpha, A, lda, g, r0, c0, beta, C,
+                                                       ldc, s);
     case 4:  // 256x128, 8 waves of 64x64: half the generated entries per flop (Cauchy kinds)
       return launch_right_t<KIND, 256, 128, 16, 4, 2, 1>(M, N, K, alpha, A, lda, g, r0, c0, beta,
                                                          C, ldc, s);
