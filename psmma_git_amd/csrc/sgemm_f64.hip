@@ -556,16 +556,13 @@ struct RightTile {
   int bm, bn;
 };
 
-// pad_lds: bytes of dynamic LDS the kernel does not use, requested only to cap the
-// number of resident workgroups per CU.
 template <int KIND, int BM, int BN, int BK, int WGM, int WGN, int MINB = 2>
 static hipError_t launch_right_t(int M, int N, int K, double alpha, const double* A, int lda,
                                  const GenPtrs& g, int r0, int c0, double beta, double* C, int ldc,
-                                 hipStream_t s, unsigned pad_lds = 0) {
+                                 hipStream_t s) {
   int nb = ((M + BM - 1) / BM) * ((N + BN - 1) / BN);
   hipLaunchKernelGGL((sgemm_f64_right_kernel<KIND, BM, BN, BK, WGM, WGN, MINB>), dim3(nb),
-                     dim3(WGM * WGN * 64), pad_lds, s, M, N, K, alpha, A, lda, g, r0, c0, beta, C,
-                     ldc);
+                     dim3(WGM * WGN * 64), 0, s, M, N, K, alpha, A, lda, g, r0, c0, beta, C, ldc);
   return hipGetLastError();
 }
 
@@ -593,18 +590,17 @@ static hipError_t launch_right_kind(int cfg, int M, int N, int K, double alpha, 
     case 4:  // 256x128, 8 waves of 64x64: half the generated entries per flop (Cauchy kinds)
       return launch_right_t<KIND, 256, 128, 16, 4, 2, 1>(M, N, K, alpha, A, lda, g, r0, c0, beta,
                                                          C, ldc, s);
-    case 5: {  // 128x128, 8 waves of 32x64: two waves per SIMD with one workgroup per CU
-      // 74 KB of static LDS would let two workgroups share a CU while others idle
-      // (chosen from one tile per CU up); 16 KB of padding keeps it to one per CU:
-      // Cauchy N = 2048 0.41 -> 0.38 ms on MI355X, N = 4096 unchanged.
-      constexpr unsigned kPad = 16 * 1024;
-      static_assert(2 * (sizeof(double) * 2 * (RightLds<KIND, 128, 128, 16>::ASZ +
-                                               RightLds<KIND, 128, 128, 16>::HSZ)) +
-                            2 * kPad > 160 * 1024 ||
-                        KIND == KIND_TOEPLITZ || KIND == KIND_HANKEL,
-                    "padding must exclude a second workgroup");
-      return launch_right_t<KIND, 128, 128, 16, 4, 2, 1>(M, N, K, alpha, A, lda, g, r0, c0, beta,
-                                                         C, ldc, s, kPad);
+    case 5: {  // 128x128, 8 waves of 32x64, BK = 32: two waves per SIMD, one workgroup per CU
+      // Chosen from one tile per CU up.  The 147 KB of LDS of the Cauchy kinds keep a
+      // second workgroup off the CU (at 74 KB with BK = 16 two could share a CU while
+      // others idled): Cauchy N = 2048 0.41 -> 0.37-0.39 ms on MI355X.
+      static_assert(KIND == KIND_TOEPLITZ || KIND == KIND_HANKEL ||
+                        2 * sizeof(double) * 2 * (RightLds<KIND, 128, 128, 32>::ASZ +
+                                                  RightLds<KIND, 128, 128, 32>::HSZ) >
+                            160 * 1024,
+                    "one workgroup per CU");
+      return launch_right_t<KIND, 128, 128, 32, 4, 2, 1>(M, N, K, alpha, A, lda, g, r0, c0, beta,
+                                                         C, ldc, s);
     }
     default:  // small problems: 64x64 workgroup tile, 4 waves of 32x32
       return launch_right_t<KIND, 64, 64, 16, 2, 2>(M, N, K, alpha, A, lda, g, r0, c0, beta, C,
@@ -625,7 +621,8 @@ int choose_right_cfg(int kind, int M, int N) {
   // Cauchy kinds (measured on MI355X, N x N x N products): the 8-wave 256x128 tile
   // from 1024 128x128 tiles up (N = 4096: 2.56 ms against 2.77 for 128x128; +5% at
   // N = 16384), the 8-wave 128x128 tile from one 128x128 tile per CU (N = 2048:
-  // 0.41 ms against 0.45 for 4 waves and 0.49 for 64x64), 64x64 below
+  // 0.37-0.39 ms with BK = 32 against 0.41 with BK = 16, 0.45 for 4 waves, 0.49 for 64x64),
+  // 64x64 below
   if (!win) return big >= 1024 ? 4 : big >= 256 ? 5 : 2;
   // Toeplitz / Hankel: 64x256 from 256 128x128 tiles (Toeplitz N = 2048: 0.32 ms against
   // 0.35 for 64x64; best of 0, 2, 3 and two 8-wave 128x128 layouts at N = 2048, 4096, 8192)
