@@ -602,6 +602,10 @@ static hipError_t launch_right_kind(int cfg, int M, int N, int K, double alpha, 
       return launch_right_t<KIND, 128, 128, 32, 4, 2, 1>(M, N, K, alpha, A, lda, g, r0, c0, beta,
                                                          C, ldc, s);
     }
+    case 6:  // 256x128, 16 waves of 32x64 (plain Cauchy from 1024 tiles: 157 ms against 162 for
+             // config 4 at N = 16384 on MI355X; the eigen-Cauchy kind spills more and loses 10%)
+      return launch_right_t<KIND, 256, 128, 16, 8, 2, 1>(M, N, K, alpha, A, lda, g, r0, c0, beta,
+                                                         C, ldc, s);
     default:  // small problems: 64x64 workgroup tile, 4 waves of 32x32
       return launch_right_t<KIND, 64, 64, 16, 2, 2>(M, N, K, alpha, A, lda, g, r0, c0, beta, C,
                                                     ldc, s);
@@ -609,7 +613,7 @@ static hipError_t launch_right_kind(int cfg, int M, int N, int K, double alpha, 
 }
 
 // Picks a tile shape: the big tile when it still yields >= 2 workgroups per CU.
-// PSMMA_F64_CFG (0..5) forces one, for tuning sweeps.
+// PSMMA_F64_CFG (0..6) forces one, for tuning sweeps.
 int choose_right_cfg(int kind, int M, int N) {
   static int forced = [] {
     const char* e = getenv("PSMMA_F64_CFG");
@@ -623,7 +627,7 @@ int choose_right_cfg(int kind, int M, int N) {
   // N = 16384), the 8-wave 128x128 tile from one 128x128 tile per CU (N = 2048:
   // 0.37-0.39 ms with BK = 32 against 0.41 with BK = 16, 0.45 for 4 waves, 0.49 for 64x64),
   // 64x64 below
-  if (!win) return big >= 1024 ? 4 : big >= 256 ? 5 : 2;
+  if (!win) return big >= 1024 ? (kind == KIND_CAUCHY ? 6 : 4) : big >= 256 ? 5 : 2;
   // Toeplitz / Hankel: 64x256 from 256 128x128 tiles (Toeplitz N = 2048: 0.32 ms against
   // 0.35 for 64x64; best of 0, 2, 3 and two 8-wave 128x128 layouts at N = 2048, 4096, 8192)
   return big >= 256 ? 3 : 2;

@@ -74,10 +74,12 @@ ORA_KIND = dict(cauchy=0, toeplitz=1, hankel=2, cauchyeig=3)
 
 @pytest.mark.parametrize("kind", KINDS)
 # (2050, 2100, 2013): 289 tiles of 128 x 128 select the large tiles (64 x 256 with BK = 32 for
-# Toeplitz / Hankel, the padded 8-wave 128 x 128 for the Cauchy kinds) with ragged M, N and K
-# (K mod 32 = 29, K mod 4 = 1)
+# Toeplitz / Hankel, the 8-wave 128 x 128 with BK = 32 for the Cauchy kinds) with ragged M, N and K
+# (K mod 32 = 29, K mod 4 = 1); (4100, 4200, 1001): 1089 tiles select the 256 x 128 tiles of the
+# Cauchy kinds (16 waves for plain Cauchy, 8 for eigen-Cauchy)
 @pytest.mark.parametrize("m,n,k", [(1, 1, 1), (63, 63, 63), (64, 64, 64), (257, 129, 300),
-                                   (130, 520, 77), (1024, 1024, 1024), (2050, 2100, 2013)])
+                                   (130, 520, 77), (1024, 1024, 1024), (2050, 2100, 2013),
+                                   (4100, 4200, 1001)])
 def test_right_fp64_parity(gpu, ora, kind, m, n, k):
     import torch
 
