@@ -234,7 +234,6 @@ __device__ __forceinline__ void right_tile(int M, int N, int K, double alpha,
           acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[i], bf[j], acc[i][j], 0, 0, 0);
     }
   };
-  constexpr int KH = (BK / 2) & ~3;
   for (int kt = 0; kt < nkt; ++kt) {
     const int cur = kt & 1;
     const bool more = (kt + 1) < nkt;
@@ -244,14 +243,13 @@ __device__ __forceinline__ void right_tile(int M, int N, int K, double alpha,
     }
     const double* as = As_ + cur * ASZ;
     const double* hs = Hs_ + cur * HSZ;
-    mfma_rows(as, hs, std::integral_constant<int, 0>{}, std::integral_constant<int, KH>{});
-    // Window kinds: a scheduling fence between the two halves and a branch-free
-    // store of the next slab (after the last slab it rewrites the idle buffer,
-    // whose last reads finished before the previous barrier, with values nobody
-    // reads): +1.5% at N = K = 8192 on MI355X, measured.  The Cauchy kinds keep
-    // the plain order (no gain measured there).
-    if constexpr (WIN) __builtin_amdgcn_sched_barrier(0);
-    mfma_rows(as, hs, std::integral_constant<int, KH>{}, std::integral_constant<int, BK>{});
+    mfma_rows(as, hs, std::integral_constant<int, 0>{}, std::integral_constant<int, BK>{});
+    // Window kinds: the next slab is stored branch-free (after the last slab it
+    // rewrites the idle buffer, whose last reads finished before the previous
+    // barrier, with values nobody reads) and a scheduling fence keeps the stores
+    // ahead of the barrier: with BK = 32, Toeplitz N = 8192 70.5 TF/s on MI355X
+    // (70.1 with a second fence between two halves of the slab's MFMAs, which
+    // helped at BK = 16).  The Cauchy kinds keep the plain order (no gain there).
     if (WIN || more) {
       finish_h((kt + 1) * BK);
       store(cur ^ 1);
