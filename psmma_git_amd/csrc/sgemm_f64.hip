@@ -525,8 +525,8 @@ template <int KIND>
 static hipError_t launch_grouped_kind(int cfg, const BatchProblem* dprobs, const int* dtbeg,
                                       int nprob, int ntiles, int nwg, hipStream_t s) {
   if (cfg == 1) {
-    if constexpr (KIND == KIND_TOEPLITZ || KIND == KIND_HANKEL)
-      hipLaunchKernelGGL((sgemm_f64_grouped_kernel<KIND, 64, 256, 16, 1, 4, 2>), dim3(nwg),
+    if constexpr (KIND == KIND_TOEPLITZ || KIND == KIND_HANKEL)  // BK = 32 as for one problem
+      hipLaunchKernelGGL((sgemm_f64_grouped_kernel<KIND, 64, 256, 32, 1, 4, 2>), dim3(nwg),
                          dim3(256), 0, s, dprobs, dtbeg, nprob, ntiles);
     else
       hipLaunchKernelGGL((sgemm_f64_grouped_kernel<KIND, 128, 128, 16, 2, 2, 2>), dim3(nwg),
