@@ -45,10 +45,23 @@ extern "C" {
  *                       accumulation over exactly representable K-chunks and
  *                       FP64 recombination.
  * The pole differences of the Cauchy kinds are always formed in FP64.
+ *
+ * slices = 0 asks for the default of the form: the fewest moduli that keep 45
+ * bits per operand at the call's K (int8), or 6 slices (bf16).  The int8 form
+ * accumulates products of residues |r| <= 128 in int32, exact for
+ * K <= PSMMA_SPLIT_I8_KMAX; a longer contraction is rejected as argument K.
+ *
+ * Split-precision calls (and the compressed products of section 6) keep their
+ * scratch memory in a grow-only workspace per (device, stream): calls on
+ * different streams never share it, and calls on one stream are ordered by it.
+ * The first call of a given size on a stream allocates (device-synchronising
+ * when it grows), so such calls are not capturable until one of that size has
+ * run on that stream.
  */
 #define PSMMA_PREC_FP64 0
 #define PSMMA_PREC_SPLIT_I8 1
 #define PSMMA_PREC_SPLIT_BF16 2
+#define PSMMA_SPLIT_I8_KMAX 131071
 
 /* Kinds of structured operand (psmma_structured_dev, batched API). */
 #define PSMMA_KIND_CAUCHY 0

@@ -148,7 +148,7 @@ def last_error() -> str:
     return lib().psmma_last_error().decode()
 
 
-def set_precision(prec: int, slices: int = 7) -> None:
+def set_precision(prec: int, slices: int = 0) -> None:
     st = lib().psmma_set_precision(int(prec), int(slices))
     if st != 0:
         raise PsmmaError(f"psmma_set_precision({prec}, {slices}) -> {st}")
@@ -247,11 +247,46 @@ def _check(st: int, what: str):
         raise PsmmaError(f"{what} returned {st}: {last_error()}")
 
 
-def _colmajor(t):
-    """(pointer, rows, cols, ld) of a column-major 2-D torch tensor view."""
-    if t.dim() != 2 or t.stride(0) != 1:
-        raise ValueError("expected a column-major 2-D tensor (stride(0) == 1)")
-    return t.data_ptr(), t.shape[0], t.shape[1], max(1, t.stride(1))
+def _on_device(t, name, dtype=None):
+    """Checks that t is a torch tensor on a HIP device with the given dtype (float64 default)."""
+    import torch
+    dtype = dtype or torch.float64
+    if not isinstance(t, torch.Tensor):
+        raise TypeError(f"{name}: expected a torch tensor, got {type(t).__name__}")
+    if t.dtype != dtype:
+        raise ValueError(f"{name}: dtype {t.dtype}, expected {dtype}")
+    if t.device.type != "cuda":
+        raise ValueError(f"{name}: expected a device (cuda/HIP) tensor, got one on {t.device}")
+    return t
+
+
+def _colmajor(t, name="matrix", dtype=None):
+    """(pointer, rows, cols, ld) of a column-major 2-D device tensor view."""
+    _on_device(t, name, dtype)
+    if t.dim() != 2 or (t.stride(0) != 1 and t.shape[0] > 1):
+        raise ValueError(f"{name}: expected a column-major 2-D tensor (stride(0) == 1)")
+    m, n = t.shape
+    if n > 1 and t.stride(1) < m:
+        raise ValueError(f"{name}: column stride {t.stride(1)} is below the row count {m}")
+    return t.data_ptr(), m, n, max(1, m, t.stride(1) if n > 1 else 1)
+
+
+def _out(c, m, n, name="c", dtype=None):
+    """Output view: column-major with at least m rows and n columns."""
+    pc, mc, nc, ldc = _colmajor(c, name, dtype)
+    if mc < m or nc < n:
+        raise ValueError(f"{name}: {mc} x {nc} is smaller than the {m} x {n} product")
+    return pc, ldc
+
+
+def _vec(t, name, need=0, dtype=None):
+    """Pointer of a contiguous 1-D device vector with at least `need` entries."""
+    _on_device(t, name, dtype)
+    if t.dim() != 1 or not t.is_contiguous():
+        raise ValueError(f"{name}: expected a contiguous 1-D tensor")
+    if t.numel() < need:
+        raise ValueError(f"{name}: {t.numel()} entries, the block needs {need}")
+    return t.data_ptr()
 
 
 def _stream_ptr(stream):
@@ -267,45 +302,57 @@ def colmajor_empty(m: int, n: int, dtype=None, device="cuda"):
     return torch.empty((n, m), dtype=dtype or torch.float64, device=device).t()
 
 
-def toeplitz_dev(alpha, a, g, korder, n, c, beta=0.0, r0=0, c0=0, prec=PREC_FP64, slices=7,
+# slices = 0: the numeric form's default (int8: the fewest moduli that keep 45 bits per
+# operand at the call's K; bf16: 6 slices).
+
+
+def toeplitz_dev(alpha, a, g, korder, n, c, beta=0.0, r0=0, c0=0, prec=PREC_FP64, slices=0,
                  stream=None):
-    pa, m, k, lda = _colmajor(a)
-    pc, mc, nc, ldc = _colmajor(c)
-    _check(lib().psmma_toeplitz_dev(m, n, k, alpha, pa, lda, g.data_ptr(), g.numel(), korder, r0,
+    """C(m x n) = alpha * A(m x k) * T(r0:r0+k, c0:c0+n) + beta * C, T(r,c) = g[korder-1+c-r]."""
+    pa, m, k, lda = _colmajor(a, "a")
+    pc, ldc = _out(c, m, n)
+    _check(lib().psmma_toeplitz_dev(m, n, k, alpha, pa, lda, _vec(g, "g"), g.numel(), korder, r0,
                                     c0, beta, pc, ldc, prec, slices, _stream_ptr(stream)),
            "psmma_toeplitz_dev")
 
 
-def hankel_dev(alpha, a, g, n, c, beta=0.0, r0=0, c0=0, prec=PREC_FP64, slices=7, stream=None):
-    pa, m, k, lda = _colmajor(a)
-    pc, mc, nc, ldc = _colmajor(c)
-    _check(lib().psmma_hankel_dev(m, n, k, alpha, pa, lda, g.data_ptr(), g.numel(), r0, c0, beta,
+def hankel_dev(alpha, a, g, n, c, beta=0.0, r0=0, c0=0, prec=PREC_FP64, slices=0, stream=None):
+    """C(m x n) = alpha * A(m x k) * H(r0:r0+k, c0:c0+n) + beta * C, H(r,c) = g[r+c]."""
+    pa, m, k, lda = _colmajor(a, "a")
+    pc, ldc = _out(c, m, n)
+    _check(lib().psmma_hankel_dev(m, n, k, alpha, pa, lda, _vec(g, "g"), g.numel(), r0, c0, beta,
                                   pc, ldc, prec, slices, _stream_ptr(stream)), "psmma_hankel_dev")
 
 
-def cauchy_dev(alpha, a, d, f, u, v, n, c, beta=0.0, r0=0, c0=0, prec=PREC_FP64, slices=7,
+def cauchy_dev(alpha, a, d, f, u, v, n, c, beta=0.0, r0=0, c0=0, prec=PREC_FP64, slices=0,
                stream=None):
-    pa, m, k, lda = _colmajor(a)
-    pc, mc, nc, ldc = _colmajor(c)
-    _check(lib().psmma_cauchy_dev(m, n, k, alpha, pa, lda, d.data_ptr(), f.data_ptr(),
-                                  u.data_ptr(), v.data_ptr(), r0, c0, beta, pc, ldc, prec, slices,
-                                  _stream_ptr(stream)), "psmma_cauchy_dev")
+    """C(m x n) = alpha * A(m x k) * H(r0:r0+k, c0:c0+n) + beta * C, H(r,c) = u_r v_c/(d_r-f_c)."""
+    pa, m, k, lda = _colmajor(a, "a")
+    pc, ldc = _out(c, m, n)
+    rows, cols = (r0 + k, c0 + n) if m and n and k else (0, 0)
+    _check(lib().psmma_cauchy_dev(m, n, k, alpha, pa, lda, _vec(d, "d", rows), _vec(f, "f", cols),
+                                  _vec(u, "u", rows), _vec(v, "v", cols), r0, c0, beta, pc, ldc,
+                                  prec, slices, _stream_ptr(stream)), "psmma_cauchy_dev")
 
 
 def cauchyeig_dev(alpha, a, fpole, u, v, difl, difr, n, c, beta=0.0, r0=0, c0=0, prec=PREC_FP64,
-                  slices=7, stream=None):
-    pa, m, k, lda = _colmajor(a)
-    pc, mc, nc, ldc = _colmajor(c)
-    _check(lib().psmma_cauchyeig_dev(m, n, k, alpha, pa, lda, fpole.data_ptr(), u.data_ptr(),
-                                     v.data_ptr(), difl.data_ptr(), difr.data_ptr(), fpole.numel(),
-                                     r0, c0, beta, pc, ldc, prec, slices, _stream_ptr(stream)),
-           "psmma_cauchyeig_dev")
+                  slices=0, stream=None):
+    """C(m x n) = alpha * A(m x k) * H(r0:r0+k, c0:c0+n) + beta * C, H the eigen-Cauchy matrix
+    of order fpole.numel()."""
+    pa, m, k, lda = _colmajor(a, "a")
+    pc, ldc = _out(c, m, n)
+    nglob = fpole.numel() if hasattr(fpole, "numel") else 0
+    ptrs = [_vec(t, nm, nglob) for t, nm in
+            ((fpole, "fpole"), (u, "u"), (v, "v"), (difl, "difl"), (difr, "difr"))]
+    _check(lib().psmma_cauchyeig_dev(m, n, k, alpha, pa, lda, *ptrs, nglob, r0, c0, beta, pc, ldc,
+                                     prec, slices, _stream_ptr(stream)), "psmma_cauchyeig_dev")
 
 
 def dft_dev(alpha, b, nd, m, c, beta=0.0, r0=0, c0=0, prec=PREC_FP64, stream=None):
     """C(m x n) = alpha * F(r0:r0+m, c0:c0+k) * B + beta*C; b, c complex128 column-major."""
-    pb, k, n, ldb = _colmajor(b)
-    pc, mc, nc, ldc = _colmajor(c)
+    import torch
+    pb, k, n, ldb = _colmajor(b, "b", torch.complex128)
+    pc, ldc = _out(c, m, n, "c", torch.complex128)
     _check(lib().pzmdft_dev(m, n, k, nd, alpha, pb, ldb, r0, c0, beta, pc, ldc, prec,
                             _stream_ptr(stream)), "pzmdft_dev")
 
@@ -316,6 +363,8 @@ def secular_dev(d, z, rho=1.0, loewner=True, stream=None):
     loewner is true, else z itself; v the column normalisers for that u."""
     import torch
     n = d.numel()
+    _vec(d, "d")
+    _vec(z, "z", n)
     difl = torch.empty_like(d)
     difr = torch.empty_like(d)
     zhat = torch.empty_like(d) if loewner else None
@@ -339,19 +388,17 @@ def cauchyeig_blr_dev(alpha, a, fpole, u, v, difl, difr, c, leaf=256, tol=1e-12,
     compressor given (BLR_RRQR / BLR_RRLU: psmma_cauchyeig_blr_ex_dev).  a, c:
     column-major (m x n) device views.  Returns the stats dict (blocks, lowrank, dense,
     zero, rank_sum)."""
-    pa, m, n, lda = _colmajor(a)
-    pc, mc, nc, ldc = _colmajor(c)
+    pa, m, n, lda = _colmajor(a, "a")
+    pc, ldc = _out(c, m, n)
+    gp = [_vec(t, nm, n) for t, nm in
+          ((fpole, "fpole"), (u, "u"), (v, "v"), (difl, "difl"), (difr, "difr"))]
     st = (ctypes.c_longlong * 5)()
     if compressor is None:
-        _check(lib().psmma_cauchyeig_blr_dev(m, n, alpha, pa, lda, fpole.data_ptr(),
-                                             u.data_ptr(), v.data_ptr(), difl.data_ptr(),
-                                             difr.data_ptr(), leaf, float(tol), beta, pc, ldc,
-                                             st, _stream_ptr(stream)),
+        _check(lib().psmma_cauchyeig_blr_dev(m, n, alpha, pa, lda, *gp, leaf, float(tol), beta, pc,
+                                             ldc, st, _stream_ptr(stream)),
                "psmma_cauchyeig_blr_dev")
     else:
-        _check(lib().psmma_cauchyeig_blr_ex_dev(m, n, alpha, pa, lda, fpole.data_ptr(),
-                                                u.data_ptr(), v.data_ptr(), difl.data_ptr(),
-                                                difr.data_ptr(), leaf, float(tol),
+        _check(lib().psmma_cauchyeig_blr_ex_dev(m, n, alpha, pa, lda, *gp, leaf, float(tol),
                                                 int(compressor), beta, pc, ldc, st,
                                                 _stream_ptr(stream)),
                "psmma_cauchyeig_blr_ex_dev")
@@ -361,17 +408,26 @@ def cauchyeig_blr_dev(alpha, a, fpole, u, v, difl, difr, c, leaf=256, tol=1e-12,
 RRLU_TOL = float(np.float32(1.0e-17))  # the reference's default-real TOL
 
 
+def _index_lists(pl, pu, nblk):
+    """pl (nblk, m) and pu (nblk, n): contiguous int32 device tensors of 1-based indices."""
+    import torch
+    for t, nm in ((pl, "pl"), (pu, "pu")):
+        _on_device(t, nm, torch.int32)
+        if t.dim() != 2 or t.shape[0] != nblk or not t.is_contiguous():
+            raise ValueError(f"{nm}: expected a contiguous ({nblk}, width) int32 tensor")
+
+
 def cauchy_blr_dev(alpha, a, d, f, u, v, c, leaf=256, tol=1e-12, beta=0.0, stream=None,
                    compressor=1):
     """X = alpha * A * C + beta * X, C(i,j) = u_i v_j / (d_i - f_j), with the off-diagonal leaf
     blocks compressed by the RRLU (BLR_RRLU, default) or the RRQR (BLR_RRQR).  a, c:
     column-major (m x n) device views; d, f, u, v: float64 device vectors of length n.
     Returns the stats dict (blocks, lowrank, dense, zero, rank_sum)."""
-    pa, m, n, lda = _colmajor(a)
-    pc, mc, nc, ldc = _colmajor(c)
+    pa, m, n, lda = _colmajor(a, "a")
+    pc, ldc = _out(c, m, n)
+    gp = [_vec(t, nm, n) for t, nm in ((d, "d"), (f, "f"), (u, "u"), (v, "v"))]
     st = (ctypes.c_longlong * 5)()
-    _check(lib().psmma_cauchy_blr_dev(m, n, alpha, pa, lda, d.data_ptr(), f.data_ptr(),
-                                      u.data_ptr(), v.data_ptr(), leaf, float(tol),
+    _check(lib().psmma_cauchy_blr_dev(m, n, alpha, pa, lda, *gp, leaf, float(tol),
                                       int(compressor), beta, pc, ldc, st, _stream_ptr(stream)),
            "psmma_cauchy_blr_dev")
     return dict(zip(("blocks", "lowrank", "dense", "zero", "rank_sum"), list(st)))
@@ -389,6 +445,10 @@ def rrlu_cauchyeig_batched_dev(pl, pu, fpole, u, v, difl, difr, tol=RRLU_TOL, km
     import torch
     nblk, m = pl.shape
     n = pu.shape[1]
+    _index_lists(pl, pu, nblk)
+    korder = fpole.numel() if hasattr(fpole, "numel") else 0
+    for t, nm in ((fpole, "fpole"), (u, "u"), (v, "v"), (difl, "difl"), (difr, "difr")):
+        _vec(t, nm, korder)
     kq = max(1, min(kmax, m, n))
     dev = pl.device
     if out is not None:
@@ -418,6 +478,8 @@ def rrlu_cauchy_batched_dev(pl, pu, d, f, u, v, tol=RRLU_TOL, kmax=32, stream=No
     import torch
     nblk, m = pl.shape
     n = pu.shape[1]
+    _index_lists(pl, pu, nblk)
+    _vec(d, "d"), _vec(u, "u", d.numel()), _vec(f, "f"), _vec(v, "v", f.numel())
     kq = max(1, min(kmax, m, n))
     dev = pl.device
     if out is not None:
@@ -442,6 +504,9 @@ def rrqr_batched_dev(blocks, tol, kmax=RRQR_KMAX, stream=None):
     column-major views q[b].T and rpt[b].T, Q (m x rank) and R P^T (rank x n) in their
     leading columns / rows, kq = min(kmax, m, n)."""
     import torch
+    _on_device(blocks, "blocks")
+    if blocks.dim() != 3:
+        raise ValueError("blocks: expected a (nblk, n, m) tensor")
     nblk, n, m = blocks.shape
     if not blocks.is_contiguous():
         raise ValueError("blocks must be contiguous (nblk, n, m)")

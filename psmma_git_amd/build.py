@@ -40,24 +40,6 @@ def _compile(src: Path) -> Path:
     return obj
 
 
-FORTRAN_DIR = PKG / "fortran"
-FORTRAN_DRIVER = FORTRAN_DIR / "psmma_fortran_driver"
-FLANG = "/opt/rocm/llvm/bin/amdflang"
-
-
-def build_fortran_driver() -> Path | None:
-    """Compile the Fortran drop-in driver against the library (needs amdflang)."""
-    if not os.path.exists(FLANG):
-        return None
-    cmd = [FLANG, "-O2", "-o", str(FORTRAN_DRIVER), str(FORTRAN_DIR / "psmma_fortran_driver.f90"),
-           f"-L{PKG}", "-lpsmma_hip", "-Wl,-rpath,$ORIGIN/..", "-L/opt/rocm/lib", "-lamdhip64"]
-    r = subprocess.run(cmd, capture_output=True, text=True, cwd=str(BUILD))
-    if r.returncode != 0:
-        sys.stderr.write(r.stdout + r.stderr)
-        raise RuntimeError("amdflang failed on psmma_fortran_driver.f90")
-    return FORTRAN_DRIVER
-
-
 def build(verbose: bool = False) -> Path:
     BUILD.mkdir(exist_ok=True)
     srcs = sources()
@@ -71,9 +53,8 @@ def build(verbose: bool = False) -> Path:
         sys.stderr.write(r.stdout + r.stderr)
         raise RuntimeError("link of libpsmma_hip.so failed")
     os.replace(tmp, LIB)
-    drv = build_fortran_driver()
     if verbose:
-        print(f"built {LIB}" + (f" and {drv}" if drv else ""))
+        print(f"built {LIB}")
     return LIB
 
 

@@ -33,6 +33,7 @@
 
 #include "../../include/psmma.h"
 #include "psmma_common.h"
+#include "workspace.h"
 
 namespace psmma {
 hipError_t launch_right_f64(int kind, int cfg, int M, int N, int K, double alpha, const double* A,
@@ -121,33 +122,8 @@ __global__ __launch_bounds__(256) void blr_diag_kernel(GenPtrs g, int leaf, doub
     H[((size_t)J * leaf + j) * leaf + i] = gen_entry<KIND>(g, J * leaf + i, J * leaf + j);
 }
 
-struct DevWs {
-  void* p = nullptr;
-  size_t bytes = 0;
-};
 std::mutex g_mu;
-std::unordered_map<int, DevWs> g_ws[2];
 std::unordered_map<int, rocblas_handle> g_blas;
-
-// Grow-only per-device workspaces: part 0 holds the RRQR outputs, part 1 the
-// rank-dependent GEMM operands.  Calls on one device share them.
-void* workspace(int part, size_t bytes) {
-  int dev = 0;
-  hipGetDevice(&dev);
-  std::lock_guard<std::mutex> lk(g_mu);
-  DevWs& w = g_ws[part][dev];
-  if (w.bytes < bytes) {
-    if (w.p) {
-      hipDeviceSynchronize();
-      hipFree(w.p);
-    }
-    w.p = nullptr;
-    w.bytes = 0;
-    if (hipMalloc(&w.p, bytes) != hipSuccess) return nullptr;
-    w.bytes = bytes;
-  }
-  return w.p;
-}
 
 rocblas_handle blas() {
   int dev = 0;
@@ -189,7 +165,8 @@ int blr_product(int kind, const GenPtrs& g, int m, int nglob, double alpha, cons
   size_t off_r = align_up(off_q + qbytes);
   size_t off_cache = align_up(off_r + qbytes);
   size_t fixed = align_up(off_cache + cbytes);
-  char* ws = (char*)workspace(0, fixed);
+  std::unique_lock<std::mutex> lk_fixed, lk_gemm;
+  char* ws = (char*)ws_lease(WS_BLR_FIXED, fixed, s, lk_fixed);
   if (!ws) return 1002;
   std::vector<int> hr0(nblk), hc0(nblk);
   for (int I = 0, b = 0; I < nb; ++I)
@@ -310,7 +287,7 @@ int blr_product(int kind, const GenPtrs& g, int m, int nglob, double alpha, cons
   const size_t off_tp = align_up(off_t + sizeof(double) * (size_t)m * S);
   const size_t off_h = align_up(off_tp + sizeof(double) * (size_t)m * SR);
   const size_t total = align_up(off_h + sizeof(double) * (size_t)nb * leaf * leaf);
-  char* ws2 = (char*)workspace(1, total);
+  char* ws2 = (char*)ws_lease(WS_BLR_GEMM, total, s, lk_gemm);
   if (!ws2) return 1002;
   PackEntry* dtab = (PackEntry*)(ws2 + off_tab);
   double* U = (double*)(ws2 + off_u);

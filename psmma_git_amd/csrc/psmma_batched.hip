@@ -4,16 +4,17 @@
 
 #include <algorithm>
 #include <cstdio>
+#include <mutex>
 #include <vector>
 
 #include "../../include/psmma.h"
 #include "psmma_common.h"
+#include "workspace.h"
 
 namespace psmma {
 hipError_t launch_grouped_f64(int kind, int cfg, const BatchProblem* dprobs, const int* dtbeg,
                               int nprob, int ntiles, int nworkgroups, hipStream_t s);
 void grouped_tile_shape(int kind, int cfg, int* bm, int* bn);
-void* split_workspace(size_t bytes);
 void* gsplit_create(const BatchProblem* probs, int nprob, int t, size_t* ws_bytes);
 int gsplit_bind(void* h, void* ws);
 int gsplit_enqueue(void* h, hipStream_t s, int* nodes);
@@ -194,6 +195,18 @@ int check_prec(int prec, int slices) {
   return 0;
 }
 
+// The int8 form's int32 accumulators are exact for K <= PSMMA_SPLIT_I8_KMAX only.
+int check_split_k(int prec, const std::vector<BatchProblem>& in) {
+  if (prec != PSMMA_PREC_SPLIT_I8) return 0;
+  for (size_t i = 0; i < in.size(); ++i)
+    if (in[i].k > PSMMA_SPLIT_I8_KMAX) {
+      fprintf(stderr, "psmma_batched: problem %zu: k = %d exceeds the int8 split limit %d\n", i,
+              in[i].k, PSMMA_SPLIT_I8_KMAX);
+      return -1000 - (int)i;
+    }
+  return 0;
+}
+
 }  // namespace
 
 extern "C" int psmma_batched_dev(int nprob, const psmma_problem* probs, int prec, int slices,
@@ -202,12 +215,14 @@ extern "C" int psmma_batched_dev(int nprob, const psmma_problem* probs, int prec
   if (st) return st;
   std::vector<BatchProblem> in;
   if ((st = collect(nprob, probs, in))) return st;
+  if ((st = check_split_k(prec, in))) return st;
   hipStream_t s = (hipStream_t)stream;
   if (prec != PSMMA_PREC_FP64) {
     int t = slices ? slices : gsplit_moduli(in.data(), nprob, kSplitBits);
     size_t bytes = 0;
     void* gs = gsplit_create(in.data(), nprob, t, &bytes);
-    void* ws = bytes ? split_workspace(bytes) : nullptr;
+    std::unique_lock<std::mutex> lk;
+    void* ws = bytes ? ws_lease(WS_SPLIT, bytes, s, lk) : nullptr;
     st = (bytes && !ws) ? 1002 : gsplit_bind(gs, ws);
     if (st == 0) st = gsplit_enqueue(gs, s, nullptr);
     hipStreamSynchronize(s);  // the table is released below
@@ -233,6 +248,7 @@ extern "C" int psmma_batched_plan_create(int nprob, const psmma_problem* probs, 
   if (st) return st;
   std::vector<BatchProblem> in;
   if ((st = collect(nprob, probs, in))) return st;
+  if ((st = check_split_k(prec, in))) return st;
   Plan* P = new Plan;
   P->prec = prec;
   P->slices = slices;

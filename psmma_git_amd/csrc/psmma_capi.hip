@@ -33,7 +33,7 @@ using namespace psmma;
 static thread_local std::string g_err;
 static thread_local int g_status = 0;
 static int g_prec = PSMMA_PREC_FP64;
-static int g_slices = 7;
+static int g_slices = 0;  // 0: chosen per call from K (psmma.h)
 
 static void set_err(const char* fmt, ...) {
   char buf[512];
@@ -66,10 +66,10 @@ static int hip_fail(hipError_t e, const char* where) {
 // check that; otherwise (the weak reference is null) the grid is 1 x 1.
 extern "C" void blacs_gridinfo_(const int*, int*, int*, int*, int*) __attribute__((weak));
 
-static bool grid_ok(const int* desc, const char* name) {
-  if (blacs_gridinfo_ == nullptr || desc == nullptr) return true;
+static bool grid_ok_ctxt(const int* ictxt, const char* name) {
+  if (blacs_gridinfo_ == nullptr || ictxt == nullptr) return true;
   int nprow = 1, npcol = 1, myrow = 0, mycol = 0;
-  blacs_gridinfo_(&desc[1], &nprow, &npcol, &myrow, &mycol);
+  blacs_gridinfo_(ictxt, &nprow, &npcol, &myrow, &mycol);
   if (nprow == 1 && npcol == 1) return true;
   set_err("%s: process grid %d x %d; this library computes one GPU's share and needs a 1 x 1 grid",
           name, nprow, npcol);
@@ -77,6 +77,15 @@ static bool grid_ok(const int* desc, const char* name) {
   g_status = -2;
   return false;
 }
+
+static bool grid_ok(const int* desc, const char* name) {
+  return desc == nullptr || grid_ok_ctxt(&desc[1], name);
+}
+
+// PZMDFTL1 / PZMDFTL take no descriptor: they read the BLACS context from
+// COMMON /CONTEXT/ ICTXT (SRC/pzmdftL1.f90:178), the global symbol context_ of a
+// Fortran caller that declares the block (weak: null when nobody does).
+extern "C" int context_ __attribute__((weak));
 
 // ---------------------------------------------------------------------------
 // Device staging buffers for the host-pointer entry points: grow-only, one set
@@ -105,13 +114,14 @@ static void* stage_buf(int slot, size_t bytes) {
 
 // FP64 twiddle tables, one per (device, Nd), built on the host in long double
 // and rounded once: tw[e] = (cos(2 pi e/Nd), -sin(2 pi e/Nd)).
+static std::mutex g_tw_mu;
 static std::unordered_map<long long, double2*> g_tw;
 
 static const double2* twiddles(int nd, hipStream_t s) {
   int dev = 0;
   hipGetDevice(&dev);
   long long key = ((long long)dev << 32) | (unsigned)nd;
-  std::lock_guard<std::mutex> lk(g_mu);
+  std::lock_guard<std::mutex> lk(g_tw_mu);
   auto it = g_tw.find(key);
   if (it != g_tw.end()) return it->second;
   std::vector<double2> h(nd);
@@ -147,37 +157,54 @@ static int run_right(int kind, int m, int n, int k, double alpha, const double* 
   return launch_right_split(kind, prec, slices, m, n, k, alpha, a, lda, g, r0, c0, beta, c, ldc, s);
 }
 
-static int check_common(int m, int n, int k, const double* a, int lda, const double* c, int ldc,
-                        int prec, int slices) {
-  if (m < 0) return -1;
-  if (n < 0) return -2;
-  if (k < 0) return -3;
-  if (lda < (m > 1 ? m : 1)) return -6;
-  if (ldc < (m > 1 ? m : 1)) return -14;
-  if (prec != PSMMA_PREC_FP64 && prec != PSMMA_PREC_SPLIT_I8 && prec != PSMMA_PREC_SPLIT_BF16)
-    return -15;
-  if (prec != PSMMA_PREC_FP64 && (slices < 1 || slices > 16)) return -16;
-  if (m > 0 && n > 0 && (c == nullptr || (k > 0 && a == nullptr))) return -5;
+static int check_common(const char* name, int m, int n, int k, const double* a, int lda,
+                        const double* c, int ldc, int prec, int slices) {
+  int st = 0;
+  if (m < 0) st = -1;
+  else if (n < 0) st = -2;
+  else if (k < 0) st = -3;
+  else if (lda < (m > 1 ? m : 1)) st = -6;
+  else if (ldc < (m > 1 ? m : 1)) st = -14;
+  else if (prec != PSMMA_PREC_FP64 && prec != PSMMA_PREC_SPLIT_I8 && prec != PSMMA_PREC_SPLIT_BF16)
+    st = -15;
+  else if (prec != PSMMA_PREC_FP64 && (slices < 0 || slices > 16)) st = -16;
+  else if (m > 0 && n > 0 && (c == nullptr || (k > 0 && a == nullptr))) st = -5;
+  if (st) {
+    set_err("%s: invalid argument %d", name, -st);
+    return st;
+  }
+  if (prec == PSMMA_PREC_SPLIT_I8 && k > PSMMA_SPLIT_I8_KMAX) {
+    set_err("%s: K = %d exceeds the int8 split form's exact-accumulation limit %d", name, k,
+            PSMMA_SPLIT_I8_KMAX);
+    return -3;
+  }
+  return 0;
+}
+
+// Toeplitz block arguments shared by psmma_toeplitz_dev and its allocation-free variant.
+static int check_toeplitz(const char* name, int m, int n, int k, const double* a, int lda,
+                          const double* g, int lg, int korder, int r0, int c0, const double* c,
+                          int ldc, int prec, int slices) {
+  int st = check_common(name, m, n, k, a, lda, c, ldc, prec, slices);
+  if (st) return st;
+  if (r0 < 0 || c0 < 0) return -10;
+  if (m > 0 && n > 0 && k > 0) {
+    long long lo = (long long)korder - 1 + c0 - (r0 + k - 1);
+    long long hi = (long long)korder - 1 + (c0 + n - 1) - r0;
+    if (g == nullptr || lo < 0 || hi >= lg) {
+      set_err("%s: generator index range [%lld,%lld] outside [0,%d)", name, lo, hi, lg);
+      return -7;
+    }
+  }
   return 0;
 }
 
 extern "C" int psmma_toeplitz_dev(int m, int n, int k, double alpha, const double* a, int lda,
                                   const double* g, int lg, int korder, int r0, int c0, double beta,
                                   double* c, int ldc, int prec, int slices, void* stream) {
-  int st = check_common(m, n, k, a, lda, c, ldc, prec, slices);
-  if (st) {
-    set_err("psmma_toeplitz_dev: invalid argument %d", -st);
-    return st;
-  }
-  if (r0 < 0 || c0 < 0) return -10;
-  if (m > 0 && n > 0 && k > 0) {
-    long long lo = (long long)korder - 1 + c0 - (r0 + k - 1);
-    long long hi = (long long)korder - 1 + (c0 + n - 1) - r0;
-    if (g == nullptr || lo < 0 || hi >= lg) {
-      set_err("psmma_toeplitz_dev: generator index range [%lld,%lld] outside [0,%d)", lo, hi, lg);
-      return -7;
-    }
-  }
+  int st = check_toeplitz("psmma_toeplitz_dev", m, n, k, a, lda, g, lg, korder, r0, c0, c, ldc,
+                          prec, slices);
+  if (st) return st;
   GenPtrs gp{g, nullptr, nullptr, nullptr, nullptr, korder, r0 + k, c0 + n};
   return run_right(KIND_TOEPLITZ, m, n, k, alpha, a, lda, gp, r0, c0, beta, c, ldc, prec, slices,
                    (hipStream_t)stream);
@@ -186,11 +213,8 @@ extern "C" int psmma_toeplitz_dev(int m, int n, int k, double alpha, const doubl
 extern "C" int psmma_hankel_dev(int m, int n, int k, double alpha, const double* a, int lda,
                                 const double* g, int lg, int r0, int c0, double beta, double* c,
                                 int ldc, int prec, int slices, void* stream) {
-  int st = check_common(m, n, k, a, lda, c, ldc, prec, slices);
-  if (st) {
-    set_err("psmma_hankel_dev: invalid argument %d", -st);
-    return st;
-  }
+  int st = check_common("psmma_hankel_dev", m, n, k, a, lda, c, ldc, prec, slices);
+  if (st) return st;
   if (r0 < 0 || c0 < 0) return -9;
   if (m > 0 && n > 0 && k > 0) {
     long long hi = (long long)(r0 + k - 1) + (c0 + n - 1);
@@ -208,11 +232,8 @@ extern "C" int psmma_cauchy_dev(int m, int n, int k, double alpha, const double*
                                 const double* d, const double* f, const double* u, const double* v,
                                 int r0, int c0, double beta, double* c, int ldc, int prec,
                                 int slices, void* stream) {
-  int st = check_common(m, n, k, a, lda, c, ldc, prec, slices);
-  if (st) {
-    set_err("psmma_cauchy_dev: invalid argument %d", -st);
-    return st;
-  }
+  int st = check_common("psmma_cauchy_dev", m, n, k, a, lda, c, ldc, prec, slices);
+  if (st) return st;
   if (r0 < 0 || c0 < 0) return -11;
   if (m > 0 && n > 0 && k > 0 && (!d || !f || !u || !v)) return -7;
   GenPtrs gp{d, f, u, v, nullptr, 0, r0 + k, c0 + n};
@@ -225,11 +246,8 @@ extern "C" int psmma_cauchyeig_dev(int m, int n, int k, double alpha, const doub
                                    const double* difl, const double* difr, int nglob, int r0,
                                    int c0, double beta, double* c, int ldc, int prec, int slices,
                                    void* stream) {
-  int st = check_common(m, n, k, a, lda, c, ldc, prec, slices);
-  if (st) {
-    set_err("psmma_cauchyeig_dev: invalid argument %d", -st);
-    return st;
-  }
+  int st = check_common("psmma_cauchyeig_dev", m, n, k, a, lda, c, ldc, prec, slices);
+  if (st) return st;
   if (r0 < 0 || c0 < 0 || r0 + k > nglob || c0 + n > nglob) {
     set_err("psmma_cauchyeig_dev: block [%d,%d)x[%d,%d) outside order %d", r0, r0 + k, c0, c0 + n,
             nglob);
@@ -271,7 +289,7 @@ extern "C" int psmma_last_status(void) { return g_status; }
 extern "C" int psmma_set_precision(int prec, int slices) {
   if (prec != PSMMA_PREC_FP64 && prec != PSMMA_PREC_SPLIT_I8 && prec != PSMMA_PREC_SPLIT_BF16)
     return -1;
-  if (prec != PSMMA_PREC_FP64 && (slices < 1 || slices > 16)) return -2;
+  if (prec != PSMMA_PREC_FP64 && (slices < 0 || slices > 16)) return -2;
   g_prec = prec;
   g_slices = slices;
   return 0;
@@ -540,6 +558,7 @@ static void entry_dft(const char* name, const int* n, const int* k, const int* n
                       double* c, const int* ldc, const int* imrow, const int* imcol) {
   g_status = 0;
   g_err.clear();
+  if (!grid_ok_ctxt(&context_, name)) return;
   int N = *n, K = *k;
   int info = 0;
   if (N < 0) info = 1;
@@ -571,9 +590,8 @@ static void entry_dft(const char* name, const int* n, const int* k, const int* n
   if (e == hipSuccess && *beta != 0.0)
     e = hipMemcpy2DAsync(dC, cplx * N, c, cplx * (*ldc), cplx * N, N, hipMemcpyHostToDevice, s);
   if (e != hipSuccess) return (void)hip_fail(e, name);
-  g_mu.unlock();
+  // the staging lock stays held: dB and dC are in use until the copy back ends
   int st = pzmdft_dev(N, N, K, N, *alpha, dB, K, 0, 0, *beta, dC, N, PSMMA_PREC_FP64, s);
-  g_mu.lock();
   if (st != 0) {
     if (g_status == 0) g_status = -3;
     fprintf(stderr, "psmma: %s: %s\n", name, g_err.c_str());

@@ -37,12 +37,19 @@
 #include <vector>
 
 #include "psmma_common.h"
+#include "workspace.h"
 
 typedef long v2l __attribute__((ext_vector_type(2)));
 
 namespace psmma {
 
 constexpr int kMaxModuli = 16;
+// Longest contraction of the int8 form: sum_k |r_a r_b| <= K * 2^14 < 2^31.
+constexpr int kSplitI8Kmax = 131071;
+// slices = 0: the fewest moduli that keep this many bits per operand (int8
+// form), or this many bf16 slices (the count that stays <= 1e-13 in the tests).
+constexpr int kAutoSplitBits = 45;
+constexpr int kAutoBf16Slices = 6;
 // Exponent of an all-zero row/column (below any FP64 exponent; sums stay in int).
 constexpr int kNoExp = -10000;
 // Pairwise coprime (distinct prime factors), all <= 256 so a symmetric
@@ -1126,33 +1133,6 @@ __global__ __launch_bounds__(256) void crt_kernel(int M, int N, const unsigned c
               (size_t)blockIdx.x * blockDim.x + threadIdx.x);
 }
 
-// ---------------------------------------------------------------------------
-// Workspace (grow-only, per device).
-struct SplitWs {
-  void* p = nullptr;
-  size_t cap = 0;
-};
-static std::mutex g_ws_mu;
-static std::unordered_map<int, SplitWs> g_ws;
-
-void* split_workspace(size_t bytes) {
-  int dev = 0;
-  hipGetDevice(&dev);
-  std::lock_guard<std::mutex> lk(g_ws_mu);
-  SplitWs& w = g_ws[dev];
-  if (w.cap < bytes) {
-    if (w.p) {
-      hipDeviceSynchronize();
-      hipFree(w.p);
-    }
-    w.p = nullptr;
-    w.cap = 0;
-    if (hipMalloc(&w.p, bytes) != hipSuccess) return nullptr;
-    w.cap = bytes;
-  }
-  return w.p;
-}
-
 // Launches the int8 modular GEMM for all t moduli.  cfg: PSMMA_I8_CFG or by size.
 template <int BMODE>
 static void launch_gemm_i8(int t, int M, int N, int kp, const signed char* ares,
@@ -1227,26 +1207,60 @@ static int finish_crt(int t, int M, int N, const unsigned char* ores, size_t opl
   return e == hipSuccess ? 0 : 1000 + (int)e;
 }
 
+// Workspace layout of one int8 modular product: row / column exponents, t
+// residue planes of A, of H (dense form) or of the generator window (Toeplitz /
+// Hankel with kp <= kWindowKmax), and t residue planes of the result.
+constexpr int kWindowKmax = 16384;  // longest contraction whose window one workgroup stages in LDS
+
+struct I8Layout {
+  bool window = false;
+  int kp = 0, ldo = 0, lpad = 0;
+  long long glo = 0, ghi = -1;
+  size_t aplane = 0, hplane = 0, oplane = 0;
+  size_t off_eh = 0, off_ares = 0, off_hres = 0, off_ores = 0, bytes = 0;
+};
+
+static I8Layout i8_layout(int kind, int t, int M, int N, int K, const GenPtrs& g, int r0, int c0) {
+  auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
+  I8Layout L;
+  L.kp = ((K + 255) / 256) * 256;
+  L.ldo = (M + 15) & ~15;  // residue column pitch (16-byte chunks)
+  L.aplane = (size_t)M * L.kp;
+  L.oplane = (size_t)L.ldo * N;
+  L.window = (kind == KIND_TOEPLITZ || kind == KIND_HANKEL) && L.kp <= kWindowKmax;
+  size_t hbytes;
+  if (L.window) {
+    const bool toe = kind == KIND_TOEPLITZ;
+    L.glo = toe ? (long long)g.korder - 1 + c0 - (r0 + K - 1) : (long long)r0 + c0;
+    L.ghi = toe ? (long long)g.korder - 1 + (c0 + N - 1) - r0 : (long long)r0 + K - 1 + c0 + N - 1;
+    L.lpad = (int)(((L.ghi - L.glo + 1) + 15) / 16 * 16);
+    L.hplane = (size_t)L.lpad;
+    hbytes = (size_t)t * L.lpad;
+  } else {
+    L.hplane = (size_t)L.kp * N;
+    hbytes = (size_t)t * L.hplane;
+  }
+  L.off_eh = al(sizeof(int) * M);
+  L.off_ares = L.off_eh + al(sizeof(int) * N);
+  L.off_hres = L.off_ares + al((size_t)t * L.aplane);
+  L.off_ores = L.off_hres + al(hbytes);
+  L.bytes = L.off_ores + al(std::max((size_t)t * L.oplane, sizeof(int)));
+  return L;
+}
+
+// Enqueues the int8 modular product in the workspace `ws` (i8_layout bytes).
 template <int KIND>
-static int run_split_i8(int t, int M, int N, int K, double alpha, const double* A, int lda,
-                        const GenPtrs& g, int r0, int c0, double beta, double* C, int ldc,
-                        hipStream_t s) {
+static int enqueue_split_i8(int t, int M, int N, int K, double alpha, const double* A, int lda,
+                            const GenPtrs& g, int r0, int c0, double beta, double* C, int ldc,
+                            const I8Layout& L, char* ws, hipStream_t s) {
   const CrtParams cp = make_crt(t);
   const int abits = choose_abits(t, K);
-  if (abits < 8) return -16;
-  const int kp = ((K + 255) / 256) * 256;
-  const int ldo = (M + 15) & ~15;  // residue column pitch (16-byte chunks)
-  const size_t aplane = (size_t)M * kp, hplane = (size_t)kp * N, oplane = (size_t)ldo * N;
-  auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
-  size_t bytes = al(sizeof(int) * M) + al(sizeof(int) * N) + al(t * aplane) + al(t * hplane) +
-                 al(t * oplane);
-  char* ws = (char*)split_workspace(bytes);
-  if (!ws) return 1002;
+  const int kp = L.kp, ldo = L.ldo;
   int* ea = (int*)ws;
-  int* eh = (int*)(ws + al(sizeof(int) * M));
-  signed char* ares = (signed char*)(ws + al(sizeof(int) * M) + al(sizeof(int) * N));
-  signed char* hres = ares + al(t * aplane);
-  unsigned char* ores = (unsigned char*)(hres + al(t * hplane));
+  int* eh = (int*)(ws + L.off_eh);
+  signed char* ares = (signed char*)(ws + L.off_ares);
+  signed char* hres = (signed char*)(ws + L.off_hres);
+  unsigned char* ores = (unsigned char*)(ws + L.off_ores);
 
   hipLaunchKernelGGL(fill_int_kernel, dim3((M + 255) / 256), dim3(256), 0, s, ea, M, kNoExp);
   hipLaunchKernelGGL(fill_int_kernel, dim3((N + 255) / 256), dim3(256), 0, s, eh, N, kNoExp);
@@ -1254,36 +1268,49 @@ static int run_split_i8(int t, int M, int N, int K, double alpha, const double* 
   hipLaunchKernelGGL(row_exp_kernel, dim3((M + 255) / 256, (K + kchunk - 1) / kchunk), dim3(256), 0,
                      s, M, K, A, lda, kchunk, ea);
   hipLaunchKernelGGL(split_a_kernel, dim3((M + 63) / 64, kp / 64), dim3(256), 0, s, M, K, A, lda,
-                     ea, abits, cp, ares, kp, aplane);
-  constexpr int KMAX = 16384;  // longest contraction the generator window form stages in LDS
+                     ea, abits, cp, ares, kp, L.aplane);
   if constexpr (KIND == KIND_TOEPLITZ || KIND == KIND_HANKEL) {
-    if (kp <= KMAX) {
+    if (L.window) {
       // one scale for the block of H; residues of the generator window
       constexpr bool TOE = (KIND == KIND_TOEPLITZ);
-      long long glo = TOE ? (long long)g.korder - 1 + c0 - (r0 + K - 1) : (long long)r0 + c0;
-      long long ghi = TOE ? (long long)g.korder - 1 + (c0 + N - 1) - r0
-                          : (long long)r0 + K - 1 + c0 + N - 1;
-      const int lpad = (int)(((ghi - glo + 1) + 15) / 16 * 16);
       int* e1 = (int*)ores;  // scratch int, consumed before the GEMM writes ores
-      signed char* gres = hres;  // t planes of lpad bytes (fit in the H residue area)
       hipLaunchKernelGGL(fill_int_kernel, dim3(1), dim3(64), 0, s, e1, 1, kNoExp);
-      long long L = ghi - glo + 1;
-      hipLaunchKernelGGL(gen_range_exp_kernel, dim3((unsigned)((L + 255) / 256)), dim3(256), 0, s,
-                         g.p0, glo, ghi, e1);
+      long long len = L.ghi - L.glo + 1;
+      hipLaunchKernelGGL(gen_range_exp_kernel, dim3((unsigned)((len + 255) / 256)), dim3(256), 0, s,
+                         g.p0, L.glo, L.ghi, e1);
       hipLaunchKernelGGL(fill_from_kernel, dim3((N + 255) / 256), dim3(256), 0, s, eh, N, e1);
-      hipLaunchKernelGGL(gen_residue_kernel, dim3((lpad + 255) / 256), dim3(256), 0, s, g.p0, glo,
-                         ghi, TOE ? 1 : 0, e1, abits, cp, gres, (size_t)lpad, lpad);
-      launch_gemm_i8<TOE ? BMODE_TOE : BMODE_HANK>(t, M, N, kp, ares, gres, aplane, (size_t)lpad,
-                                                     lpad, cp, ores, oplane, ldo, s);
-      return finish_crt(t, M, N, ores, oplane, ldo, cp, ea, eh, abits, alpha, beta, C, ldc, s);
+      hipLaunchKernelGGL(gen_residue_kernel, dim3((L.lpad + 255) / 256), dim3(256), 0, s, g.p0,
+                         L.glo, L.ghi, TOE ? 1 : 0, e1, abits, cp, hres, (size_t)L.lpad, L.lpad);
+      launch_gemm_i8<TOE ? BMODE_TOE : BMODE_HANK>(t, M, N, kp, ares, hres, L.aplane,
+                                                     (size_t)L.lpad, L.lpad, cp, ores, L.oplane,
+                                                     ldo, s);
+      return finish_crt(t, M, N, ores, L.oplane, ldo, cp, ea, eh, abits, alpha, beta, C, ldc, s);
     }
   }
   hipLaunchKernelGGL(col_exp_kernel<KIND>, dim3((N + 255) / 256, (K + kchunk - 1) / kchunk),
                      dim3(256), 0, s, K, N, g, r0, c0, kchunk, eh);
-  hipLaunchKernelGGL(split_h_kernel<KIND>, dim3((N + 15) / 16, (kp + 255) / 256), dim3(256), 0, s, K, N, g,
-                     r0, c0, eh, abits, cp, hres, kp, hplane);
-  launch_gemm_i8<BMODE_DENSE>(t, M, N, kp, ares, hres, aplane, hplane, 0, cp, ores, oplane, ldo, s);
-  return finish_crt(t, M, N, ores, oplane, ldo, cp, ea, eh, abits, alpha, beta, C, ldc, s);
+  hipLaunchKernelGGL(split_h_kernel<KIND>, dim3((N + 15) / 16, (kp + 255) / 256), dim3(256), 0, s,
+                     K, N, g, r0, c0, eh, abits, cp, hres, kp, L.hplane);
+  launch_gemm_i8<BMODE_DENSE>(t, M, N, kp, ares, hres, L.aplane, L.hplane, 0, cp, ores, L.oplane,
+                              ldo, s);
+  return finish_crt(t, M, N, ores, L.oplane, ldo, cp, ea, eh, abits, alpha, beta, C, ldc, s);
+}
+
+static int enqueue_split_i8_kind(int kind, int t, int M, int N, int K, double alpha,
+                                 const double* A, int lda, const GenPtrs& g, int r0, int c0,
+                                 double beta, double* C, int ldc, const I8Layout& L, char* ws,
+                                 hipStream_t s) {
+  switch (kind) {
+    case KIND_CAUCHY:
+      return enqueue_split_i8<KIND_CAUCHY>(t, M, N, K, alpha, A, lda, g, r0, c0, beta, C, ldc, L, ws, s);
+    case KIND_TOEPLITZ:
+      return enqueue_split_i8<KIND_TOEPLITZ>(t, M, N, K, alpha, A, lda, g, r0, c0, beta, C, ldc, L, ws, s);
+    case KIND_HANKEL:
+      return enqueue_split_i8<KIND_HANKEL>(t, M, N, K, alpha, A, lda, g, r0, c0, beta, C, ldc, L, ws, s);
+    case KIND_CAUCHYEIG:
+      return enqueue_split_i8<KIND_CAUCHYEIG>(t, M, N, K, alpha, A, lda, g, r0, c0, beta, C, ldc, L, ws, s);
+  }
+  return -1;
 }
 
 hipError_t launch_right_f64(int kind, int cfg, int M, int N, int K, double alpha, const double* A,
@@ -1294,13 +1321,12 @@ static int launch_right_split_i8(int kind, int t, int M, int N, int K, double al
                                  const double* A, int lda, const GenPtrs& g, int r0, int c0,
                                  double beta, double* C, int ldc, hipStream_t s) {
   if (t < 2 || t > kMaxModuli) return -16;
-  switch (kind) {
-    case KIND_CAUCHY: return run_split_i8<KIND_CAUCHY>(t, M, N, K, alpha, A, lda, g, r0, c0, beta, C, ldc, s);
-    case KIND_TOEPLITZ: return run_split_i8<KIND_TOEPLITZ>(t, M, N, K, alpha, A, lda, g, r0, c0, beta, C, ldc, s);
-    case KIND_HANKEL: return run_split_i8<KIND_HANKEL>(t, M, N, K, alpha, A, lda, g, r0, c0, beta, C, ldc, s);
-    case KIND_CAUCHYEIG: return run_split_i8<KIND_CAUCHYEIG>(t, M, N, K, alpha, A, lda, g, r0, c0, beta, C, ldc, s);
-  }
-  return -1;
+  if (choose_abits(t, K) < 8) return -16;
+  const I8Layout L = i8_layout(kind, t, M, N, K, g, r0, c0);
+  std::unique_lock<std::mutex> lk;
+  char* ws = (char*)ws_lease(WS_SPLIT, L.bytes, s, lk);
+  if (!ws) return 1002;
+  return enqueue_split_i8_kind(kind, t, M, N, K, alpha, A, lda, g, r0, c0, beta, C, ldc, L, ws, s);
 }
 
 // ===========================================================================
@@ -1517,7 +1543,8 @@ static int run_split_bf16(int s, int M, int N, int K, double alpha, const double
   const size_t aplane = (size_t)M * kp, hplane = (size_t)kp * N;
   auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
   size_t bytes = al(sizeof(int) * M) + al(sizeof(int) * N) + al(2 * s * aplane) + al(2 * s * hplane);
-  char* ws = (char*)split_workspace(bytes);
+  std::unique_lock<std::mutex> lk;
+  char* ws = (char*)ws_lease(WS_SPLIT, bytes, st, lk);
   if (!ws) return 1002;
   int* ea = (int*)ws;
   int* eh = (int*)(ws + al(sizeof(int) * M));
@@ -1948,23 +1975,6 @@ int gsplit_moduli(const BatchProblem* probs, int nprob, int bits) {
   return split_batch_moduli(probs, nprob, bits);
 }
 
-// Workspace bytes one split-precision call needs (for callers that must
-// allocate it before a graph capture).
-size_t split_workspace_bytes(int prec, int slices, int M, int N, int K) {
-  auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
-  if (prec == 1) {
-    const int kp = ((K + 255) / 256) * 256;
-    const size_t aplane = (size_t)M * kp, hplane = (size_t)kp * N,
-                 oplane = (size_t)((M + 15) & ~15) * N;
-    return al(sizeof(int) * M) + al(sizeof(int) * N) + al(slices * aplane) + al(slices * hplane) +
-           al(slices * oplane);
-  }
-  const int kp = ((K + 255) / 256) * 256;
-  const size_t aplane = (size_t)M * kp, hplane = (size_t)kp * N;
-  return al(sizeof(int) * M) + al(sizeof(int) * N) + al(2 * slices * aplane) +
-         al(2 * slices * hplane);
-}
-
 // Fewest int8 moduli that keep `bits` bits per operand for a contraction of
 // length K (43 bits give a relative error below 1e-12 in the tests and bench).
 int split_moduli_for_bits(int K, int bits) {
@@ -1983,7 +1993,13 @@ int launch_right_split(int kind, int prec, int slices, int M, int N, int K, doub
     hipError_t e = launch_right_f64(kind, -1, M, N, K, alpha, A, lda, g, r0, c0, beta, C, ldc, s);
     return e == hipSuccess ? 0 : 1000 + (int)e;
   }
-  if (prec == 1) return launch_right_split_i8(kind, slices, M, N, K, alpha, A, lda, g, r0, c0, beta, C, ldc, s);
+  if (prec == 1) {
+    // int32 accumulation of products of residues |r| <= 128 is exact for K < 2^17
+    if (K > kSplitI8Kmax) return -3;
+    const int t = slices ? slices : split_moduli_for_bits(K, kAutoSplitBits);
+    return launch_right_split_i8(kind, t, M, N, K, alpha, A, lda, g, r0, c0, beta, C, ldc, s);
+  }
+  if (slices == 0) slices = kAutoBf16Slices;
   return launch_right_split_bf16(kind, slices, M, N, K, alpha, A, lda, g, r0, c0, beta, C, ldc, s);
 }
 

@@ -1,0 +1,21 @@
+// Device workspaces of the library's multi-kernel products (see workspace.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <mutex>
+
+namespace psmma {
+
+// Workspace slots (one grow-only buffer per device, stream and slot).
+enum WsSlot : int {
+  WS_SPLIT = 0,     // split-precision products (single and batched eager calls)
+  WS_BLR_FIXED = 1, // compressed products: block lists and compressor outputs
+  WS_BLR_GEMM = 2,  // compressed products: rank-dependent GEMM operands
+};
+
+// Returns a buffer of at least `bytes` for (current device, s, slot), or nullptr
+// when the allocation fails.  On success `lk` holds the entry until the caller
+// releases it: keep it until every kernel using the buffer has been enqueued.
+void* ws_lease(int slot, size_t bytes, hipStream_t s, std::unique_lock<std::mutex>& lk);
+
+}  // namespace psmma

@@ -126,5 +126,28 @@ def test_dft_info_codes(capfd):
 
 def test_set_precision_validates():
     assert api.lib().psmma_set_precision(9, 3) == -1
-    assert api.lib().psmma_set_precision(api.PREC_SPLIT_I8, 0) == -2
+    assert api.lib().psmma_set_precision(api.PREC_SPLIT_I8, -1) == -2
+    assert api.lib().psmma_set_precision(api.PREC_SPLIT_I8, 17) == -2
+    assert api.lib().psmma_set_precision(api.PREC_SPLIT_I8, 0) == 0  # 0: chosen from K
     assert api.lib().psmma_set_precision(api.PREC_FP64, 0) == 0
+
+
+GRID_CHECK = Path(__file__).resolve().parent / "fortran" / "_build" / "grid_check"
+
+
+@pytest.mark.skipif(not FLANG.exists(), reason="amdflang not installed")
+def test_fortran_caller_on_a_larger_grid_is_refused():
+    """A Fortran caller whose BLACS reports a 2 x 2 grid: PSMMA_TOEPLITZ (context from
+    its descriptor) and PZMDFTL1 / PZMDFTL (context from COMMON /CONTEXT/) refuse it
+    before any device work and leave their outputs untouched (tests/fortran/grid_check.f90,
+    built with the library)."""
+    import subprocess
+
+    if not GRID_CHECK.exists():
+        subprocess.run(["make", "-s", "-C", str(GRID_CHECK.parents[1])], check=True)
+    r = subprocess.run([str(GRID_CHECK)], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stderr
+    vals = dict(line.split() for line in r.stdout.strip().splitlines())
+    assert vals == {"toeplitz_status": "-2", "pzmdftl1_status": "-2", "pzmdftl_status": "-2",
+                    "outputs_untouched": "1"}
+    assert r.stderr.count("process grid 2 x 2") == 3

@@ -1,16 +1,17 @@
 """The Fortran drop-in: a Fortran program calls PSMMA_TOEPLITZ / PSMMA_HANKEL with
 the reference's argument lists and links against libpsmma_hip.so (MI355X only)."""
 import subprocess
+from pathlib import Path
 
 import pytest
 
-from psmma_git_amd import build as pbuild
-
 pytestmark = pytest.mark.gpu
+
+DRIVER = Path(__file__).resolve().parent / "fortran" / "_build" / "psmma_fortran_driver"
 
 
 def test_fortran_driver_links_and_matches_closed_form(gpu):
-    drv = pbuild.FORTRAN_DRIVER
+    drv = DRIVER
     if not drv.exists():
         pytest.skip("Fortran driver not built (amdflang unavailable at build time)")
     r = subprocess.run([str(drv)], capture_output=True, text=True, timeout=120)

@@ -285,3 +285,61 @@ def test_fortran_dft_entry(gpu, ora, entry):
     want = ora.dft_matrix(n, 0, n, 0, n) @ b0[:n] + 2.0
     assert ora.rel_fro(c, want) < TOL
     assert np.array_equal(b, b0)  # one process row: no ring shift, B intact
+
+
+# ---------------------------------------------------------------------------
+# The compute-level routines (PS*_COMPUTE) called directly through the C ABI.
+
+@pytest.mark.parametrize("kind", ["cauchy", "toeplitz", "hankel"])
+def test_ps_compute_entries(gpu, ora, kind):
+    # SRC/pscauchy_compute.f90:2-3 and siblings: C = alpha*A*H + beta*C with local
+    # LDA / LDC and the generator array B(LDB, *) as the drivers pass it.
+    n = 230
+    rng = np.random.default_rng(case_seed("ps_compute", kind))
+    lda, ldc = n + 3, n + 6
+    a = np.asfortranarray(rng.uniform(-1, 1, (lda, n)))
+    c = np.asfortranarray(rng.standard_normal((ldc, n)))
+    c0 = c.copy()
+    desca = api.descinit(n, n, 32, 32, lda)
+    if kind == "cauchy":
+        b = inputs.cauchy_generators(n, rng)
+        gens = dict(d=b[:, 0], f=b[:, 1], u=b[:, 2], v=b[:, 3])
+        api.pscauchy_compute(n, n, 1.5, a, lda, desca, b, n, -0.5, c, ldc)
+    else:
+        b = rng.standard_normal(2 * n - 1)
+        gens = dict(g=b, korder=n)
+        fn = api.pstoeplitz_compute if kind == "toeplitz" else api.pshankel_compute
+        fn(n, n, 1.5, a, lda, desca, b, 2 * n - 1, -0.5, c, ldc)
+    assert api.last_status() == 0
+    want = ora.structured_gemm_np(ORA_KIND[kind], 1.5, a[:n], gens, 0, 0, n, -0.5, c0[:n])
+    assert ora.rel_fro(c[:n], want) < TOL
+    assert np.array_equal(c[n:], c0[n:])  # rows beyond N untouched
+
+
+def test_pscauchyeig_compute_entry(gpu, ora):
+    n = 176
+    e = inputs.eig_merge_inputs(n, seed=case_seed("pscauchyeig_compute") % 1000, with_q=True)
+    lda, ldc = n + 2, n + 5
+    a = np.zeros((lda, n), order="F")
+    a[:n] = e["q"]
+    c = np.zeros((ldc, n), order="F")
+    desca = api.descinit(n, n, 16, 16, lda)
+    lam = e["f"] + e["difl"]
+    api.pscauchyeig_compute(n, n, 16, 16, 1.0, a, lda, desca, e["f"], lam, e["u"], e["v"],
+                            e["difl"], e["difr"], 0.0, c, ldc)
+    assert api.last_status() == 0
+    want = ora.structured_gemm_np(3, 1.0, e["q"], e, 0, 0, n)
+    assert ora.rel_fro(c[:n], want) < TOL
+
+
+def test_dev_wrappers_check_shapes_before_the_call(gpu):
+    import torch
+
+    a = torch.zeros((64, 64), dtype=torch.float64, device="cuda").t()
+    g = torch.zeros(127, dtype=torch.float64, device="cuda")
+    small = torch.zeros((60, 64), dtype=torch.float64, device="cuda").t()  # 64 x 60
+    with pytest.raises(ValueError, match="smaller than"):
+        api.toeplitz_dev(1.0, a, g, 64, 64, small)
+    d = torch.zeros(63, dtype=torch.float64, device="cuda")  # one short of r0 + k
+    with pytest.raises(ValueError, match="needs 64"):
+        api.cauchy_dev(1.0, a, d, g, g, g, 64, a.clone())
