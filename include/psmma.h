@@ -158,6 +158,24 @@ int psmma_toeplitz_dev(int m, int n, int k, double alpha, const double* a, int l
                        const double* g, int lg, int korder, int r0, int c0, double beta, double* c,
                        int ldc, int prec, int slices, void* stream);
 
+/* Allocation-free variant of psmma_toeplitz_dev.  No counterpart in the reference:
+ * it is for callers that cannot allocate (code under HIP graph capture, serving
+ * loops or harnesses that forbid hipMalloc).  Same arguments and results as
+ * psmma_toeplitz_dev for prec = PSMMA_PREC_FP64 (which needs no scratch) and
+ * PSMMA_PREC_SPLIT_I8, whose scratch is a static array of
+ * PSMMA_NOALLOC_SCRATCH_BYTES per GPU (held from the library's first use on that
+ * GPU until the process ends) instead of the grow-only workspace; the call
+ * performs no allocation.  A product whose int8 layout exceeds the array
+ * (about 2 t M K + 2 t M N bytes for t moduli, e.g. M = N = K = 2048 at t = 14)
+ * is rejected with -1 and a message naming the bytes it needs.
+ * PSMMA_PREC_SPLIT_BF16 is rejected (-15).  All calls on one GPU share the
+ * array, which keeps the last call's contents: calls that overlap on one GPU
+ * (other streams or threads) race; order them on one stream. */
+#define PSMMA_NOALLOC_SCRATCH_BYTES (128u << 20)
+int psmma_toeplitz_noalloc_dev(int m, int n, int k, double alpha, const double* a, int lda,
+                               const double* g, int lg, int korder, int r0, int c0, double beta,
+                               double* c, int ldc, int prec, int slices, void* stream);
+
 /* Hankel block: H(r,c) = g[r + c] (0-based).  Requires r0+k-1 + c0+n-1 < lg. */
 int psmma_hankel_dev(int m, int n, int k, double alpha, const double* a, int lda, const double* g,
                      int lg, int r0, int c0, double beta, double* c, int ldc, int prec, int slices,

@@ -60,6 +60,8 @@ _SIGS = {
     "psmma_last_error": ([], ctypes.c_char_p),
     "psmma_version": ([], ctypes.c_char_p),
     "psmma_toeplitz_dev": ([_I, _I, _I, _D, _P, _I, _P, _I, _I, _I, _I, _D, _P, _I, _I, _I, _P], _I),
+    "psmma_toeplitz_noalloc_dev": ([_I, _I, _I, _D, _P, _I, _P, _I, _I, _I, _I, _D, _P, _I, _I,
+                                    _I, _P], _I),
     "psmma_hankel_dev": ([_I, _I, _I, _D, _P, _I, _P, _I, _I, _I, _D, _P, _I, _I, _I, _P], _I),
     "psmma_cauchy_dev": ([_I, _I, _I, _D, _P, _I, _P, _P, _P, _P, _I, _I, _D, _P, _I, _I, _I, _P], _I),
     "psmma_cauchyeig_dev": ([_I, _I, _I, _D, _P, _I, _P, _P, _P, _P, _P, _I, _I, _I, _D, _P, _I,
@@ -314,6 +316,20 @@ def toeplitz_dev(alpha, a, g, korder, n, c, beta=0.0, r0=0, c0=0, prec=PREC_FP64
     _check(lib().psmma_toeplitz_dev(m, n, k, alpha, pa, lda, _vec(g, "g"), g.numel(), korder, r0,
                                     c0, beta, pc, ldc, prec, slices, _stream_ptr(stream)),
            "psmma_toeplitz_dev")
+
+
+NOALLOC_SCRATCH_BYTES = 128 << 20
+
+
+def toeplitz_noalloc_dev(alpha, a, g, korder, n, c, beta=0.0, r0=0, c0=0, prec=PREC_SPLIT_I8,
+                         slices=0, stream=None):
+    """psmma_toeplitz_noalloc_dev: toeplitz_dev without any allocation (FP64 or the int8
+    form in the library's static per-GPU scratch array)."""
+    pa, m, k, lda = _colmajor(a, "a")
+    pc, ldc = _out(c, m, n)
+    _check(lib().psmma_toeplitz_noalloc_dev(m, n, k, alpha, pa, lda, _vec(g, "g"), g.numel(),
+                                            korder, r0, c0, beta, pc, ldc, prec, slices,
+                                            _stream_ptr(stream)), "psmma_toeplitz_noalloc_dev")
 
 
 def hankel_dev(alpha, a, g, n, c, beta=0.0, r0=0, c0=0, prec=PREC_FP64, slices=0, stream=None):

@@ -24,6 +24,10 @@ hipError_t launch_dft_f64(int M, int N, int K, int Nd, double alpha, const doubl
 int launch_right_split(int kind, int prec, int slices, int M, int N, int K, double alpha,
                        const double* A, int lda, const GenPtrs& g, int r0, int c0, double beta,
                        double* C, int ldc, hipStream_t s);
+int launch_right_split_i8_noalloc(int kind, int slices, int M, int N, int K, double alpha,
+                                  const double* A, int lda, const GenPtrs& g, int r0, int c0,
+                                  double beta, double* C, int ldc, hipStream_t s, size_t* need);
+size_t split_noalloc_capacity();
 }  // namespace psmma
 
 using namespace psmma;
@@ -208,6 +212,40 @@ extern "C" int psmma_toeplitz_dev(int m, int n, int k, double alpha, const doubl
   GenPtrs gp{g, nullptr, nullptr, nullptr, nullptr, korder, r0 + k, c0 + n};
   return run_right(KIND_TOEPLITZ, m, n, k, alpha, a, lda, gp, r0, c0, beta, c, ldc, prec, slices,
                    (hipStream_t)stream);
+}
+
+// No counterpart in the reference: psmma_toeplitz_dev for callers that cannot
+// allocate.  The int8 form's scratch is the library's static per-GPU array.
+extern "C" int psmma_toeplitz_noalloc_dev(int m, int n, int k, double alpha, const double* a,
+                                          int lda, const double* g, int lg, int korder, int r0,
+                                          int c0, double beta, double* c, int ldc, int prec,
+                                          int slices, void* stream) {
+  const char* name = "psmma_toeplitz_noalloc_dev";
+  int st = check_toeplitz(name, m, n, k, a, lda, g, lg, korder, r0, c0, c, ldc, prec, slices);
+  if (st) return st;
+  if (prec == PSMMA_PREC_SPLIT_BF16) {
+    set_err("%s: the bf16 form is not available without allocation (use FP64 or SPLIT_I8)", name);
+    return -15;
+  }
+  if (m == 0 || n == 0) return 0;
+  GenPtrs gp{g, nullptr, nullptr, nullptr, nullptr, korder, r0 + k, c0 + n};
+  hipStream_t s = (hipStream_t)stream;
+  if (prec == PSMMA_PREC_FP64 || k == 0) {
+    hipError_t e = launch_right_f64(KIND_TOEPLITZ, -1, m, n, k, alpha, a, lda, gp, r0, c0, beta, c,
+                                    ldc, s);
+    return e == hipSuccess ? 0 : hip_fail(e, name);
+  }
+  size_t need = 0;
+  st = launch_right_split_i8_noalloc(KIND_TOEPLITZ, slices, m, n, k, alpha, a, lda, gp, r0, c0,
+                                     beta, c, ldc, s, &need);
+  if (st == -1)
+    set_err("%s: the product needs %zu bytes of scratch; the static scratch holds %zu", name, need,
+            split_noalloc_capacity());
+  else if (st == -16)
+    set_err("%s: invalid slice count %d for K = %d", name, slices, k);
+  else if (st > 0)
+    set_err("%s: runtime error %d", name, st);
+  return st;
 }
 
 extern "C" int psmma_hankel_dev(int m, int n, int k, double alpha, const double* a, int lda,

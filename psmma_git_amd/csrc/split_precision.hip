@@ -2003,4 +2003,61 @@ int launch_right_split(int kind, int prec, int slices, int M, int N, int K, doub
   return launch_right_split_bf16(kind, slices, M, N, K, alpha, A, lda, g, r0, c0, beta, C, ldc, s);
 }
 
+// ===========================================================================
+// Allocation-free int8 modular product (psmma_toeplitz_noalloc_dev).
+//
+// The scratch memory of the int8 form lives in a file-scope device array of
+// fixed size instead of the grow-only per-stream workspace, so a call allocates
+// nothing (callers under graph capture, serving loops that forbid hipMalloc).
+// Every GPU the library runs on holds one copy of the array until the process
+// ends.  In host code `split_scratch` names a host-side stand-in; the device
+// address of each GPU's copy comes from hipGetSymbolAddress, taken once per
+// device with that device current, and only that address is passed to kernels.
+// All calls on one GPU share the array: calls that overlap on one GPU race.
+constexpr size_t kNoallocScratchBytes = size_t(128) << 20;
+static __device__ __align__(256) unsigned char split_scratch[kNoallocScratchBytes];
+
+static std::mutex g_scratch_mu;
+static std::unordered_map<int, char*> g_scratch_ptr;
+
+// Device address of the current device's scratch array (nullptr on failure).
+static char* scratch_address() {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return nullptr;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  auto it = g_scratch_ptr.find(dev);
+  if (it != g_scratch_ptr.end()) return it->second;
+  void* p = nullptr;
+  if (hipGetSymbolAddress(&p, HIP_SYMBOL(split_scratch)) != hipSuccess || p == nullptr)
+    return nullptr;
+  g_scratch_ptr[dev] = (char*)p;
+  return (char*)p;
+}
+
+size_t split_noalloc_capacity() { return kNoallocScratchBytes; }
+
+// Scratch bytes the int8 form needs for one product (0 when it cannot run).
+size_t split_i8_bytes(int kind, int slices, int M, int N, int K, const GenPtrs& g, int r0, int c0) {
+  const int t = slices ? slices : split_moduli_for_bits(K, kAutoSplitBits);
+  if (t < 2 || t > kMaxModuli || choose_abits(t, K) < 8) return 0;
+  return i8_layout(kind, t, M, N, K, g, r0, c0).bytes;
+}
+
+// The int8 form in the static scratch array.  Returns 0, -16 (slices), -1 with
+// *need set when the product needs more scratch than the array holds, or a
+// runtime error (> 0).
+int launch_right_split_i8_noalloc(int kind, int slices, int M, int N, int K, double alpha,
+                                  const double* A, int lda, const GenPtrs& g, int r0, int c0,
+                                  double beta, double* C, int ldc, hipStream_t s, size_t* need) {
+  if (K > kSplitI8Kmax) return -3;
+  const int t = slices ? slices : split_moduli_for_bits(K, kAutoSplitBits);
+  if (t < 2 || t > kMaxModuli || choose_abits(t, K) < 8) return -16;
+  const I8Layout L = i8_layout(kind, t, M, N, K, g, r0, c0);
+  if (need) *need = L.bytes;
+  if (L.bytes > kNoallocScratchBytes) return -1;
+  char* ws = scratch_address();
+  if (!ws) return 1000 + (int)hipErrorInvalidSymbol;
+  return enqueue_split_i8_kind(kind, t, M, N, K, alpha, A, lda, g, r0, c0, beta, C, ldc, L, ws, s);
+}
+
 }  // namespace psmma
