@@ -152,7 +152,19 @@ def other_configs(seed, torch, api, inputs):
     out["dft_n4096_k512"] = {"fp64_ms": round(msd, 4),
                              "fp64_tflops": round(8 * n * n * k / msd / 1e9, 2),
                              "rel_err_vs_fft": rel(dc, ref), "hipfft_ms": round(msf, 4)}
-    del db, dc, fft_in, ref
+    # split-FP32 form (8-bit slices on FP32 MFMA): error and time per slice count
+    ref64 = dc.clone()
+    sp = {}
+    for sl in (2, 3, 5):
+        mss = ev_time(lambda: api.dft_dev(1.0 / np.sqrt(n), db, n, n, dc,
+                                          prec=api.PREC_SPLIT_FP32, slices=sl), 5, torch)
+        sp[f"slices_{sl}"] = {"ms": round(mss, 4), "rel_err_vs_fft": rel(dc, ref),
+                              "rel_err_vs_fp64": rel(dc, ref64),
+                              "speedup_vs_fp64": round(msd / mss, 3)}
+    out["dft_n4096_k512"]["split_fp32"] = sp
+    out["dft_n4096_k512"]["split_fp32_ms"] = sp["slices_5"]["ms"]
+    out["dft_n4096_k512"]["split_fp32_rel_err"] = sp["slices_5"]["rel_err_vs_fft"]
+    del db, dc, fft_in, ref, ref64
     out["batched_512"] = batched_config(seed, torch, api, inputs)
     torch.cuda.empty_cache()
     out["eig_merge_n16384"] = eig_merge_config(seed, torch, api, inputs)
@@ -395,6 +407,35 @@ def profile_leg(leg, seed, torch, api, inputs):
     """One workload alone, for kernel traces and counters."""
     if leg == "batched":
         return batched_config(seed, torch, api, inputs)
+    if leg == "dft":  # FP64 and split-FP32 (5 slices) at N = 4096, 512 columns
+        n, k = 4096, 512
+        bb = inputs.dft_inputs(n, k, seed=seed)
+        db = torch.from_numpy(np.ascontiguousarray(bb.T)).cuda().t()
+        dc = api.colmajor_empty(n, k, dtype=torch.complex128)
+        ms = ev_time(lambda: api.dft_dev(1.0 / np.sqrt(n), db, n, n, dc), 3, torch)
+        mss = ev_time(lambda: api.dft_dev(1.0 / np.sqrt(n), db, n, n, dc, prec=api.PREC_SPLIT_FP32,
+                                          slices=5), 3, torch)
+        return {"leg": leg, "n": n, "k": k, "fp64_ms": round(ms, 4), "split_fp32_ms": round(mss, 4)}
+    if leg in ("hankel", "bf16"):  # Hankel FP64, or Toeplitz in 3 bf16 slices, N = 8192
+        n = 8192
+        g_np, a_np = inputs.toeplitz_hankel_inputs(n, seed=seed)
+        g = torch.from_numpy(g_np).cuda()
+        a = colmajor_from_numpy(a_np, torch)
+        c = api.colmajor_empty(n, n)
+        if leg == "hankel":
+            ms = ev_time(lambda: api.hankel_dev(1.0, a, g, n, c), 3, torch)
+        else:
+            ms = ev_time(lambda: api.toeplitz_dev(1.0, a, g, n, n, c, prec=api.PREC_SPLIT_BF16,
+                                                  slices=3), 2, torch)
+        return {"leg": leg, "n": n, "ms": round(ms, 3)}
+    if leg == "cauchy2048":  # plain Cauchy x dense, N = 2048, FP64
+        n = 2048
+        b_np, a_np = inputs.cauchy_dense_inputs(n, seed=seed)
+        d, f, u, v = (torch.from_numpy(np.ascontiguousarray(b_np[:, i])).cuda() for i in range(4))
+        a = colmajor_from_numpy(a_np, torch)
+        c = api.colmajor_empty(n, n)
+        ms = ev_time(lambda: api.cauchy_dev(1.0, a, d, f, u, v, n, c), 10, torch)
+        return {"leg": leg, "n": n, "ms": round(ms, 4)}
     if leg == "split":
         n = 8192
         g_np, a_np = inputs.toeplitz_hankel_inputs(n, seed=seed)
@@ -414,6 +455,9 @@ def profile_leg(leg, seed, torch, api, inputs):
     gen.manual_seed(seed)
     a = torch.randn((n, n), dtype=torch.float64, device="cuda", generator=gen).t()
     x = api.colmajor_empty(n, n)
+    if leg == "eig_fp64":  # the plain FP64 eigen-Cauchy product at N = 16384
+        ms = ev_time(lambda: api.cauchyeig_dev(1.0, a, td, u, v, difl, difr, n, x), 2, torch)
+        return {"leg": leg, "n": n, "ms": round(ms, 3)}
     stats = {}
     comp = api.BLR_RRLU if leg == "blr_rrlu" else None
     ms = ev_time(lambda: stats.update(api.cauchyeig_blr_dev(1.0, a, td, u, v, difl, difr, x,
@@ -435,12 +479,15 @@ def main():
                     help="skip the other benchmark configurations (reported beside the headline)")
     ap.add_argument("--copy-calib", action="store_true",
                     help="run one 512 MiB device copy first (calibrates HBM byte counters)")
-    ap.add_argument("--profile-leg", choices=["split", "batched", "blr", "blr_rrlu", "rrlu"],
+    ap.add_argument("--profile-leg", choices=["split", "batched", "blr", "blr_rrlu", "rrlu", "dft",
+                                              "hankel", "bf16", "cauchy2048", "eig_fp64"],
                     help="run only one workload (for rocprofv3): the N=8192 Toeplitz split product "
                          "(t=13, 3 launches), the 512-problem batch, the N=16384 compressed "
                          "merge product with the RRQR or the RRLU compressor (3 launches), or the generator-only RRLU of the N=16384 "
-                         "merge's 4032 leaf blocks (two tolerances); prints a short JSON line, not "
-                         "the bench line")
+                         "merge's 4032 leaf blocks (two tolerances), the DFT (FP64 and split-FP32), "
+                         "Hankel N=8192, Toeplitz N=8192 in 3 bf16 slices, Cauchy N=2048 or the "
+                         "plain eigen-Cauchy product N=16384; prints a short JSON line, not the "
+                         "bench line")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:

@@ -61,6 +61,13 @@ extern "C" {
 #define PSMMA_PREC_FP64 0
 #define PSMMA_PREC_SPLIT_I8 1
 #define PSMMA_PREC_SPLIT_BF16 2
+/* Split-FP32 form of the DFT product (pzmdft_dev): F and the column-scaled B cut
+ * into `slices` (1..6, 0 = 5) fixed-point slices of 8 bits, exact in FP32; all
+ * slice products of a class p+q are summed exactly in FP32 over K-slabs of 8 on
+ * v_mfma_f32_16x16x4f32 and recombined in FP64.  Error about 2^(-8 slices)
+ * normwise; S(S+1)/2 slice pairs cost (S(S+1)/2)/2 times the plain FP64 form's
+ * MFMA time, since FP32 MFMA runs at twice the FP64 rate on MI355X. */
+#define PSMMA_PREC_SPLIT_FP32 3
 #define PSMMA_SPLIT_I8_KMAX 131071
 
 /* Kinds of structured operand (psmma_structured_dev, batched API). */
@@ -137,8 +144,11 @@ void pscauchyeig_compute_(const int* n, const int* k, const int* mb, const int* 
                           const double* beta, double* c, const int* ldc, const int* imrow,
                           const int* imcol, double* work);
 
-/* Numeric form used by the Fortran entry points (default PSMMA_PREC_FP64).
- * Returns 0, or -1 / -2 for an invalid prec / slice count. */
+/* Numeric form used by the Fortran entry points (default PSMMA_PREC_FP64).  The
+ * real kinds use PSMMA_PREC_SPLIT_I8 / _BF16 when selected (FP64 otherwise), the
+ * DFT entries PSMMA_PREC_SPLIT_FP32 when selected (FP64 otherwise).  slices = 0
+ * picks each form's default.  Returns 0, or -1 / -2 for an invalid prec / slice
+ * count. */
 int psmma_set_precision(int prec, int slices);
 /* Status of the last Fortran entry call on this thread: 0 or the reference's
  * INFO code (argument number), or a negative runtime error. */
@@ -198,9 +208,12 @@ int psmma_cauchyeig_dev(int m, int n, int k, double alpha, const double* a, int 
 
 /* DFT block (left-structured, complex128):
  * C(m x n) = alpha * F(r0:r0+m, c0:c0+k) * B(k x n) + beta * C,
- * F(r,c) = exp(-2*pi*i * (r*c mod nd) / nd).  prec: PSMMA_PREC_FP64 only. */
+ * F(r,c) = exp(-2*pi*i * (r*c mod nd) / nd).  prec: PSMMA_PREC_FP64 or
+ * PSMMA_PREC_SPLIT_FP32 (slices 1..6, 0 = 5; ignored for FP64).  The Fortran
+ * entries pzmdftl1_/pzmdftl_ use the split-FP32 form when psmma_set_precision
+ * selected it, the FP64 form otherwise. */
 int pzmdft_dev(int m, int n, int k, int nd, double alpha, const double* b, int ldb, int r0, int c0,
-               double beta, double* c, int ldc, int prec, void* stream);
+               double beta, double* c, int ldc, int prec, int slices, void* stream);
 
 /* ===========================================================================
  * 3. Batched uneven products (device pointers).

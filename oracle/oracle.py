@@ -165,6 +165,53 @@ def dft_matrix(nd: int, r0: int, m: int, c0: int, k: int) -> np.ndarray:
     return np.cos(ang) - 1j * np.sin(ang)
 
 
+_TWO_PI_LD = np.longdouble("6.283185307179586476925286766559005768")
+
+
+def dft_twiddles(nd: int) -> np.ndarray:
+    """tw[e] = exp(-2 pi i e / nd) for e in [0, nd), each part rounded once from long double
+    (the FP64 table the DFT kernels read)."""
+    ang = _TWO_PI_LD * np.arange(nd).astype(np.longdouble) / np.longdouble(nd)
+    return np.cos(ang).astype(np.float64) - 1j * np.sin(ang).astype(np.float64)
+
+
+def slice8(x, slices: int) -> list:
+    """Fixed-point slices of x in [-1, 1]: x_1 = rint(x 2^8) 2^-8, x_p = rint(r 2^(8p)) 2^(-8p)
+    of the remainder r; |x - sum x_p| <= 2^(-8 slices - 1)."""
+    out, r = [], np.array(x, np.float64, copy=True)
+    for p in range(slices):
+        sc = 2.0 ** (8 * (p + 1))
+        q = np.rint(r * sc) / sc
+        out.append(q)
+        r = r - q
+    return out
+
+
+def dft_split_fp32(nd: int, r0: int, m: int, c0: int, b, slices: int) -> np.ndarray:
+    """The split-FP32 DFT product F(r0:r0+m, c0:c0+k) * B as the split scheme defines it:
+    columns of B scaled by 2^-e_j (max |Re|, |Im| < 2^e_j), F and the scaled B cut into
+    `slices` 8-bit fixed-point slices, slice pairs p + q <= slices + 1 multiplied exactly and
+    summed, then scaled back.  Test infrastructure for the split-FP32 form of the product of
+    SRC/pzmdftL1.f90:488-498."""
+    b = np.asarray(b, np.complex128)
+    k, n = b.shape
+    tw = dft_twiddles(nd)
+    e = ((np.arange(r0, r0 + m, dtype=np.int64)[:, None] *
+          np.arange(c0, c0 + k, dtype=np.int64)[None, :]) % nd)
+    f = tw[e]
+    amax = np.maximum(np.abs(b.real), np.abs(b.imag)).max(axis=0) if k else np.zeros(n)
+    ex = np.frexp(amax)[1]
+    scale = np.where(amax > 0, np.ldexp(1.0, -ex), 0.0)
+    bs = b * scale[None, :]
+    fr, fi = slice8(f.real, slices), slice8(f.imag, slices)
+    br, bi = slice8(bs.real, slices), slice8(bs.imag, slices)
+    x = np.zeros((m, n), np.complex128)
+    for p in range(slices):
+        for q in range(slices - p):
+            x += (fr[p] + 1j * fi[p]) @ (br[q] + 1j * bi[q])
+    return x * np.where(amax > 0, np.ldexp(1.0, ex), 0.0)[None, :]
+
+
 def structured_gemm_np(kind: int, alpha, a, gens: dict, r0: int, c0: int, n: int, beta=0.0,
                        c=None) -> np.ndarray:
     """C = alpha * A * H(r0:r0+k, c0:c0+n) + beta * C in FP64 with an explicit H."""

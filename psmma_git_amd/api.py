@@ -25,7 +25,7 @@ import numpy as np
 
 LIB_PATH = Path(__file__).resolve().parent / "libpsmma_hip.so"
 
-PREC_FP64, PREC_SPLIT_I8, PREC_SPLIT_BF16 = 0, 1, 2
+PREC_FP64, PREC_SPLIT_I8, PREC_SPLIT_BF16, PREC_SPLIT_FP32 = 0, 1, 2, 3
 KIND_CAUCHY, KIND_TOEPLITZ, KIND_HANKEL, KIND_CAUCHYEIG = 0, 1, 2, 3
 
 
@@ -66,7 +66,7 @@ _SIGS = {
     "psmma_cauchy_dev": ([_I, _I, _I, _D, _P, _I, _P, _P, _P, _P, _I, _I, _D, _P, _I, _I, _I, _P], _I),
     "psmma_cauchyeig_dev": ([_I, _I, _I, _D, _P, _I, _P, _P, _P, _P, _P, _I, _I, _I, _D, _P, _I,
                              _I, _I, _P], _I),
-    "pzmdft_dev": ([_I, _I, _I, _I, _D, _P, _I, _I, _I, _D, _P, _I, _I, _P], _I),
+    "pzmdft_dev": ([_I, _I, _I, _I, _D, _P, _I, _I, _I, _D, _P, _I, _I, _I, _P], _I),
     "psmma_secular_dev": ([_I, _P, _P, _D, _P, _P, _P, _P, _P], _I),
     "psmma_rrqr_batched_dev": ([_I, _I, _I, _P, _I, ctypes.c_longlong, _D, _I, _P, _P, _P, _I,
                                 ctypes.c_longlong, _P, _I, ctypes.c_longlong, _P], _I),
@@ -364,12 +364,13 @@ def cauchyeig_dev(alpha, a, fpole, u, v, difl, difr, n, c, beta=0.0, r0=0, c0=0,
                                      prec, slices, _stream_ptr(stream)), "psmma_cauchyeig_dev")
 
 
-def dft_dev(alpha, b, nd, m, c, beta=0.0, r0=0, c0=0, prec=PREC_FP64, stream=None):
-    """C(m x n) = alpha * F(r0:r0+m, c0:c0+k) * B + beta*C; b, c complex128 column-major."""
+def dft_dev(alpha, b, nd, m, c, beta=0.0, r0=0, c0=0, prec=PREC_FP64, slices=0, stream=None):
+    """C(m x n) = alpha * F(r0:r0+m, c0:c0+k) * B + beta*C; b, c complex128 column-major.
+    prec: PREC_FP64 or PREC_SPLIT_FP32 (slices 1..6, 0 = 5)."""
     import torch
     pb, k, n, ldb = _colmajor(b, "b", torch.complex128)
     pc, ldc = _out(c, m, n, "c", torch.complex128)
-    _check(lib().pzmdft_dev(m, n, k, nd, alpha, pb, ldb, r0, c0, beta, pc, ldc, prec,
+    _check(lib().pzmdft_dev(m, n, k, nd, alpha, pb, ldb, r0, c0, beta, pc, ldc, prec, slices,
                             _stream_ptr(stream)), "pzmdft_dev")
 
 

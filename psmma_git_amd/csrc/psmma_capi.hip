@@ -21,6 +21,9 @@ hipError_t launch_right_f64(int kind, int cfg, int M, int N, int K, double alpha
 hipError_t launch_dft_f64(int M, int N, int K, int Nd, double alpha, const double2* tw,
                           const double2* B, int ldb, int r0, int c0, double beta, double2* C, int ldc,
                           hipStream_t s);
+int launch_dft_split_f32(int M, int N, int K, int Nd, int slices, double alpha, const double2* tw,
+                         const double2* B, int ldb, int r0, int c0, double beta, double2* C,
+                         int ldc, hipStream_t s);
 int launch_right_split(int kind, int prec, int slices, int M, int N, int K, double alpha,
                        const double* A, int lda, const GenPtrs& g, int r0, int c0, double beta,
                        double* C, int ldc, hipStream_t s);
@@ -298,23 +301,36 @@ extern "C" int psmma_cauchyeig_dev(int m, int n, int k, double alpha, const doub
 }
 
 extern "C" int pzmdft_dev(int m, int n, int k, int nd, double alpha, const double* b, int ldb,
-                          int r0, int c0, double beta, double* c, int ldc, int prec, void* stream) {
-  if (m < 0) return -1;
-  if (n < 0) return -2;
-  if (k < 0) return -3;
-  if (nd < 1) return -4;
-  if (ldb < (k > 1 ? k : 1)) return -7;
-  if (r0 < 0) return -8;
-  if (c0 < 0) return -9;
-  if (ldc < (m > 1 ? m : 1)) return -12;
-  if (prec != PSMMA_PREC_FP64) {
-    set_err("pzmdft_dev: only PSMMA_PREC_FP64 is implemented for the DFT product");
-    return -13;
+                          int r0, int c0, double beta, double* c, int ldc, int prec, int slices,
+                          void* stream) {
+  int st = 0;
+  if (m < 0) st = -1;
+  else if (n < 0) st = -2;
+  else if (k < 0) st = -3;
+  else if (nd < 1) st = -4;
+  else if (ldb < (k > 1 ? k : 1)) st = -7;
+  else if (r0 < 0) st = -8;
+  else if (c0 < 0) st = -9;
+  else if (ldc < (m > 1 ? m : 1)) st = -12;
+  else if (prec != PSMMA_PREC_FP64 && prec != PSMMA_PREC_SPLIT_FP32) st = -13;
+  else if (prec == PSMMA_PREC_SPLIT_FP32 && (slices < 0 || slices > 6)) st = -14;
+  else if (m > 0 && n > 0 && (c == nullptr || (k > 0 && b == nullptr))) st = -6;
+  if (st) {
+    set_err(st == -13 ? "pzmdft_dev: the DFT product takes PSMMA_PREC_FP64 or PSMMA_PREC_SPLIT_FP32"
+                      : "pzmdft_dev: invalid argument %d",
+            -st);
+    return st;
   }
   if (m == 0 || n == 0) return 0;
   hipStream_t s = (hipStream_t)stream;
   const double2* tw = twiddles(nd, s);
   if (!tw) return hip_fail(hipErrorOutOfMemory, "pzmdft_dev twiddle table");
+  if (prec == PSMMA_PREC_SPLIT_FP32) {
+    st = launch_dft_split_f32(m, n, k, nd, slices, alpha, tw, (const double2*)b, ldb, r0, c0, beta,
+                              (double2*)c, ldc, s);
+    if (st > 0) set_err("pzmdft_dev: runtime error %d", st);
+    return st;
+  }
   hipError_t e = launch_dft_f64(m, n, k, nd, alpha, tw, (const double2*)b, ldb, r0, c0, beta,
                                 (double2*)c, ldc, s);
   if (e != hipSuccess) return hip_fail(e, "launch_dft_f64");
@@ -325,7 +341,8 @@ extern "C" const char* psmma_version(void) { return "psmma-hip 0.1 (gfx950)"; }
 extern "C" const char* psmma_last_error(void) { return g_err.c_str(); }
 extern "C" int psmma_last_status(void) { return g_status; }
 extern "C" int psmma_set_precision(int prec, int slices) {
-  if (prec != PSMMA_PREC_FP64 && prec != PSMMA_PREC_SPLIT_I8 && prec != PSMMA_PREC_SPLIT_BF16)
+  if (prec != PSMMA_PREC_FP64 && prec != PSMMA_PREC_SPLIT_I8 && prec != PSMMA_PREC_SPLIT_BF16 &&
+      prec != PSMMA_PREC_SPLIT_FP32)
     return -1;
   if (prec != PSMMA_PREC_FP64 && (slices < 0 || slices > 16)) return -2;
   g_prec = prec;
@@ -341,6 +358,9 @@ extern "C" int psmma_set_precision(int prec, int slices) {
 // them as intersecting, so the step is the dense branch: generate the full
 // K x K tile and DGEMM (SRC/pscauchy_compute.f90:391-407 and siblings).  That
 // step is exactly what the kernels compute.
+
+// Numeric form of the real-kind drop-ins (the DFT's split-FP32 form does not apply to them).
+static int real_prec() { return g_prec == PSMMA_PREC_SPLIT_FP32 ? PSMMA_PREC_FP64 : g_prec; }
 
 namespace {
 
@@ -427,7 +447,7 @@ void compute_right(int kind, const char* name, int N, int K, double alpha, const
   if (kind == KIND_HANKEL && K > 0 && LDB < 2 * K - 1) return xerbla(name, 8);
   if (kind == KIND_CAUCHY && LDB < K) return xerbla(name, 8);
 
-  int prec = g_prec, slices = g_slices;
+  int prec = real_prec(), slices = g_slices;
   host_right(name, N, K, K, alpha, A, LDA, beta, C, LDC,
              [&](const double* dA, int lda, double* dC, int ldc, hipStream_t s) -> int {
                if (kind == KIND_CAUCHY) {
@@ -540,7 +560,7 @@ static void compute_eig(int N, int K, int NB, int KB, double alpha, const double
   if (LDA < N) return xerbla(name, 7);
   if (LDC < N) return xerbla(name, 17);
   if ((alpha == 0.0 || K == 0) && beta == 1.0) return;
-  int prec = g_prec, slices = g_slices;
+  int prec = real_prec(), slices = g_slices;
   host_right(name, N, K, K, alpha, A, LDA, beta, C, LDC,
              [&](const double* dA, int lda, double* dC, int ldc, hipStream_t s) -> int {
                double* dg = (double*)stage_buf(2, sizeof(double) * (size_t)K * 5);
@@ -629,7 +649,9 @@ static void entry_dft(const char* name, const int* n, const int* k, const int* n
     e = hipMemcpy2DAsync(dC, cplx * N, c, cplx * (*ldc), cplx * N, N, hipMemcpyHostToDevice, s);
   if (e != hipSuccess) return (void)hip_fail(e, name);
   // the staging lock stays held: dB and dC are in use until the copy back ends
-  int st = pzmdft_dev(N, N, K, N, *alpha, dB, K, 0, 0, *beta, dC, N, PSMMA_PREC_FP64, s);
+  const int prec = g_prec == PSMMA_PREC_SPLIT_FP32 ? PSMMA_PREC_SPLIT_FP32 : PSMMA_PREC_FP64;
+  const int slices = (prec == PSMMA_PREC_SPLIT_FP32 && g_slices <= 6) ? g_slices : 0;
+  int st = pzmdft_dev(N, N, K, N, *alpha, dB, K, 0, 0, *beta, dC, N, prec, slices, s);
   if (st != 0) {
     if (g_status == 0) g_status = -3;
     fprintf(stderr, "psmma: %s: %s\n", name, g_err.c_str());

@@ -5,7 +5,7 @@ module psmma_hip
   use iso_c_binding
   implicit none
   integer(c_int), parameter :: PSMMA_PREC_FP64 = 0, PSMMA_PREC_SPLIT_I8 = 1, &
-                               PSMMA_PREC_SPLIT_BF16 = 2
+                               PSMMA_PREC_SPLIT_BF16 = 2, PSMMA_PREC_SPLIT_FP32 = 3
   integer(c_int), parameter :: PSMMA_BLR_RRQR = 0, PSMMA_BLR_RRLU = 1
   interface
     integer(c_int) function psmma_toeplitz_dev(m, n, k, alpha, a, lda, g, lg, korder, r0, c0, &
@@ -39,9 +39,9 @@ module psmma_hip
       type(c_ptr), value :: a, fpole, u, v, difl, difr, c, stream
     end function
     integer(c_int) function pzmdft_dev(m, n, k, nd, alpha, b, ldb, r0, c0, beta, c, ldc, prec, &
-                                       stream) bind(C, name="pzmdft_dev")
+                                       slices, stream) bind(C, name="pzmdft_dev")
       import :: c_int, c_double, c_ptr
-      integer(c_int), value :: m, n, k, nd, ldb, r0, c0, ldc, prec
+      integer(c_int), value :: m, n, k, nd, ldb, r0, c0, ldc, prec, slices
       real(c_double), value :: alpha, beta
       type(c_ptr), value :: b, c, stream
     end function
