@@ -261,6 +261,49 @@ int psmma_batched_plan_info(void* plan, int* tile_cfg, int* nodes);
 int psmma_batched_plan_destroy(void* plan);
 
 /* ===========================================================================
+ * 3b. The reference's P x Q process grid, one rank's share at a time.
+ *
+ * psmma_constindex: ConstIndex (SRC/Auxil_2p5D.f90:2-66), the 1-based global
+ * indices owned by process column `mycol` of `npcol` in a block-cyclic layout
+ * of n with block nb (index may be NULL to get the count).  Returns the number
+ * of indices, or -1 for invalid arguments.
+ *
+ * psmma_is_intersect: is_intersect (SRC/Auxil_2p5D.f90:752-792) with the
+ * reference's sense: 1 when the two increasing index lists share NO element
+ * (the ring step is then off-diagonal: PS*_COMPUTE's low-rank branch), else 0.
+ *
+ * psmma_ring_step_dev: one ring step of PS*_COMPUTE (the dense branch of
+ * SRC/pscauchy_compute.f90:391-407, pstoeplitz_compute.f90:290-293,
+ * pshankel_compute.f90:249-250, pscauchyeig_compute.f90:399-402):
+ *   C(m x n) = alpha * A(m x k) * H(rindex, cindex) + beta * C,
+ * with H the global structured matrix of problem p (kind, generators, lg,
+ * korder as in psmma_problem; r0, c0 ignored) and rindex (k entries), cindex
+ * (n entries) the 1-based global rows and columns of the step's tile, HOST
+ * arrays (as ConstIndex fills them; uploaded on `stream`).  Every index is
+ * checked against the generators first: -2 / -3 for a row / column out of
+ * range, -1 for other invalid fields.  Asynchronous; returns 0 or > 0 on a
+ * runtime error.
+ *
+ * psmma_redistribute_dev: PDGEMR2D between two block-cyclic layouts of an m x n
+ * matrix on an nprow x npcol grid (RSRC = CSRC = 0), e.g. the NB-cyclic and
+ * the ceil(N/P)-block layouts of REDIST (SRC/psmma_cauchy.f90:113-162), for the
+ * whole grid at once: a holds the P*Q source local arrays back to back in rank
+ * order r = prow * npcol + pcol, each column-major with leading dimension its
+ * local row count, at the offsets psmma_local_offsets gives; b receives the
+ * destination local arrays likewise.  A device permutation (each element read
+ * once and written once); asynchronous.  Returns 0 or -(argument number).
+ * psmma_local_offsets writes the P*Q + 1 offsets (elements) of such a layout
+ * and returns the total element count (-1 for invalid arguments). */
+int psmma_constindex(int n, int nb, int mycol, int npcol, int* index);
+int psmma_is_intersect(int m, const int* seta, int n, const int* setb);
+int psmma_ring_step_dev(const psmma_problem* p, const int* rindex, const int* cindex,
+                        void* stream);
+long long psmma_local_offsets(int m, int n, int nprow, int npcol, int mb, int nb,
+                              long long* offsets);
+int psmma_redistribute_dev(int m, int n, int nprow, int npcol, int src_mb, int src_nb,
+                           const double* a, int dst_mb, int dst_nb, double* b, void* stream);
+
+/* ===========================================================================
  * 4. Generators of the eigen-Cauchy operand (device pointers).
  *
  * Replaces the merge step that produces psmma_cauchyeig's inputs

@@ -87,6 +87,11 @@ _SIGS = {
     "psmma_batched_plan_launch": ([_P, _P], _I),
     "psmma_batched_plan_info": ([_P, _PI, _PI], _I),
     "psmma_batched_plan_destroy": ([_P], _I),
+    "psmma_constindex": ([_I, _I, _I, _I, _P], _I),
+    "psmma_is_intersect": ([_I, _P, _I, _P], _I),
+    "psmma_ring_step_dev": ([_P, _P, _P, _P], _I),
+    "psmma_local_offsets": ([_I, _I, _I, _I, _I, _I, _P], ctypes.c_longlong),
+    "psmma_redistribute_dev": ([_I, _I, _I, _I, _I, _I, _P, _I, _I, _P, _P], _I),
 }
 
 
@@ -601,3 +606,55 @@ class BatchedPlan:
             self.close()
         except Exception:
             pass
+
+
+# ---------------------------------------------------------------------------
+# The reference's P x Q grid, one rank's share at a time (include/psmma.h 3b).
+
+def constindex(n: int, nb: int, mycol: int, npcol: int) -> np.ndarray:
+    """ConstIndex (SRC/Auxil_2p5D.f90:2-66): 1-based global indices of process column mycol."""
+    cnt = lib().psmma_constindex(n, nb, mycol, npcol, None)
+    if cnt < 0:
+        raise ValueError(f"constindex({n}, {nb}, {mycol}, {npcol}): invalid arguments")
+    out = np.zeros(max(cnt, 1), dtype=np.int32)
+    lib().psmma_constindex(n, nb, mycol, npcol, out.ctypes.data_as(ctypes.c_void_p))
+    return out[:cnt]
+
+
+def is_intersect(a, b) -> bool:
+    """is_intersect (SRC/Auxil_2p5D.f90:752-792), the reference's sense: True iff disjoint."""
+    a = np.ascontiguousarray(a, dtype=np.int32)
+    b = np.ascontiguousarray(b, dtype=np.int32)
+    return bool(lib().psmma_is_intersect(a.size, a.ctypes.data_as(ctypes.c_void_p), b.size,
+                                         b.ctypes.data_as(ctypes.c_void_p)))
+
+
+def ring_step_dev(problem: "Problem", rindex, cindex, stream=None):
+    """One ring step C = alpha * A * H(rindex, cindex) + beta * C (problem.r0 / c0 ignored);
+    rindex, cindex: 1-based global indices (host arrays)."""
+    r = np.ascontiguousarray(rindex, dtype=np.int32)
+    c = np.ascontiguousarray(cindex, dtype=np.int32)
+    if r.size != problem.k or c.size != problem.n:
+        raise ValueError("rindex / cindex lengths must equal the step's k / n")
+    _check(lib().psmma_ring_step_dev(ctypes.byref(problem), r.ctypes.data_as(ctypes.c_void_p),
+                                     c.ctypes.data_as(ctypes.c_void_p), _stream_ptr(stream)),
+           "psmma_ring_step_dev")
+
+
+def local_offsets(m, n, nprow, npcol, mb, nb) -> np.ndarray:
+    """Offsets (elements) of the P*Q local arrays of a block-cyclic layout, rank-major."""
+    out = np.zeros(nprow * npcol + 1, dtype=np.int64)
+    tot = lib().psmma_local_offsets(m, n, nprow, npcol, mb, nb, out.ctypes.data_as(ctypes.c_void_p))
+    if tot < 0:
+        raise ValueError("local_offsets: invalid arguments")
+    return out
+
+
+def redistribute_dev(m, n, nprow, npcol, src_mb, src_nb, a, dst_mb, dst_nb, b, stream=None):
+    """PDGEMR2D between two block-cyclic layouts of one grid; a, b: 1-D float64 device tensors
+    holding all ranks' local arrays (local_offsets order)."""
+    na = int(local_offsets(m, n, nprow, npcol, src_mb, src_nb)[-1])
+    nb_ = int(local_offsets(m, n, nprow, npcol, dst_mb, dst_nb)[-1])
+    _check(lib().psmma_redistribute_dev(m, n, nprow, npcol, src_mb, src_nb, _vec(a, "a", na),
+                                        dst_mb, dst_nb, _vec(b, "b", nb_), _stream_ptr(stream)),
+           "psmma_redistribute_dev")

@@ -41,6 +41,10 @@ struct GenPtrs {
   int korder;  // order K of the Toeplitz matrix (index shift); unused otherwise
   int nrows;   // global row count of H (bounds for generator reads)
   int ncols;   // global column count of H
+  // Index-list blocks (ring steps): 1-based global rows / columns of the block,
+  // as ConstIndex lists them; null for contiguous blocks at (r0, c0).
+  const int* ridx = nullptr;
+  const int* cidx = nullptr;
 };
 
 // Entry H(r, c) of the structured matrix, 0-based global indices.

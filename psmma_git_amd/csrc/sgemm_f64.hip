@@ -32,18 +32,20 @@ namespace psmma {
 // (The k-slot assignment only has to agree between A and B.)
 
 // LDS footprint (doubles) of one right-structured tile configuration.
-template <int KIND, int BM, int BN, int BK>
+template <int KIND, int BM, int BN, int BK, bool IDX = false>
 struct RightLds {
-  static constexpr bool WIN = (KIND == KIND_TOEPLITZ || KIND == KIND_HANKEL);
+  static constexpr bool WIN = (KIND == KIND_TOEPLITZ || KIND == KIND_HANKEL) && !IDX;
   static constexpr int ALD = BM + 16, HLD = BN + 16;
   static constexpr int ASZ = BK * ALD;
   static constexpr int HSZ = WIN ? BK + BN : BK * HLD;
 };
 
-// One BM x BN output tile at (m0, n0) of C = alpha * A * H(r0:, c0:) + beta * C.
+// One BM x BN output tile at (m0, n0) of C = alpha * A * H(r0:, c0:) + beta * C,
+// or with IDX of C = alpha * A * H(ridx, cidx) + beta * C (the rows and columns of
+// H given by the 1-based index lists of g; r0, c0 unused).
 // As/Hs: LDS areas of 2 * RightLds::ASZ and 2 * RightLds::HSZ doubles.  Ends
 // with all waves past a barrier, so the caller may reuse the LDS at once.
-template <int KIND, int BM, int BN, int BK, int WGM, int WGN>
+template <int KIND, int BM, int BN, int BK, int WGM, int WGN, bool IDX = false>
 __device__ __forceinline__ void right_tile(int M, int N, int K, double alpha,
                                            const double* __restrict__ A, int lda, const GenPtrs& g,
                                            int r0, int c0, double beta, double* __restrict__ C,
@@ -55,13 +57,15 @@ __device__ __forceinline__ void right_tile(int M, int N, int K, double alpha,
   constexpr int ALD = BM + 16, HLD = BN + 16;
   constexpr int AREG = BM * BK / NT;  // dense elements staged per thread per slab
   // Toeplitz and Hankel tiles are functions of (col - row) / (row + col): one
-  // slab needs only a window of BK+BN-1 generator values.
-  constexpr bool WIN = (KIND == KIND_TOEPLITZ || KIND == KIND_HANKEL);
+  // slab of a contiguous block needs only a window of BK+BN-1 generator values.
+  // Index-list blocks generate every entry, as the Cauchy kinds do.
+  constexpr bool WIN = (KIND == KIND_TOEPLITZ || KIND == KIND_HANKEL) && !IDX;
+  constexpr bool GATHER = (KIND == KIND_TOEPLITZ || KIND == KIND_HANKEL) && IDX;
   constexpr int WLEN = BK + BN - 1;
   constexpr int WREG = (WLEN + NT - 1) / NT;
   constexpr int HREG = WIN ? 1 : BK * BN / NT;  // generated entries per thread per slab
-  constexpr int ASZ = RightLds<KIND, BM, BN, BK>::ASZ;
-  constexpr int HSZ = RightLds<KIND, BM, BN, BK>::HSZ;
+  constexpr int ASZ = RightLds<KIND, BM, BN, BK, IDX>::ASZ;
+  constexpr int HSZ = RightLds<KIND, BM, BN, BK, IDX>::HSZ;
   static_assert(BM * BK % NT == 0 && BK * BN % NT == 0, "tile/threads mismatch");
   static_assert(WIN || NT % BN == 0, "generated tiles assume a fixed column per thread");
   static_assert(WM % 16 == 0 && WN % 16 == 0 && BK % 4 == 0, "bad wave tile");
@@ -81,6 +85,7 @@ __device__ __forceinline__ void right_tile(int M, int N, int K, double alpha,
   double wreg[WREG];     // window values (Toeplitz/Hankel)
   double rowa[HREG], rowb[HREG];  // row generators (Cauchy kinds)
   double hreg[HREG];
+  int growv[HREG];  // global rows of the slab (eigen-Cauchy index lists)
 
   // Generator index range of the whole block: window slots outside it are only
   // used by padded (row, col) pairs and are set to 0 so 0 * value stays 0.
@@ -96,8 +101,9 @@ __device__ __forceinline__ void right_tile(int M, int N, int K, double alpha,
   // Column invariants of the Cauchy kinds: this thread always generates column
   // hcol of every slab.
   const int hcol = tid % BN;
-  const int gcol = c0 + n0 + hcol;
   const bool colok = (n0 + hcol) < N;
+  int gcol = c0 + n0 + hcol;
+  if constexpr (IDX) gcol = colok ? g.cidx[n0 + hcol] - 1 : 0;
   double cv0 = 0.0, cv1 = 0.0, cv2 = 0.0, cv3 = 0.0, cv4 = 0.0;
   if constexpr (KIND == KIND_CAUCHY) {
     if (colok) {
@@ -143,14 +149,21 @@ __device__ __forceinline__ void right_tile(int M, int N, int K, double alpha,
         int row = tid / BN + q * (NT / BN);
         int gk = k0 + row;
         if (gk < K) {
-          if constexpr (KIND == KIND_CAUCHY) {
-            rowa[q] = g.p0[r0 + gk];  // D_r
-            rowb[q] = g.p2[r0 + gk];  // U_r
+          int gr = r0 + gk;
+          if constexpr (IDX) gr = g.ridx[gk] - 1;
+          if constexpr (GATHER) {
+            // index-list Toeplitz / Hankel entries are gathered straight from g
+            hreg[q] = colok ? gen_entry<KIND>(g, gr, gcol) : 0.0;
+          } else if constexpr (KIND == KIND_CAUCHY) {
+            rowa[q] = g.p0[gr];  // D_r
+            rowb[q] = g.p2[gr];  // U_r
           } else {
-            rowa[q] = g.p0[r0 + gk];  // F_r
-            rowb[q] = g.p1[r0 + gk];  // U_r
+            rowa[q] = g.p0[gr];  // F_r
+            rowb[q] = g.p1[gr];  // U_r
           }
+          if constexpr (KIND == KIND_CAUCHYEIG) growv[q] = gr;
         } else {
+          if constexpr (GATHER) hreg[q] = 0.0;
           rowa[q] = 0.0;
           rowb[q] = 0.0;
         }
@@ -159,7 +172,7 @@ __device__ __forceinline__ void right_tile(int M, int N, int K, double alpha,
   };
   // Evaluate the prefetched slab (the FP64 divisions of the Cauchy kinds).
   auto finish_h = [&](int k0) {
-    if constexpr (!WIN) {
+    if constexpr (!WIN && !GATHER) {
 #pragma unroll
       for (int q = 0; q < HREG; ++q) {
         int row = tid / BN + q * (NT / BN);
@@ -170,7 +183,7 @@ __device__ __forceinline__ void right_tile(int M, int N, int K, double alpha,
             v = (rowb[q] * cv1) / (rowa[q] - cv0);
           } else {
             // SRC/CauchyHssEig.f90:1764-1768: d_r - lambda_c from stored differences
-            int gr = r0 + gk;
+            int gr = IDX ? growv[q] : r0 + gk;
             double den = (gr <= gcol) ? ((rowa[q] - cv0) - cv3) : ((rowa[q] - cv1) - cv4);
             v = (rowb[q] * cv2) / den;
           }
@@ -223,9 +236,9 @@ __device__ __forceinline__ void right_tile(int M, int N, int K, double alpha,
 #pragma unroll
       for (int j = 0; j < TN; ++j) {
         int jj = wn0 + j * 16 + lr, kr = kk + lk;
-        if constexpr (KIND == KIND_TOEPLITZ)
+        if constexpr (WIN && KIND == KIND_TOEPLITZ)
           bf[j] = hs[(BK - 1) - kr + jj];
-        else if constexpr (KIND == KIND_HANKEL)
+        else if constexpr (WIN && KIND == KIND_HANKEL)
           bf[j] = hs[kr + jj];
         else
           bf[j] = hs[kr * HLD + jj];
@@ -294,6 +307,25 @@ __global__ __launch_bounds__(WGM* WGN * 64, MINB) void sgemm_f64_right_kernel(
   const int bid = xcd_remap(blockIdx.x, nbn * nbm);
   right_tile<KIND, BM, BN, BK, WGM, WGN>(M, N, K, alpha, A, lda, g, r0, c0, beta, C, ldc,
                                          (bid / nbn) * BM, (bid % nbn) * BN, As, Hs);
+}
+
+// One ring step of PS*_COMPUTE on the rank's own index sets: C = alpha * A *
+// H(RIndex, CIndex) + beta * C with the 1-based global index lists of g (ConstIndex,
+// SRC/Auxil_2p5D.f90:2-66, consumed by ConstCauchy / ConstToeplitz / ConstHankel /
+// ConstCauchyEig, SRC/Auxil_2p5D.f90:68-322, :532-637).  The tile body of the
+// contiguous kernel with every entry of the slab generated through the lists.
+template <int KIND, int BM, int BN, int BK, int WGM, int WGN>
+__global__ __launch_bounds__(WGM* WGN * 64, 2) void sgemm_f64_indexed_kernel(
+    int M, int N, int K, double alpha, const double* __restrict__ A, int lda, GenPtrs g,
+    double beta, double* __restrict__ C, int ldc) {
+  using L = RightLds<KIND, BM, BN, BK, true>;
+  __shared__ double As[2 * L::ASZ];
+  __shared__ double Hs[2 * L::HSZ];
+  const int nbn = (N + BN - 1) / BN;
+  const int nbm = (M + BM - 1) / BM;
+  const int bid = xcd_remap(blockIdx.x, nbn * nbm);
+  right_tile<KIND, BM, BN, BK, WGM, WGN, true>(M, N, K, alpha, A, lda, g, 0, 0, beta, C, ldc,
+                                               (bid / nbn) * BM, (bid % nbn) * BN, As, Hs);
 }
 
 // ---------------------------------------------------------------------------
@@ -649,6 +681,36 @@ hipError_t launch_right_f64(int kind, int cfg, int M, int N, int K, double alpha
     case KIND_CAUCHYEIG:
       return launch_right_kind<KIND_CAUCHYEIG>(cfg, M, N, K, alpha, A, lda, g, r0, c0, beta, C, ldc,
                                                s);
+  }
+  return hipErrorInvalidValue;
+}
+
+template <int KIND>
+static hipError_t launch_indexed_kind(int M, int N, int K, double alpha, const double* A, int lda,
+                                      const GenPtrs& g, double beta, double* C, int ldc,
+                                      hipStream_t s) {
+  const long long big = (long long)((M + 127) / 128) * ((N + 127) / 128);
+  if (big >= 256) {  // 128 x 128 tiles, 4 waves, once they fill the chip
+    int nb = ((M + 127) / 128) * ((N + 127) / 128);
+    hipLaunchKernelGGL((sgemm_f64_indexed_kernel<KIND, 128, 128, 16, 2, 2>), dim3(nb), dim3(256),
+                       0, s, M, N, K, alpha, A, lda, g, beta, C, ldc);
+  } else {
+    int nb = ((M + 63) / 64) * ((N + 63) / 64);
+    hipLaunchKernelGGL((sgemm_f64_indexed_kernel<KIND, 64, 64, 16, 2, 2>), dim3(nb), dim3(256), 0,
+                       s, M, N, K, alpha, A, lda, g, beta, C, ldc);
+  }
+  return hipGetLastError();
+}
+
+// C = alpha * A * H(ridx, cidx) + beta * C on the FP64 kernel (g.ridx / g.cidx set).
+hipError_t launch_indexed_f64(int kind, int M, int N, int K, double alpha, const double* A,
+                              int lda, const GenPtrs& g, double beta, double* C, int ldc,
+                              hipStream_t s) {
+  switch (kind) {
+    case KIND_CAUCHY: return launch_indexed_kind<KIND_CAUCHY>(M, N, K, alpha, A, lda, g, beta, C, ldc, s);
+    case KIND_TOEPLITZ: return launch_indexed_kind<KIND_TOEPLITZ>(M, N, K, alpha, A, lda, g, beta, C, ldc, s);
+    case KIND_HANKEL: return launch_indexed_kind<KIND_HANKEL>(M, N, K, alpha, A, lda, g, beta, C, ldc, s);
+    case KIND_CAUCHYEIG: return launch_indexed_kind<KIND_CAUCHYEIG>(M, N, K, alpha, A, lda, g, beta, C, ldc, s);
   }
   return hipErrorInvalidValue;
 }

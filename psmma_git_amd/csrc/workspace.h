@@ -11,6 +11,7 @@ enum WsSlot : int {
   WS_SPLIT = 0,     // split-precision products (single and batched eager calls)
   WS_BLR_FIXED = 1, // compressed products: block lists and compressor outputs
   WS_BLR_GEMM = 2,  // compressed products: rank-dependent GEMM operands
+  WS_RING = 3,      // ring steps and redistributions: index lists, offset tables
 };
 
 // Returns a buffer of at least `bytes` for (current device, s, slot), or nullptr
