@@ -240,7 +240,8 @@ typedef struct psmma_problem {
 } psmma_problem;
 
 /* All problems in one go on `stream`.  prec: PSMMA_PREC_FP64 (one persistent
- * grouped launch over the tiles of every problem) or PSMMA_PREC_SPLIT_I8 (each
+ * launch over the tiles of every problem, all kinds and tile classes, longest
+ * tiles first, taken from an atomic work counter) or PSMMA_PREC_SPLIT_I8 (each
  * phase of the int8 modular pipeline is one grouped launch over all problems;
  * slices = number of moduli, 0 = the fewest that keep 45 bits per operand for
  * the longest contraction of the batch).  The problem table is copied to the
@@ -249,9 +250,11 @@ typedef struct psmma_problem {
 int psmma_batched_dev(int nprob, const psmma_problem* probs, int prec, int slices, void* stream);
 
 /* A reusable plan: validates the problems, uploads the table, picks the tile
- * shape (FP64: both grouped shapes are timed on the batch when every beta is 0),
- * and captures the whole batch in a hipGraph.  Launching the plan replays the
- * graph on `stream`.  No allocation happens at launch time. */
+ * classes (FP64: the size from which problems take the large tiles is timed on
+ * the batch when every beta is 0), and captures the whole batch in a hipGraph
+ * (FP64: one kernel node).  Launching the plan replays the graph on `stream`.
+ * No allocation happens at launch time.  A plan's launches must not overlap
+ * each other (they share its work counters and workspace). */
 int psmma_batched_plan_create(int nprob, const psmma_problem* probs, int prec, int slices,
                               void** plan);
 int psmma_batched_plan_launch(void* plan, void* stream);

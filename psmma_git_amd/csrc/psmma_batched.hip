@@ -12,9 +12,9 @@
 #include "workspace.h"
 
 namespace psmma {
-hipError_t launch_grouped_f64(int kind, int cfg, const BatchProblem* dprobs, const int* dtbeg,
-                              int nprob, int ntiles, int nworkgroups, hipStream_t s);
-void grouped_tile_shape(int kind, int cfg, int* bm, int* bn);
+hipError_t launch_batch_f64(const BatchProblem* dprobs, const int* dtbeg, int nprob, int ntiles,
+                            unsigned* ctr, int nworkgroups, hipStream_t s);
+void batch_tile_shape(int kind, int cls, int* bm, int* bn);
 void* gsplit_create(const BatchProblem* probs, int nprob, int t, size_t* ws_bytes);
 int gsplit_bind(void* h, void* ws);
 int gsplit_enqueue(void* h, hipStream_t s, int* nodes);
@@ -66,81 +66,79 @@ int device_workgroups(int per_cu) {
   return cus * per_cu;
 }
 
-// FP64 batch layout: problems split into groups by (kind, tile class); each
-// group is one persistent grouped launch with its own compact tile index.
+// FP64 batch layout: one tile list over every problem (all kinds, both tile
+// classes), problems ordered by the cost of one tile (K x tile area, longest
+// first), each problem's tiles contiguous; one persistent launch runs it.
 // A problem uses the large tiles when min(m, n) >= big_min.
-struct F64Group {
-  int kind, cfg, first, count, ntiles;
-};
 struct F64Table {
   std::vector<BatchProblem> probs;
   std::vector<int> tbeg;
-  std::vector<F64Group> groups;
   int ntiles = 0;
 };
 struct F64Dev {
   BatchProblem* probs = nullptr;
   int* tbeg = nullptr;
+  unsigned* ctr = nullptr;  // work counter and exit counter of the persistent launch
 };
 
 void build_f64(const std::vector<BatchProblem>& in, int big_min, F64Table& T) {
   T = F64Table{};
-  for (int kind = PSMMA_KIND_CAUCHY; kind <= PSMMA_KIND_CAUCHYEIG; ++kind) {
-    for (int cfg = 1; cfg >= 0; --cfg) {
-      std::vector<BatchProblem> g;
-      for (const auto& b : in) {
-        if (b.kind != kind || b.m == 0 || b.n == 0) continue;
-        bool big = std::min(b.m, b.n) >= big_min;
-        if (big == (cfg == 1)) g.push_back(b);
-      }
-      if (g.empty()) continue;
-      std::stable_sort(g.begin(), g.end(),
-                       [](const BatchProblem& x, const BatchProblem& y) { return x.k > y.k; });
-      int bm, bn;
-      grouped_tile_shape(kind, cfg, &bm, &bn);
-      F64Group G{kind, cfg, (int)T.probs.size(), (int)g.size(), 0};
-      for (auto& b : g) {
-        b.ntm = (b.m + bm - 1) / bm;
-        b.ntn = (b.n + bn - 1) / bn;
-        b.tile_begin = G.ntiles;
-        T.probs.push_back(b);
-        T.tbeg.push_back(G.ntiles);
-        G.ntiles += b.ntm * b.ntn;
-      }
-      T.ntiles += G.ntiles;
-      T.groups.push_back(G);
-    }
+  std::vector<BatchProblem> all;
+  for (auto b : in) {
+    if (b.m == 0 || b.n == 0) continue;
+    b.cls = std::min(b.m, b.n) >= big_min ? 1 : 0;
+    int bm, bn;
+    batch_tile_shape(b.kind, b.cls, &bm, &bn);
+    b.ntm = (b.m + bm - 1) / bm;
+    b.ntn = (b.n + bn - 1) / bn;
+    all.push_back(b);
+  }
+  auto cost = [](const BatchProblem& b) {
+    int bm, bn;
+    batch_tile_shape(b.kind, b.cls, &bm, &bn);
+    return (long long)std::max(b.k, 1) * bm * bn;
+  };
+  std::stable_sort(all.begin(), all.end(), [&](const BatchProblem& x, const BatchProblem& y) {
+    return cost(x) > cost(y);
+  });
+  for (auto& b : all) {
+    b.tile_begin = T.ntiles;
+    T.probs.push_back(b);
+    T.tbeg.push_back(T.ntiles);
+    T.ntiles += b.ntm * b.ntn;
   }
 }
 
 void free_f64(F64Dev& D) {
   if (D.probs) hipFree(D.probs);
   if (D.tbeg) hipFree(D.tbeg);
+  if (D.ctr) hipFree(D.ctr);
   D = F64Dev{};
 }
 
 int upload_f64(const F64Table& T, F64Dev& D) {
   if (T.probs.empty()) return 0;
   if (hipMalloc(&D.probs, sizeof(BatchProblem) * T.probs.size()) != hipSuccess ||
-      hipMalloc(&D.tbeg, sizeof(int) * T.tbeg.size()) != hipSuccess)
+      hipMalloc(&D.tbeg, sizeof(int) * T.tbeg.size()) != hipSuccess ||
+      hipMalloc(&D.ctr, sizeof(unsigned) * 2) != hipSuccess)
     return 1002;
   if (hipMemcpy(D.probs, T.probs.data(), sizeof(BatchProblem) * T.probs.size(),
                 hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(D.tbeg, T.tbeg.data(), sizeof(int) * T.tbeg.size(), hipMemcpyHostToDevice) !=
-          hipSuccess)
+          hipSuccess ||
+      hipMemset(D.ctr, 0, sizeof(unsigned) * 2) != hipSuccess)
     return 1003;
   return 0;
 }
 
-// Enqueues one grouped launch per group on stream s; *nodes = launches made.
+// Enqueues the batch's one persistent launch on stream s; *nodes = launches made.
 int enqueue_f64(const F64Table& T, const F64Dev& D, hipStream_t s, int* nodes) {
   int n = 0;
-  for (const auto& G : T.groups) {
-    if (G.ntiles == 0) continue;
-    hipError_t e = launch_grouped_f64(G.kind, G.cfg, D.probs + G.first, D.tbeg + G.first, G.count,
-                                      G.ntiles, device_workgroups(2), s);
+  if (T.ntiles > 0) {
+    hipError_t e = launch_batch_f64(D.probs, D.tbeg, (int)T.probs.size(), T.ntiles, D.ctr,
+                                    device_workgroups(2), s);
     if (e != hipSuccess) return 1000 + (int)e;
-    ++n;
+    n = 1;
   }
   if (nodes) *nodes = n;
   return 0;

@@ -138,6 +138,7 @@ struct BatchProblem {
   int ldc;
   int tile_begin;  // first index of this problem's tiles in the batch tile list
   int ntm, ntn;    // tiles along m and n
+  int cls;         // tile class: 0 small (64 x 64), 1 large (shape by kind)
 };
 
 }  // namespace psmma

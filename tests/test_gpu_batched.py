@@ -81,7 +81,7 @@ def test_batched_fp64_plan_replays(gpu):
     plan = api.BatchedPlan(probs)
     cfg, nodes = plan.info()
     print("plan cfg", cfg, "nodes", nodes)
-    assert 1 <= nodes <= 8
+    assert nodes == 1  # one persistent launch over every problem of every kind
     for rep in range(3):
         for dc, _, _, _ in keep:
             dc.fill_(float("nan"))  # eager work on the same stream between replays
