@@ -307,6 +307,29 @@ int psmma_redistribute_dev(int m, int n, int nprow, int npcol, int src_mb, int s
                            const double* a, int dst_mb, int dst_nb, double* b, void* stream);
 
 /* ===========================================================================
+ * 3c. Autotuner.
+ *
+ * psmma_autotune_f64: times every tile configuration of the FP64 structured
+ * kernel on an m x n x k product of `kind` (synthetic operands in memory it
+ * allocates and frees; the device is synchronised) and caches the fastest for
+ * the (device, kind, ceil log2 of m, n, k) bucket; every later FP64 launch of
+ * that bucket uses it.  With PSMMA_AUTOTUNE=1 in the environment a bucket is
+ * tuned at its first launch (which then allocates and synchronises).  Returns
+ * 0 and the configuration / its time, or -(argument number), > 0 on errors.
+ *
+ * psmma_split_slices_dev: the fewest int8 moduli (prec PSMMA_PREC_SPLIT_I8) or
+ * bf16 slices (PSMMA_PREC_SPLIT_BF16) whose relative Frobenius error, measured
+ * on a block of up to 64 of problem p's own columns against the FP64 kernel,
+ * is at most `target`; p's output is not touched (the sample goes to the
+ * stream's workspace).  Synchronises `stream`.  Returns 0, 1 when no depth
+ * reaches the target (the largest is returned), -(argument number).  Batched
+ * split plans created with slices = 0 take the largest depth this chooses at
+ * 1e-12 over the longest-K problem of each kind. */
+int psmma_autotune_f64(int kind, int m, int n, int k, int* cfg, float* ms);
+int psmma_split_slices_dev(const psmma_problem* p, int prec, double target, int* slices,
+                           double* err, void* stream);
+
+/* ===========================================================================
  * 4. Generators of the eigen-Cauchy operand (device pointers).
  *
  * Replaces the merge step that produces psmma_cauchyeig's inputs

@@ -87,6 +87,8 @@ _SIGS = {
     "psmma_batched_plan_launch": ([_P, _P], _I),
     "psmma_batched_plan_info": ([_P, _PI, _PI], _I),
     "psmma_batched_plan_destroy": ([_P], _I),
+    "psmma_autotune_f64": ([_I, _I, _I, _I, _P, _P], _I),
+    "psmma_split_slices_dev": ([_P, _I, _D, _P, _P, _P], _I),
     "psmma_constindex": ([_I, _I, _I, _I, _P], _I),
     "psmma_is_intersect": ([_I, _P, _I, _P], _I),
     "psmma_ring_step_dev": ([_P, _P, _P, _P], _I),
@@ -658,3 +660,26 @@ def redistribute_dev(m, n, nprow, npcol, src_mb, src_nb, a, dst_mb, dst_nb, b, s
     _check(lib().psmma_redistribute_dev(m, n, nprow, npcol, src_mb, src_nb, _vec(a, "a", na),
                                         dst_mb, dst_nb, _vec(b, "b", nb_), _stream_ptr(stream)),
            "psmma_redistribute_dev")
+
+
+# ---------------------------------------------------------------------------
+# Autotuner (include/psmma.h 3c).
+
+def autotune_f64(kind: int, m: int, n: int, k: int):
+    """Times the FP64 tile configurations for this (kind, size bucket) and caches the fastest.
+    Returns (cfg, ms)."""
+    cfg, ms = ctypes.c_int(-1), ctypes.c_float(0.0)
+    _check(lib().psmma_autotune_f64(kind, m, n, k, ctypes.byref(cfg), ctypes.byref(ms)),
+           "psmma_autotune_f64")
+    return cfg.value, ms.value
+
+
+def split_slices_dev(problem: "Problem", prec=PREC_SPLIT_I8, target=1e-12, stream=None):
+    """Fewest moduli / slices whose error on a sampled column block is <= target.
+    Returns (slices, measured error, reached)."""
+    t, err = ctypes.c_int(0), ctypes.c_double(0.0)
+    st = lib().psmma_split_slices_dev(ctypes.byref(problem), prec, float(target), ctypes.byref(t),
+                                      ctypes.byref(err), _stream_ptr(stream))
+    if st < 0 or st > 1:
+        raise PsmmaError(f"psmma_split_slices_dev returned {st}")
+    return t.value, err.value, st == 0

@@ -30,6 +30,9 @@ namespace {
 // N = 8192 bench shape; two more bits cover wider-range problems of a batch.
 constexpr int kSplitBits = 45;
 
+}  // namespace
+
+namespace psmma {
 // Validates one problem (same rules as the single-problem device calls) and
 // converts it to the kernel's descriptor.  Returns 0 or -(field index + 1).
 int to_batch(const psmma_problem& q, BatchProblem& b) {
@@ -58,6 +61,9 @@ int to_batch(const psmma_problem& q, BatchProblem& b) {
                    0, 0, 0};
   return 0;
 }
+}  // namespace psmma
+
+namespace {
 
 int device_workgroups(int per_cu) {
   int dev = 0, cus = 256;
@@ -193,6 +199,29 @@ int check_prec(int prec, int slices) {
   return 0;
 }
 
+// Split depth of a plan created with slices = 0: the largest depth the
+// autotuner measures at 1e-12 relative error (psmma_split_slices_dev) over the
+// longest-K problem of each kind; 0 when a measurement fails.
+int measured_moduli(int nprob, const psmma_problem* probs, hipStream_t s) {
+  int pick[4] = {-1, -1, -1, -1};
+  for (int i = 0; i < nprob; ++i) {
+    const psmma_problem& q = probs[i];
+    if (q.m == 0 || q.n == 0 || q.k == 0) continue;
+    int& j = pick[q.kind];
+    if (j < 0 || q.k > probs[j].k || (q.k == probs[j].k && q.n > probs[j].n)) j = i;
+  }
+  int t = 0;
+  for (int kind = 0; kind < 4; ++kind) {
+    if (pick[kind] < 0) continue;
+    int tk = 0;
+    double err = 0.0;
+    if (psmma_split_slices_dev(&probs[pick[kind]], PSMMA_PREC_SPLIT_I8, 1e-12, &tk, &err, s) != 0)
+      return 0;
+    t = std::max(t, tk);
+  }
+  return t;
+}
+
 // The int8 form's int32 accumulators are exact for K <= PSMMA_SPLIT_I8_KMAX only.
 int check_split_k(int prec, const std::vector<BatchProblem>& in) {
   if (prec != PSMMA_PREC_SPLIT_I8) return 0;
@@ -290,7 +319,10 @@ extern "C" int psmma_batched_plan_create(int nprob, const psmma_problem* probs, 
   } else {
     // the plan owns its workspace and table: capture allocates nothing, and no
     // later call can move memory the graph points to
-    int t = slices ? slices : gsplit_moduli(in.data(), nprob, kSplitBits);
+    // measured on the default stream: its workspaces are reused by every plan
+    // (the capture stream is new per plan)
+    int t = slices ? slices : measured_moduli(nprob, probs, nullptr);
+    if (t <= 0) t = gsplit_moduli(in.data(), nprob, kSplitBits);
     size_t bytes = 0;
     P->gsplit = gsplit_create(in.data(), nprob, t, &bytes);
     P->slices = t;

@@ -680,14 +680,18 @@ static hipError_t launch_right_kind(int cfg, int M, int N, int K, double alpha, 
   }
 }
 
-// Picks a tile shape: the big tile when it still yields >= 2 workgroups per CU.
-// PSMMA_F64_CFG (0..6) forces one, for tuning sweeps.
-int choose_right_cfg(int kind, int M, int N) {
+int tuned_right_cfg(int kind, int M, int N, int K);
+
+// Picks a tile shape: PSMMA_F64_CFG (0..6) if set; the autotuner's choice for the
+// (kind, size bucket) if one is cached; else the measured rule below.
+int choose_right_cfg(int kind, int M, int N, int K) {
   static int forced = [] {
     const char* e = getenv("PSMMA_F64_CFG");
     return e ? atoi(e) : -1;
   }();
   if (forced >= 0) return forced;
+  const int tuned = tuned_right_cfg(kind, M, N, K);
+  if (tuned >= 0) return tuned;
   long long big = (long long)((M + 127) / 128) * ((N + 127) / 128);
   const bool win = kind == KIND_TOEPLITZ || kind == KIND_HANKEL;
   // Cauchy kinds (measured on MI355X, N x N x N products): the 8-wave 256x128 tile
@@ -704,7 +708,7 @@ int choose_right_cfg(int kind, int M, int N) {
 hipError_t launch_right_f64(int kind, int cfg, int M, int N, int K, double alpha, const double* A,
                             int lda, const GenPtrs& g, int r0, int c0, double beta, double* C,
                             int ldc, hipStream_t s) {
-  if (cfg < 0) cfg = choose_right_cfg(kind, M, N);
+  if (cfg < 0) cfg = choose_right_cfg(kind, M, N, K);
   switch (kind) {
     case KIND_CAUCHY:
       return launch_right_kind<KIND_CAUCHY>(cfg, M, N, K, alpha, A, lda, g, r0, c0, beta, C, ldc, s);

@@ -12,6 +12,7 @@ enum WsSlot : int {
   WS_BLR_FIXED = 1, // compressed products: block lists and compressor outputs
   WS_BLR_GEMM = 2,  // compressed products: rank-dependent GEMM operands
   WS_RING = 3,      // ring steps and redistributions: index lists, offset tables
+  WS_TUNE = 4,      // split-depth measurement: sampled FP64 and split results
 };
 
 // Returns a buffer of at least `bytes` for (current device, s, slot), or nullptr
