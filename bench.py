@@ -61,26 +61,50 @@ def time_steps(fn, steps, warmup, torch):
 
 
 def cpu_baseline(seed: int):
-    """C restatement of PSTOEPLITZ_COMPUTE on an emulated P x Q grid, one thread per
-    rank (as the reference runs one MPI rank per core), dense branch at every step."""
+    """The C restatement of PSTOEPLITZ_COMPUTE on an emulated P x Q grid, one thread per rank
+    with a sequential host-BLAS DGEMM each (the reference runs one MPI rank per core with MKL,
+    Test/run.sh:1, SLMakes/SLmake.inc.intel:24-35), P*Q = the host cores this job may use (at
+    most 16).  value: the reference-faithful ring (off-diagonal steps through the RRQR branch,
+    SRC/pstoeplitz_compute.f90:263-295) on the full N = 8192 step; beside it the dense-branch
+    ring and the Cauchy benchmark's N = 2048 case on its stated 2 x 2 grid."""
     from oracle import inputs, oracle
 
+    blas = oracle.use_host_blas()
     cores = min(16, os.cpu_count() or 1)
     p = 4 if cores >= 16 else 2
     q = max(1, cores // p)
     n = 8192
     g, a = inputs.toeplitz_hankel_inputs(n, seed=seed)
-    t0 = time.perf_counter()
-    oracle.ps_compute(oracle.TOEPLITZ, p, q, 64, 1.0, a, g, mode=oracle.MODE_DENSE,
-                      nthreads=p * q)
-    dt = time.perf_counter() - t0
+
+    def run(kind, pp, qq, aa, gen, mode):
+        t0 = time.perf_counter()
+        _, stats = oracle.ps_compute(kind, pp, qq, 64, 1.0, aa, gen, mode=mode, nthreads=pp * qq)
+        return time.perf_counter() - t0, [int(x) for x in stats]
+
+    t_f, st_f = run(oracle.TOEPLITZ, p, q, a, g, oracle.MODE_FAITHFUL)
+    t_d, _ = run(oracle.TOEPLITZ, p, q, a, g, oracle.MODE_DENSE)
+    del a
+    n0 = 2048
+    b0, a0 = inputs.cauchy_dense_inputs(n0, seed=seed)
+    t0f, st0 = run(oracle.CAUCHY, 2, 2, a0, b0, oracle.MODE_FAITHFUL)
+    t0d, _ = run(oracle.CAUCHY, 2, 2, a0, b0, oracle.MODE_DENSE)
+    tf = lambda nn, dt: round(2.0 * nn ** 3 / dt / 1e12, 5)
     return {
-        "value": round(2.0 * n ** 3 / dt / 1e12, 5),
+        "value": tf(n, t_f),
         "unit": "TFLOP/s",
         "cores": p * q,
         "kind": "port",
-        "sample": (f"the full Toeplitz x dense N={n} FP64 step, ring restatement on a "
-                   f"{p}x{q} emulated grid (one thread per rank), NB=64, dense branch, {dt:.2f} s"),
+        "sample": (f"the full Toeplitz x dense N={n} FP64 step: ring restatement on a {p}x{q} "
+                   f"emulated grid (one thread per rank, sequential host DGEMM), NB=64, "
+                   f"reference-faithful branches, {t_f:.2f} s"),
+        "blas": blas,
+        "mode": "faithful",
+        "steps_dense_lowrank_fullrank_stale": st_f,
+        "dense_branch": {"value": tf(n, t_d), "seconds": round(t_d, 3)},
+        "cauchy_n2048_grid2x2": {"cores": 4, "faithful_tflops": tf(n0, t0f),
+                                 "faithful_seconds": round(t0f, 3),
+                                 "steps_dense_lowrank_fullrank_stale": st0,
+                                 "dense_branch_tflops": tf(n0, t0d)},
     }
 
 
@@ -523,6 +547,15 @@ def main():
         torch.cuda.synchronize()
         del src, dst
 
+    # the autotuner times every FP64 tile configuration for the headline bucket and the
+    # Cauchy benchmark's (its choice is what every later launch of the bucket uses)
+    autotune = {}
+    for name, kind, nn in (("toeplitz_n%d" % n, api.KIND_TOEPLITZ, n),
+                           ("cauchy_n2048", api.KIND_CAUCHY, 2048)):
+        cfg, ms = api.autotune_f64(kind, nn, nn, nn)
+        autotune[name] = {"cfg": cfg, "ms": round(ms, 4)}
+    log(f"[bench] autotuned FP64 tile configurations: {autotune}")
+
     log("[bench] timing FP64 MFMA form")
     wall, per = time_steps(step, args.steps, args.warmup, torch)
     flops = 2.0 * n ** 3
@@ -549,8 +582,11 @@ def main():
     split = None
     if not args.no_split:
         try:
+            # the autotuner's depth: fewest moduli with <= 1e-12 on a sampled column block
+            prob = api.make_problem(api.KIND_TOEPLITZ, 1.0, a, [g], n, c, lg=2 * n - 1, korder=n)
+            t_meas, e_meas, _ = api.split_slices_dev(prob, api.PREC_SPLIT_I8, 1e-12)
             best = None
-            for slices in (12, 13, 14):
+            for slices in sorted({12, 13, 14, t_meas}):
                 def sstep(sl=slices):
                     api.toeplitz_dev(1.0, a, g, n, n, c, prec=api.PREC_SPLIT_I8, slices=sl)
                 sw, sper = time_steps(sstep, max(3, args.steps // 2), 1, torch)
@@ -563,6 +599,8 @@ def main():
                     best = rec
             if best is not None:
                 best["speedup_vs_fp64"] = round(kern_ms / best["ms"], 3)
+                best["autotuned_slices"] = t_meas
+                best["autotuned_sample_err"] = e_meas
             split = best
         except api.PsmmaError as e:
             split = {"error": str(e)}
@@ -607,6 +645,7 @@ def main():
                      "unit": "TFLOP/s", "frac": round(achieved / FP64_PEAK, 4),
                      "traffic": traffic},
         "split_precision": split,
+        "autotune": autotune,
         "other_configs": configs,
         "cpu_baseline": cpu,
     }

@@ -73,6 +73,8 @@ def _declare(L):
     L.ora_is_intersect.restype = I
     L.ora_dgemm.argtypes = [I, I, I, D, P, I, P, I, D, P, I]
     L.ora_dgemm.restype = None
+    L.ora_set_blas_dgemm.argtypes = [P]
+    L.ora_set_blas_dgemm.restype = None
     L.ora_ps_compute.argtypes = [ctypes.POINTER(OraProblem), I, P]
     L.ora_ps_compute.restype = I
     L.ora_structured_gemm.argtypes = [I, I, I, I, D, P, I, P, I, I, I, I, D, P, I]
@@ -267,6 +269,35 @@ def ps_compute(kind: int, nprow: int, npcol: int, nb: int, alpha, a, gen, beta=0
     if st != 0:
         raise RuntimeError(f"ora_ps_compute failed ({st})")
     return out, stats
+
+
+_BLAS = {"name": "the oracle's C DGEMM", "lib": None}
+
+MKL_PATHS = ("/opt/conda/lib/libmkl_rt.so", "/opt/conda/lib/libmkl_rt.so.1")
+
+
+def use_host_blas() -> str:
+    """Route the ring restatement's DGEMMs to the host BLAS, one sequential DGEMM per emulated
+    rank thread as the reference runs one MPI rank per core (Test/run.sh:1): MKL's libmkl_rt
+    (the reference's BLAS, SLMakes/SLmake.inc.intel:24-35) with its sequential threading layer
+    when installed.  Returns the name of the BLAS in use."""
+    if _BLAS["lib"] is not None:
+        return _BLAS["name"]
+    for path in MKL_PATHS:
+        if os.path.exists(path):
+            try:
+                mkl = ctypes.CDLL(path, mode=ctypes.RTLD_GLOBAL)
+                mkl.MKL_Set_Interface_Layer(0)   # LP64: 32-bit INTEGER, as the reference links
+                mkl.MKL_Set_Threading_Layer(1)   # sequential
+                ver = ctypes.create_string_buffer(256)
+                mkl.MKL_Get_Version_String(ver, 256)
+                addr = ctypes.cast(mkl.dgemm_, ctypes.c_void_p).value
+            except (OSError, AttributeError):
+                continue
+            lib().ora_set_blas_dgemm(addr)
+            _BLAS.update(name=ver.value.decode().strip() + " (sequential, LP64)", lib=mkl)
+            return _BLAS["name"]
+    return _BLAS["name"]
 
 
 def dgemm_c(alpha, a, b, beta=0.0, c=None) -> np.ndarray:

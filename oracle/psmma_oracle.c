@@ -124,6 +124,28 @@ void ora_const_cauchyeig(int lm, int ln, double* t, const double* f, const doubl
   }
 }
 
+/* The ring's DGEMM: the host BLAS's Fortran dgemm_ when one is set (the reference
+ * links MKL, SLMakes/SLmake.inc.intel:24-35), else ora_dgemm below. */
+typedef void (*ora_blas_dgemm_fn)(const char*, const char*, const int*, const int*, const int*,
+                                  const double*, const double*, const int*, const double*,
+                                  const int*, const double*, double*, const int*);
+static ora_blas_dgemm_fn g_blas_dgemm = NULL;
+void ora_dgemm(int m, int n, int k, double alpha, const double* a, int lda, const double* b,
+               int ldb, double beta, double* c, int ldc);
+
+void ora_set_blas_dgemm(void* fn) { g_blas_dgemm = (ora_blas_dgemm_fn)fn; }
+
+static void ring_dgemm(int m, int n, int k, double alpha, const double* a, int lda,
+                       const double* b, int ldb, double beta, double* c, int ldc) {
+  if (g_blas_dgemm && m > 0 && n > 0) {
+    const char nn = 'N';
+    int ka = lda > 0 ? lda : 1, kb = ldb > 0 ? ldb : 1, kc = ldc > 0 ? ldc : 1;
+    g_blas_dgemm(&nn, &nn, &m, &n, &k, &alpha, a, &ka, b, &kb, &beta, c, &kc);
+    return;
+  }
+  ora_dgemm(m, n, k, alpha, a, lda, b, ldb, beta, c, ldc);
+}
+
 /* DGEMM('N','N') semantics: C = alpha*A*B + beta*C (beta == 0 => C not read).
  * Blocked for cache; FP64 accumulation in natural k order within a block. */
 void ora_dgemm(int m, int n, int k, double alpha, const double* a, int lda, const double* b,
@@ -233,16 +255,16 @@ static void* rank_body(void* arg) {
       if (st == ORA_TILE_STALE) jb->stale_steps++;
       if (rk == (kbp < nq ? kbp : nq)) {
         jb->fullrank_steps++;
-        ora_dgemm(mp, nq, kbp, p->alpha, aloc, mp, hbuf, kbp, tbeta, cloc, mp);
+        ring_dgemm(mp, nq, kbp, p->alpha, aloc, mp, hbuf, kbp, tbeta, cloc, mp);
       } else {
         jb->lowrank_steps++;
-        ora_dgemm(mp, rk, kbp, 1.0, aloc, mp, hbuf, kbp, 0.0, tmp, mp);
-        ora_dgemm(mp, nq, rk, p->alpha, tmp, mp, hbuf + (size_t)kbp * rk, rk, tbeta, cloc, mp);
+        ring_dgemm(mp, rk, kbp, 1.0, aloc, mp, hbuf, kbp, 0.0, tmp, mp);
+        ring_dgemm(mp, nq, rk, p->alpha, tmp, mp, hbuf + (size_t)kbp * rk, rk, tbeta, cloc, mp);
       }
     } else {
       jb->dense_steps++;
       build_tile(p, kbp, nq, ridx, cidx, hbuf);
-      ora_dgemm(mp, nq, kbp, p->alpha, aloc, mp, hbuf, kbp, tbeta, cloc, mp);
+      ring_dgemm(mp, nq, kbp, p->alpha, aloc, mp, hbuf, kbp, tbeta, cloc, mp);
     }
     tbeta = 1.0;
   }

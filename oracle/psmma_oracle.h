@@ -36,6 +36,8 @@ void ora_const_hankel(int lm, int ln, const double* g, double* t, const int* rid
 void ora_const_cauchyeig(int lm, int ln, double* t, const double* f, const double* u,
                          const double* v, const double* difl, const double* difr,
                          const int* ridx, const int* cidx);
+/* Host BLAS Fortran dgemm_ for the ring restatement (NULL: the C DGEMM). */
+void ora_set_blas_dgemm(void* fn);
 void ora_dgemm(int m, int n, int k, double alpha, const double* a, int lda, const double* b,
                int ldb, double beta, double* c, int ldc);
 int ora_ps_compute(const ora_problem* p, int nthreads, long long* stats);
