@@ -304,6 +304,21 @@ def eig_merge_config(seed, torch, api, inputs, n=16384):
         "ms": round(msl, 3), "eff_tflops_dense_equiv": round(flops / msl / 1e9, 2),
         "speedup_vs_fp64": round(ms64 / msl, 3), "rel_err": el, "orth_max": lm, "orth_fro": lf,
         "stats": stats_lu}
+    # nested HSS (randomized construction, leaf 256, tol 1e-12): built once, then applied
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    hs = api.Hss(api.KIND_CAUCHYEIG, [td, u, v, difl, difr], leaf=256, tol=1e-12, seed=1)
+    torch.cuda.synchronize()
+    build_ms = (time.perf_counter() - t0) * 1e3
+    msh = ev_time(lambda: hs.product(1.0, tq, x), 3, torch)
+    eh = rel(x, ref)
+    hm, hf = ortho(x)
+    out["hss_leaf256_tol1e-12"] = {
+        "ms": round(msh, 3), "build_ms": round(build_ms, 3),
+        "eff_tflops_dense_equiv": round(flops / msh / 1e9, 2),
+        "speedup_vs_fp64": round(ms64 / msh, 3), "rel_err": eh, "orth_max": hm, "orth_fro": hf,
+        "stats": hs.stats}
+    hs.close()
     del x, ref, tq
     torch.cuda.empty_cache()
     out["rrlu_leaf256_blocks"] = rrlu_blocks(td, u, v, difl, difr, torch, api)
@@ -482,6 +497,12 @@ def profile_leg(leg, seed, torch, api, inputs):
     if leg == "eig_fp64":  # the plain FP64 eigen-Cauchy product at N = 16384
         ms = ev_time(lambda: api.cauchyeig_dev(1.0, a, td, u, v, difl, difr, n, x), 2, torch)
         return {"leg": leg, "n": n, "ms": round(ms, 3)}
+    if leg == "hss":  # nested HSS build, then 3 products, N = 16384
+        hs = api.Hss(api.KIND_CAUCHYEIG, [td, u, v, difl, difr], leaf=256, tol=1e-12, seed=1)
+        ms = ev_time(lambda: hs.product(1.0, a, x), 2, torch)
+        st = hs.stats
+        hs.close()
+        return {"leg": leg, "n": n, "ms": round(ms, 3), "stats": st}
     stats = {}
     comp = api.BLR_RRLU if leg == "blr_rrlu" else None
     ms = ev_time(lambda: stats.update(api.cauchyeig_blr_dev(1.0, a, td, u, v, difl, difr, x,
@@ -504,14 +525,14 @@ def main():
     ap.add_argument("--copy-calib", action="store_true",
                     help="run one 512 MiB device copy first (calibrates HBM byte counters)")
     ap.add_argument("--profile-leg", choices=["split", "batched", "blr", "blr_rrlu", "rrlu", "dft",
-                                              "hankel", "bf16", "cauchy2048", "eig_fp64"],
+                                              "hankel", "bf16", "cauchy2048", "eig_fp64", "hss"],
                     help="run only one workload (for rocprofv3): the N=8192 Toeplitz split product "
                          "(t=13, 3 launches), the 512-problem batch, the N=16384 compressed "
                          "merge product with the RRQR or the RRLU compressor (3 launches), or the generator-only RRLU of the N=16384 "
                          "merge's 4032 leaf blocks (two tolerances), the DFT (FP64 and split-FP32), "
-                         "Hankel N=8192, Toeplitz N=8192 in 3 bf16 slices, Cauchy N=2048 or the "
-                         "plain eigen-Cauchy product N=16384; prints a short JSON line, not the "
-                         "bench line")
+                         "Hankel N=8192, Toeplitz N=8192 in 3 bf16 slices, Cauchy N=2048, the "
+                         "plain eigen-Cauchy product N=16384 or its nested HSS form (build, then "
+                         "3 products); prints a short JSON line, not the bench line")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:

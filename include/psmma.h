@@ -463,6 +463,49 @@ int psmma_cauchy_blr_dev(int m, int n, double alpha, const double* a, int lda, c
                          int compressor, double beta, double* c, int ldc, long long* stats,
                          void* stream);
 
+/* ===========================================================================
+ * 6b. Nested HSS representation and HSS x dense product (device pointers).
+ *
+ * psmma_hss_build_dev: an HSS representation of the n x n eigen-Cauchy matrix
+ * (kind PSMMA_KIND_CAUCHYEIG, g0..g4 = fpole, u, v, difl, difr as in
+ * psmma_cauchyeig_dev) or plain Cauchy matrix (PSMMA_KIND_CAUCHY, g0..g3 = d, f,
+ * u, v), the counterpart of Cauchy2hssEig_omp (SRC/CauchyHssEig.f90:1955-2259):
+ * a complete binary tree over n / leaf leaves (leaf <= 256, n / leaf a power of
+ * two), nested row and column bases U, V and skeleton couplings B per node (the
+ * HSSMM generators, SRC/aux_hss.f90:27-38), each node's bases from an
+ * interpolative decomposition of `samples` (1..128) random samples of its
+ * block row / column, truncated at |R_kk| <= tol |R_00|; samples = 0 chooses
+ * the count: 64, then 96, then 128 while some node's rank comes within 10 of
+ * it (the oversampling margin).  `seed` fixes the samples.  The generators stay
+ * on the device (the generator arrays must outlive the handle); the device is
+ * synchronised.  stats (may be NULL, 6 entries): levels, largest rank, sum of
+ * ranks, IDs whose rank came within 10 of the sample count below full width
+ * (accuracy not guaranteed there), generator storage (doubles), samples.
+ * Returns 0 or -(argument number), > 0 on runtime errors.
+ *
+ * psmma_hss_product_dev: X(m x n) = alpha * A(m x n) * C + beta * X with C in
+ * HSS form (up pass, sibling couplings, down pass, leaf diagonal blocks
+ * generated from the generators; one persistent launch per stage).  Synchronises
+ * `stream` once (after the stages).
+ *
+ * psmma_hss_layout: the HSSMM-like layout, nodes in post order (the reference's
+ * tree order): parent (-1 for the root), row / column range [lo, lo + size),
+ * dims of U (w x r), V (w' x r'), B (r x r'_sibling), their offsets (3 per node,
+ * doubles) into the device array *h; any output may be NULL.  For siblings a, b:
+ * C(I_a, I_b) ~ Ut_a B_a Vt_b^T, Ut = U for leaves and diag(Ut_left, Ut_right) U
+ * for parents (likewise Vt).
+ *
+ * These have no single counterpart among the reference's externals (its HSS
+ * construction is a module routine, not on a PSMMA entry path). */
+int psmma_hss_build_dev(int kind, int n, const double* g0, const double* g1, const double* g2,
+                        const double* g3, const double* g4, int leaf, double tol, int samples,
+                        unsigned long long seed, void** hss, long long* stats, void* stream);
+int psmma_hss_product_dev(void* hss, int m, double alpha, const double* a, int lda, double beta,
+                          double* c, int ldc, void* stream);
+int psmma_hss_layout(void* hss, int* nnodes, int* parent, int* lo, int* size, int* udim,
+                     int* vdim, int* bdim, long long* offsets, const double** h);
+int psmma_hss_destroy(void* hss);
+
 /* Library version string. */
 const char* psmma_version(void);
 

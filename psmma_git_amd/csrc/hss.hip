@@ -1,0 +1,873 @@
+// Nested HSS representation of the eigenvector-Cauchy (or plain Cauchy) matrix
+// and its product with a dense matrix: the counterpart of the reference's
+// Cauchy2hssEig_omp / ComprcauchyEig (SRC/CauchyHssEig.f90:1955-2449) with the
+// HSSMM layout of its generators (SRC/aux_hss.f90:27-48, hssexpmm2 :418-492).
+//
+// Tree: a complete binary tree over n / leaf leaves (a power of two); node tau
+// covers rows and columns I_tau.  Generators, as HSSMM keeps them per node:
+//   U_tau  (w x r):  nested row basis; leaves w = leaf, parents w = r_a + r_b;
+//   V_tau  (w' x r'): nested column basis (C(rows outside, I) ~ C(., J') V^T);
+//   B_tau  (r x r'_sibling) = C(J_tau, J'_sibling), the skeleton coupling;
+//   D_tau  = C(I_tau, I_tau) for leaves, never stored: the product generates it
+//            from the generators straight into LDS, as every kernel here does.
+// So that C(I_a, I_b) ~ Ut_a B_a Vt_b^T for siblings a, b (Ut, Vt the nested bases).
+//
+// Construction (MI355X form): the reference compresses each node's block row and
+// block column by generator-only rook-pivoted LU (rrluCauchyEigrow / -col); here
+// the compressions are interpolative decompositions (column-pivoted QR, truncated
+// at |R_kk| <= tol |R_00|, the truncation rule of Compress_RRQR) of random samples
+// of those blocks, the randomized construction of Martinsson (SIAM J. Matrix
+// Anal. Appl. 32 (2011) 1251-1274): Y = C Omega and Z = C^T Psi with s Gaussian
+// columns come from the FP64 structured kernel; leaf samples have their diagonal
+// block's part removed; a parent's samples are its children's samples at their
+// skeletons minus the sibling couplings times the compressed random matrices.
+// Every ID works on an s x w block (s <= 128, w <= 256), one workgroup per node
+// and level, a column per thread in registers.  The sample count is adaptive:
+// 64, then 96, then 128 while some node's rank comes within 10 of it (at
+// N = 16384 and tol 1e-12 the top level's ranks reach the 70s).
+//
+// Product X = alpha * A * C + beta * X (A m x n dense): up pass G_tau = A(:, I) U
+// (leaves) or [G_a G_b] U (parents), couplings F_b = G_a B_a, down pass
+// [F_a F_b] += F_tau V_tau^T, leaves X(:, I) = alpha (A(:, I) D + F V^T) + beta X.
+// Each stage is one launch of the persistent batch kernel over all nodes of a
+// level (kinds: the Cauchy kinds for D and the samples, explicit matrices for the
+// generators).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <climits>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <vector>
+
+#include "../../include/psmma.h"
+#include "psmma_common.h"
+#include "workspace.h"
+
+namespace psmma {
+int batch_tiles(const std::vector<BatchProblem>& in, int big_min, std::vector<BatchProblem>& out,
+                std::vector<int>& tbeg);
+hipError_t launch_batch_f64(const BatchProblem* dprobs, const int* dtbeg, int nprob, int ntiles,
+                            unsigned* ctr, int nworkgroups, hipStream_t s, int kset);
+}  // namespace psmma
+
+using namespace psmma;
+
+namespace {
+
+constexpr int kHssMaxSamples = 128;
+// A node whose rank comes within this many of the sample count may have been
+// truncated by the sampling rather than by the tolerance (the usual oversampling
+// margin of randomized range finders).
+constexpr int kHssOversample = 10;
+constexpr int kHssMaxWidth = 256;
+constexpr int kHssThreads = 256;
+constexpr int kKsetHss = 1;
+
+// ---------------------------------------------------------------------------
+// Device kernels.
+
+__device__ __forceinline__ unsigned long long splitmix64(unsigned long long x) {
+  x += 0x9E3779B97F4A7C15ull;
+  x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+  x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+  return x ^ (x >> 31);
+}
+
+// Standard normal entries (Box-Muller on two counter-based 53-bit uniforms).
+__global__ void hss_gauss_kernel(long long n, unsigned long long seed, double* __restrict__ out) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const unsigned long long a = splitmix64(seed ^ (2ull * (unsigned long long)i));
+  const unsigned long long b = splitmix64(seed ^ (2ull * (unsigned long long)i + 1ull));
+  const double u1 = ((double)(a >> 11) + 1.0) * 0x1.0p-53;  // (0, 1]
+  const double u2 = (double)(b >> 11) * 0x1.0p-53;           // [0, 1)
+  out[i] = sqrt(-2.0 * log(u1)) * cos(6.283185307179586476925286766559 * u2);
+}
+
+// One interpolative decomposition: the s x w matrix M (column-major, ld s) is
+// factored by Householder QR with column pivoting (largest remaining column
+// norm first, the lowest column index on ties), stopped at the first
+// |R_kk| <= tol |R_00| (rank r), so that M ~ M(:, P[0:r]) [I T] P^T with
+// T = R11^-1 R12.  Outputs: U (w x r, column major, ld w) = P [I; T^T], so that
+// M^T ~ U M(:, P[0:r])^T; the pivots P[0:r]; the skeleton columns M(:, P[0:r])
+// (s x r); the rank.
+//
+// Thread t owns column t in registers (S rows, s <= S real, the rest zero).
+// Columns are never swapped: a pivoted column is marked done and its step
+// recorded, so U's rows are the threads.  T is formed during the
+// factorisation, with no triangular solve at the end: after step k, rows
+// 0..k of a column c not yet pivoted hold t_c = R11^-1 R(0:k, c) for the
+// (k+1) x (k+1) leading R11, and the block inverse of R11 gives
+//   t_c[k] = R(k, c) / R(k, k),   t_c[0:k] -= t_c[k] * t_p[0:k],
+// with t_p the pivot column's own vector from before the step.  Each step
+// passes one S-vector through LDS (the pivot's t_p above row k, its
+// Householder vector from row k on); every register index is a compile-time
+// constant (fully unrolled loops with predicates), so the column stays in
+// VGPRs.
+struct IdJob {
+  const double* M;
+  int w;
+  double* U;
+  int* perm;
+  double* sk;
+  int* rank;
+};
+
+template <int S>
+__global__ __launch_bounds__(kHssThreads, 1) void hss_id_kernel(const IdJob* __restrict__ jobs,
+                                                               int s, double tol) {
+  // Per-step broadcasts, so that no register index or predicate depends on the
+  // step k: mk = [i >= k] and ek = [i == k] (double-buffered by the parity of
+  // k: step k writes step k + 1's); from the pivot: vv its Householder vector
+  // (zero above row k) and tp its t_p (zero from row k on).
+  __shared__ double mk[2][S], ek[2][S], vv[S], tp[S];
+  __shared__ double red_v[kHssThreads / 64];
+  __shared__ int red_i[kHssThreads / 64];
+  __shared__ int piv[S];
+  __shared__ double beta_s, tau_s;
+  const IdJob J = jobs[blockIdx.x];
+  const int w = J.w, t = threadIdx.x;
+  const bool own = t < w;
+  double col[S];
+#pragma unroll
+  for (int i = 0; i < S; ++i) col[i] = (own && i < s) ? J.M[(size_t)t * s + i] : 0.0;
+  for (int i = t; i < S; i += kHssThreads) {
+    mk[0][i] = 1.0;
+    ek[0][i] = (i == 0) ? 1.0 : 0.0;
+  }
+  __syncthreads();
+  int mypos = -1;  // the step at which this column was the pivot
+  const int mn = s < w ? s : w;
+  int r = mn;
+  double r00 = 0.0;
+  for (int k = 0; k < mn; ++k) {
+    const int par = k & 1;
+    // remaining norm of this column (rows k..S-1) and its row k; first index of
+    // the largest norm
+    double a = 0.0, ck = 0.0;
+#pragma unroll
+    for (int i = 0; i < S; ++i) {
+      a += mk[par][i] * (col[i] * col[i]);
+      ck += ek[par][i] * col[i];
+    }
+    double b = (own && mypos < 0) ? sqrt(a) : -1.0;
+    int bi = (own && mypos < 0) ? t : INT_MAX;
+    for (int o = 32; o > 0; o >>= 1) {
+      const double b2 = __shfl_xor(b, o);
+      const int i2 = __shfl_xor(bi, o);
+      if (b2 > b || (b2 == b && i2 < bi)) {
+        b = b2;
+        bi = i2;
+      }
+    }
+    if ((t & 63) == 0) {
+      red_v[t >> 6] = b;
+      red_i[t >> 6] = bi;
+    }
+    __syncthreads();  // (1)
+    b = red_v[0];
+    bi = red_i[0];
+    for (int q = 1; q < kHssThreads / 64; ++q)
+      if (red_v[q] > b || (red_v[q] == b && red_i[q] < bi)) {
+        b = red_v[q];
+        bi = red_i[q];
+      }
+    if (k == 0) r00 = b;
+    if (!(b > tol * r00) || b == 0.0) {  // uniform
+      r = k;
+      break;
+    }
+    // the pivot column's reflector (LAPACK's DLARFG form: v_k = 1)
+    if (t == bi) {
+      const double x0 = ck;
+      const double beta = (x0 >= 0.0) ? -b : b;
+      const double sc = 1.0 / (x0 - beta);
+#pragma unroll
+      for (int i = 0; i < S; ++i) {
+        const double m = mk[par][i], e = ek[par][i];
+        vv[i] = e + (m - e) * (col[i] * sc);  // exact 1 at row k, 0 above
+        tp[i] = (1.0 - m) * col[i];
+      }
+      beta_s = beta;
+      tau_s = (beta - x0) / beta;
+      piv[k] = t;
+      mypos = k;
+    }
+    for (int i = t; i < S; i += kHssThreads) {
+      mk[par ^ 1][i] = (i > k) ? 1.0 : 0.0;
+      ek[par ^ 1][i] = (i == k + 1) ? 1.0 : 0.0;
+    }
+    __syncthreads();  // (2)
+    // Columns not yet pivoted: reflect rows k.. (R(k, c) = c_k - tau (v . c)
+    // lands in row k), then extend t_c by row k:
+    //   y = c - tau (v . c) v;  x = y_k / beta;  c = e_k x + (1 - e_k) (y - x t_p)
+    // (rows < k: t_c - x t_p; row k: x; rows > k: y).
+    const bool upd = own && mypos < 0;
+    double d = 0.0;
+    if (upd) {
+#pragma unroll
+      for (int i = 0; i < S; ++i) d += vv[i] * col[i];
+    }
+    // No data passes through this barrier: it makes the compiler load vv again
+    // below instead of keeping S more values in registers.
+    __syncthreads();  // (3)
+    if (upd) {
+      d *= tau_s;
+      const double x = (ck - d) / beta_s;
+#pragma unroll
+      for (int i = 0; i < S; ++i) {
+        const double y = col[i] - d * vv[i];
+        const double e = ek[par][i];
+        col[i] = e * x + (1.0 - e) * (y - x * tp[i]);  // exact selects: e is 0 or 1
+      }
+    }
+    // Next step: its mk / ek were written before (2); red_v / red_i are
+    // rewritten before its (1), after every reader passed (2); vv, tp, beta_s,
+    // tau_s and this step's mk / ek buffers are rewritten after its (1), which
+    // every thread reaches only after finishing the update above.
+  }
+  __syncthreads();  // piv complete
+  if (own) {
+#pragma unroll
+    for (int j = 0; j < S; ++j)
+      if (j < r) J.U[(size_t)j * w + t] = (mypos >= 0) ? (mypos == j ? 1.0 : 0.0) : col[j];
+  }
+  if (t < r) J.perm[t] = piv[t];
+  for (int e = t; e < s * r; e += kHssThreads) {
+    const int i = e % s, j = e / s;
+    J.sk[e] = J.M[(size_t)piv[j] * s + i];
+  }
+  if (t == 0) *J.rank = r;
+}
+
+// Skeleton couplings B = C(rows, cols) evaluated from the generators.
+struct BJob {
+  const int* rows;  // 0-based global rows
+  const int* cols;
+  int nr, nc;
+  double* out;      // nr x nc, column-major
+};
+
+template <int KIND>
+__global__ __launch_bounds__(256) void hss_b_kernel(const BJob* __restrict__ jobs, GenPtrs g) {
+  const BJob J = jobs[blockIdx.x];
+  for (int e = threadIdx.x; e < J.nr * J.nc; e += blockDim.x) {
+    const int i = e % J.nr, j = e / J.nr;
+    J.out[e] = gen_entry<KIND>(g, J.rows[i], J.cols[j]);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Host side.
+
+struct Node {
+  int level, lo, n;            // rows and columns [lo, lo + n)
+  int parent = -1, left = -1, right = -1, sib = -1;
+  int wr = 0, wc = 0;          // ID widths (row / column)
+  int r = 0, rc = 0;           // row / column ranks
+  long long offU = 0, offV = 0, offB = 0;  // into H: U (wr x r), V (wc x rc), B (r x rc_sib)
+  std::vector<int> Jr, Jc;     // global skeleton rows / columns
+  int post = 0;                // post-order number
+};
+
+int ilog2(int x) {
+  int l = 0;
+  while ((1 << l) < x) ++l;
+  return l;
+}
+
+}  // namespace
+
+namespace psmma {
+
+struct Hss {
+  int kind = KIND_CAUCHYEIG, n = 0, leaf = 0, s = 0, L = 0;
+  GenPtrs g{};
+  std::vector<Node> nodes;            // level order: node (l, i) at index 2^l - 1 + i
+  std::vector<std::vector<int>> lev;  // node indices per level
+  double* H = nullptr;                // all U, V, B (device)
+  size_t hsize = 0;
+  long long stats[6] = {0, 0, 0, 0, 0, 0};
+  // product offsets per level: column offsets of G (sum r) and F (sum rc)
+  std::vector<std::vector<long long>> goff, foff;
+  std::vector<long long> gsum, fsum;
+};
+
+}  // namespace psmma
+
+namespace {
+
+// Uploads the stage tables and runs each stage as one persistent launch (in order on s).
+int run_stages(const std::vector<std::vector<BatchProblem>>& stages, hipStream_t s) {
+  std::vector<std::vector<BatchProblem>> probs(stages.size());
+  std::vector<std::vector<int>> tbeg(stages.size());
+  std::vector<int> ntiles(stages.size());
+  auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
+  size_t bytes = 0;
+  for (size_t i = 0; i < stages.size(); ++i) {
+    ntiles[i] = batch_tiles(stages[i], 256, probs[i], tbeg[i]);
+    bytes += al(sizeof(BatchProblem) * probs[i].size()) + al(sizeof(int) * tbeg[i].size()) +
+             al(2 * sizeof(unsigned));
+  }
+  std::vector<char> host(bytes, 0);
+  std::vector<size_t> offp(stages.size()), offt(stages.size()), offc(stages.size());
+  size_t o = 0;
+  for (size_t i = 0; i < stages.size(); ++i) {
+    offp[i] = o;
+    if (!probs[i].empty())
+      memcpy(host.data() + o, probs[i].data(), sizeof(BatchProblem) * probs[i].size());
+    o += al(sizeof(BatchProblem) * probs[i].size());
+    offt[i] = o;
+    if (!tbeg[i].empty()) memcpy(host.data() + o, tbeg[i].data(), sizeof(int) * tbeg[i].size());
+    o += al(sizeof(int) * tbeg[i].size());
+    offc[i] = o;  // counters: zero
+    o += al(2 * sizeof(unsigned));
+  }
+  std::unique_lock<std::mutex> lk;
+  char* d = (char*)ws_lease(WS_HSS_TABLES, bytes, s, lk);
+  if (!d) return 1002;
+  hipError_t e = hipMemcpyAsync(d, host.data(), bytes, hipMemcpyHostToDevice, s);
+  if (e != hipSuccess) return 1000 + (int)e;
+  int dev = 0, cus = 256;
+  hipGetDevice(&dev);
+  hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  for (size_t i = 0; i < stages.size(); ++i) {
+    if (ntiles[i] == 0) continue;
+    e = launch_batch_f64((const BatchProblem*)(d + offp[i]), (const int*)(d + offt[i]),
+                         (int)probs[i].size(), ntiles[i], (unsigned*)(d + offc[i]), 2 * cus, s,
+                         kKsetHss);
+    if (e != hipSuccess) return 1000 + (int)e;
+  }
+  // the tables must outlive the launches: they stay leased until the stream has them
+  e = hipStreamSynchronize(s);
+  return e == hipSuccess ? 0 : 1000 + (int)e;
+}
+
+BatchProblem dense_problem(int kind, int m, int n, int k, double alpha, const double* a, int lda,
+                           const double* h, int ldh, double beta, double* c, int ldc) {
+  BatchProblem b{};
+  b.kind = kind;
+  b.m = m;
+  b.n = n;
+  b.k = k;
+  b.alpha = alpha;
+  b.beta = beta;
+  b.A = a;
+  b.lda = std::max(1, lda);
+  b.g = GenPtrs{h, nullptr, nullptr, nullptr, nullptr, std::max(1, ldh), 0, 0};
+  b.C = c;
+  b.ldc = std::max(1, ldc);
+  return b;
+}
+
+BatchProblem gen_problem(const Hss& h, int kind, int m, int n, int k, double alpha, const double* a,
+                         int lda, int r0, int c0, double beta, double* c, int ldc) {
+  BatchProblem b{};
+  b.kind = kind;
+  b.m = m;
+  b.n = n;
+  b.k = k;
+  b.alpha = alpha;
+  b.beta = beta;
+  b.A = a;
+  b.lda = std::max(1, lda);
+  b.g = h.g;
+  b.g.nrows = r0 + k;
+  b.g.ncols = c0 + n;
+  b.r0 = r0;
+  b.c0 = c0;
+  b.C = c;
+  b.ldc = std::max(1, ldc);
+  return b;
+}
+
+void build_tree(Hss& h) {
+  const int n = h.n, nl = n / h.leaf, L = ilog2(nl);
+  h.L = L;
+  h.nodes.assign(2 * nl - 1, Node{});
+  h.lev.assign(L + 1, {});
+  for (int l = 0; l <= L; ++l)
+    for (int i = 0; i < (1 << l); ++i) {
+      const int id = (1 << l) - 1 + i;
+      Node& nd = h.nodes[id];
+      nd.level = l;
+      nd.n = n >> l;
+      nd.lo = i * nd.n;
+      if (l > 0) {
+        nd.parent = (1 << (l - 1)) - 1 + i / 2;
+        nd.sib = (1 << l) - 1 + (i ^ 1);
+      }
+      if (l < L) {
+        nd.left = (1 << (l + 1)) - 1 + 2 * i;
+        nd.right = nd.left + 1;
+      }
+      h.lev[l].push_back(id);
+    }
+  // post-order numbering (the order of the reference's tree arrays)
+  int cnt = 0;
+  std::vector<std::pair<int, bool>> stack{{0, false}};
+  while (!stack.empty()) {
+    const auto top = stack.back();
+    stack.pop_back();
+    const Node& nd = h.nodes[top.first];
+    if (top.second || nd.left < 0) {
+      h.nodes[top.first].post = cnt++;
+      continue;
+    }
+    stack.push_back({top.first, true});
+    stack.push_back({nd.right, false});
+    stack.push_back({nd.left, false});
+  }
+}
+
+int build(Hss& h, double tol, unsigned long long seed, hipStream_t strm) {
+  build_tree(h);
+  const int n = h.n, s = h.s, nl = n / h.leaf, L = h.L;
+  // generator storage: fixed per-node regions sized for the widest ID
+  long long off = 0;
+  for (auto& nd : h.nodes) {
+    if (nd.level == 0) continue;
+    const long long wd = nd.level == L ? nd.n : 2 * s;
+    nd.offU = off;
+    off += wd * s;
+    nd.offV = off;
+    off += wd * s;
+    nd.offB = off;
+    off += (long long)s * s;
+  }
+  h.hsize = (size_t)std::max<long long>(off, 1);
+  if (hipMalloc(&h.H, sizeof(double) * h.hsize) != hipSuccess) return 1002;
+  // the regions are sized for the widest ID: zero what the ranks leave unused
+  if (hipMemsetAsync(h.H, 0, sizeof(double) * h.hsize, strm) != hipSuccess) return 1002;
+  h.stats[5] = s;
+  if (L == 0) return 0;  // one leaf: the product is its diagonal block
+
+  // work memory of the construction (freed at the end)
+  const size_t sn = (size_t)s * n;
+  const int nid = (int)h.nodes.size();
+  auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
+  const size_t wmax = (size_t)std::max(h.leaf, 2 * s);
+  const size_t pbytes = sizeof(double) * (size_t)s * 2 * s * std::max(1, nl / 2);
+  const size_t slot = sizeof(double) * (size_t)s * s * nid;
+  const size_t bytes = 4 * al(sizeof(double) * sn) + 2 * al(pbytes) + 4 * al(slot) +
+                       2 * al(sizeof(int) * wmax * nid) + 2 * al(sizeof(int) * nid) +
+                       al(sizeof(IdJob) * 2 * nid) + al(sizeof(BJob) * nid) +
+                       2 * al(sizeof(int) * (size_t)s * nid);
+  char* w = nullptr;
+  if (hipMalloc(&w, bytes) != hipSuccess) return 1002;
+  size_t o = 0;
+  auto take = [&](size_t b) {
+    char* p = w + o;
+    o += al(b);
+    return p;
+  };
+  double* Om = (double*)take(sizeof(double) * sn);
+  double* Ps = (double*)take(sizeof(double) * sn);
+  double* Y = (double*)take(sizeof(double) * sn);
+  double* Z = (double*)take(sizeof(double) * sn);
+  double* Yp = (double*)take(pbytes);
+  double* Zp = (double*)take(pbytes);
+  double* Ysk = (double*)take(slot);
+  double* Zsk = (double*)take(slot);
+  double* Omh = (double*)take(slot);
+  double* Psh = (double*)take(slot);
+  int* permr = (int*)take(sizeof(int) * wmax * nid);
+  int* permc = (int*)take(sizeof(int) * wmax * nid);
+  int* rankr = (int*)take(sizeof(int) * nid);
+  int* rankc = (int*)take(sizeof(int) * nid);
+  IdJob* djobs = (IdJob*)take(sizeof(IdJob) * 2 * nid);
+  BJob* dbjobs = (BJob*)take(sizeof(BJob) * nid);
+  int* djr = (int*)take(sizeof(int) * (size_t)s * nid);
+  int* djc = (int*)take(sizeof(int) * (size_t)s * nid);
+  auto done = [&](int code) {
+    hipStreamSynchronize(strm);
+    hipFree(w);
+    return code;
+  };
+  auto ss = [](size_t slotidx, int s_) { return (size_t)slotidx * s_ * s_; };
+  hipError_t e;
+  int st;
+
+  // samples: Om^T, Ps^T (s x n) Gaussian; Y^T = Om^T C^T, Z^T = Ps^T C (s x n)
+  const unsigned gb = (unsigned)((sn + 255) / 256);
+  hipLaunchKernelGGL(hss_gauss_kernel, dim3(gb), dim3(256), 0, strm, (long long)sn, seed, Om);
+  hipLaunchKernelGGL(hss_gauss_kernel, dim3(gb), dim3(256), 0, strm, (long long)sn,
+                     seed ^ 0xA5A5A5A5DEADBEEFull, Ps);
+  const int kT = (h.kind == KIND_CAUCHYEIG) ? KIND_CAUCHYEIG_T : KIND_CAUCHY_T;
+  {
+    std::vector<std::vector<BatchProblem>> stg(2);
+    stg[0].push_back(gen_problem(h, kT, s, n, n, 1.0, Om, s, 0, 0, 0.0, Y, s));
+    stg[0].push_back(gen_problem(h, h.kind, s, n, n, 1.0, Ps, s, 0, 0, 0.0, Z, s));
+    for (int id : h.lev[L]) {  // leaf diagonal blocks removed from the samples
+      const Node& nd = h.nodes[id];
+      stg[1].push_back(gen_problem(h, kT, s, nd.n, nd.n, -1.0, Om + (size_t)nd.lo * s, s, nd.lo,
+                                   nd.lo, 1.0, Y + (size_t)nd.lo * s, s));
+      stg[1].push_back(gen_problem(h, h.kind, s, nd.n, nd.n, -1.0, Ps + (size_t)nd.lo * s, s,
+                                   nd.lo, nd.lo, 1.0, Z + (size_t)nd.lo * s, s));
+    }
+    if ((st = run_stages(stg, strm))) return done(st);
+  }
+  // level inputs: the ID matrices (s x width) of every node of the level
+  std::vector<const double*> Min(nid, nullptr), Zin(nid, nullptr);
+  for (int id : h.lev[L]) {
+    Node& nd = h.nodes[id];
+    nd.wr = nd.wc = nd.n;
+    Min[id] = Y + (size_t)nd.lo * s;
+    Zin[id] = Z + (size_t)nd.lo * s;
+  }
+  long long rank_sum = 0, rank_max = 0, saturated = 0, storage = 0;
+  for (int l = L; l >= 1; --l) {
+    const std::vector<int>& ids = h.lev[l];
+    // interpolative decompositions of the row and column samples
+    std::vector<IdJob> jobs;
+    for (int id : ids) {
+      const Node& nd = h.nodes[id];
+      jobs.push_back(IdJob{Min[id], nd.wr, h.H + nd.offU, permr + (size_t)id * wmax,
+                           Ysk + ss(id, s), rankr + id});
+      jobs.push_back(IdJob{Zin[id], nd.wc, h.H + nd.offV, permc + (size_t)id * wmax,
+                           Zsk + ss(id, s), rankc + id});
+    }
+    e = hipMemcpyAsync(djobs, jobs.data(), sizeof(IdJob) * jobs.size(), hipMemcpyHostToDevice,
+                       strm);
+    if (e != hipSuccess) return done(1000 + (int)e);
+    if (s <= 64)
+      hipLaunchKernelGGL(hss_id_kernel<64>, dim3((unsigned)jobs.size()), dim3(kHssThreads), 0,
+                         strm, djobs, s, tol);
+    else if (s <= 96)
+      hipLaunchKernelGGL(hss_id_kernel<96>, dim3((unsigned)jobs.size()), dim3(kHssThreads), 0,
+                         strm, djobs, s, tol);
+    else
+      hipLaunchKernelGGL(hss_id_kernel<128>, dim3((unsigned)jobs.size()), dim3(kHssThreads), 0,
+                         strm, djobs, s, tol);
+    std::vector<int> hr(nid), hc(nid), hpr(wmax * nid), hpc(wmax * nid);
+    if ((e = hipMemcpyAsync(hr.data(), rankr, sizeof(int) * nid, hipMemcpyDeviceToHost, strm)) ||
+        (e = hipMemcpyAsync(hc.data(), rankc, sizeof(int) * nid, hipMemcpyDeviceToHost, strm)) ||
+        (e = hipMemcpyAsync(hpr.data(), permr, sizeof(int) * wmax * nid, hipMemcpyDeviceToHost,
+                            strm)) ||
+        (e = hipMemcpyAsync(hpc.data(), permc, sizeof(int) * wmax * nid, hipMemcpyDeviceToHost,
+                            strm)) ||
+        (e = hipStreamSynchronize(strm)))
+      return done(1000 + (int)e);
+    for (int id : ids) {
+      Node& nd = h.nodes[id];
+      nd.r = hr[id];
+      nd.rc = hc[id];
+      // the global indices the ID's columns stand for
+      std::vector<int> rsrc, csrc;
+      if (l == L) {
+        for (int j = 0; j < nd.n; ++j) rsrc.push_back(nd.lo + j);
+        csrc = rsrc;
+      } else {
+        const Node& a = h.nodes[nd.left];
+        const Node& b = h.nodes[nd.right];
+        rsrc = a.Jr;
+        rsrc.insert(rsrc.end(), b.Jr.begin(), b.Jr.end());
+        csrc = a.Jc;
+        csrc.insert(csrc.end(), b.Jc.begin(), b.Jc.end());
+      }
+      nd.Jr.clear();
+      nd.Jc.clear();
+      for (int i = 0; i < nd.r; ++i) nd.Jr.push_back(rsrc[hpr[(size_t)id * wmax + i]]);
+      for (int i = 0; i < nd.rc; ++i) nd.Jc.push_back(csrc[hpc[(size_t)id * wmax + i]]);
+      rank_sum += nd.r + nd.rc;
+      rank_max = std::max<long long>(rank_max, std::max(nd.r, nd.rc));
+      saturated += (nd.r > s - kHssOversample && nd.r < nd.wr) +
+                   (nd.rc > s - kHssOversample && nd.rc < nd.wc);
+      storage += (long long)nd.wr * nd.r + (long long)nd.wc * nd.rc;
+    }
+    // compressed random matrices: Omh^T = Om^T(:, I) V (leaves) or [Omh_a Omh_b] V,
+    // Psh^T likewise with U (two stages: the halves of a parent accumulate)
+    std::vector<std::vector<BatchProblem>> stg(2);
+    for (int id : ids) {
+      const Node& nd = h.nodes[id];
+      double* om = Omh + ss(id, s);
+      double* ps = Psh + ss(id, s);
+      if (l == L) {
+        stg[0].push_back(dense_problem(KIND_DENSE, s, nd.rc, nd.n, 1.0, Om + (size_t)nd.lo * s,
+                                       s, h.H + nd.offV, nd.wc, 0.0, om, s));
+        stg[0].push_back(dense_problem(KIND_DENSE, s, nd.r, nd.n, 1.0, Ps + (size_t)nd.lo * s, s,
+                                       h.H + nd.offU, nd.wr, 0.0, ps, s));
+      } else {
+        const Node& a = h.nodes[nd.left];
+        const Node& b = h.nodes[nd.right];
+        stg[0].push_back(dense_problem(KIND_DENSE, s, nd.rc, a.rc, 1.0, Omh + ss(nd.left, s), s,
+                                       h.H + nd.offV, nd.wc, 0.0, om, s));
+        stg[1].push_back(dense_problem(KIND_DENSE, s, nd.rc, b.rc, 1.0, Omh + ss(nd.right, s), s,
+                                       h.H + nd.offV + a.rc, nd.wc, 1.0, om, s));
+        stg[0].push_back(dense_problem(KIND_DENSE, s, nd.r, a.r, 1.0, Psh + ss(nd.left, s), s,
+                                       h.H + nd.offU, nd.wr, 0.0, ps, s));
+        stg[1].push_back(dense_problem(KIND_DENSE, s, nd.r, b.r, 1.0, Psh + ss(nd.right, s), s,
+                                       h.H + nd.offU + a.r, nd.wr, 1.0, ps, s));
+      }
+    }
+    if ((st = run_stages(stg, strm))) return done(st);
+    // skeleton couplings B_a = C(J_a, J'_b) of the siblings of this level
+    std::vector<int> hjr((size_t)s * nid, 0), hjc((size_t)s * nid, 0);
+    std::vector<BJob> bj;
+    for (int id : ids) {
+      const Node& nd = h.nodes[id];
+      std::copy(nd.Jr.begin(), nd.Jr.end(), hjr.begin() + (size_t)s * id);
+      std::copy(nd.Jc.begin(), nd.Jc.end(), hjc.begin() + (size_t)s * id);
+    }
+    for (int id : ids) {
+      const Node& nd = h.nodes[id];
+      const Node& sb = h.nodes[nd.sib];
+      bj.push_back(BJob{djr + (size_t)s * id, djc + (size_t)s * nd.sib, nd.r, sb.rc,
+                        h.H + nd.offB});
+      storage += (long long)nd.r * sb.rc;
+    }
+    if ((e = hipMemcpyAsync(djr, hjr.data(), sizeof(int) * hjr.size(), hipMemcpyHostToDevice,
+                            strm)) ||
+        (e = hipMemcpyAsync(djc, hjc.data(), sizeof(int) * hjc.size(), hipMemcpyHostToDevice,
+                            strm)) ||
+        (e = hipMemcpyAsync(dbjobs, bj.data(), sizeof(BJob) * bj.size(), hipMemcpyHostToDevice,
+                            strm)))
+      return done(1000 + (int)e);
+    if (h.kind == KIND_CAUCHYEIG)
+      hipLaunchKernelGGL(hss_b_kernel<KIND_CAUCHYEIG>, dim3((unsigned)bj.size()), dim3(256), 0,
+                         strm, dbjobs, h.g);
+    else
+      hipLaunchKernelGGL(hss_b_kernel<KIND_CAUCHY>, dim3((unsigned)bj.size()), dim3(256), 0, strm,
+                         dbjobs, h.g);
+    // the host index lists above must outlive their copies
+    if ((e = hipStreamSynchronize(strm))) return done(1000 + (int)e);
+    if (l == 1) break;  // the root is not compressed
+    // the parents' samples: [Ysk_a - Omh_b B_a^T, Ysk_b - Omh_a B_b^T] and
+    // [Zsk_a - Psh_b B_b, Zsk_b - Psh_a B_a]
+    std::vector<std::vector<BatchProblem>> pst(1);
+    long long yo = 0, zo = 0;
+    for (int pid : h.lev[l - 1]) {
+      Node& pn = h.nodes[pid];
+      const Node& a = h.nodes[pn.left];
+      const Node& b = h.nodes[pn.right];
+      pn.wr = a.r + b.r;
+      pn.wc = a.rc + b.rc;
+      double* yp = Yp + (size_t)yo * s;
+      double* zp = Zp + (size_t)zo * s;
+      Min[pid] = yp;
+      Zin[pid] = zp;
+      if ((e = hipMemcpyAsync(yp, Ysk + ss(pn.left, s), sizeof(double) * s * a.r,
+                              hipMemcpyDeviceToDevice, strm)) ||
+          (e = hipMemcpyAsync(yp + (size_t)s * a.r, Ysk + ss(pn.right, s),
+                              sizeof(double) * s * b.r, hipMemcpyDeviceToDevice, strm)) ||
+          (e = hipMemcpyAsync(zp, Zsk + ss(pn.left, s), sizeof(double) * s * a.rc,
+                              hipMemcpyDeviceToDevice, strm)) ||
+          (e = hipMemcpyAsync(zp + (size_t)s * a.rc, Zsk + ss(pn.right, s),
+                              sizeof(double) * s * b.rc, hipMemcpyDeviceToDevice, strm)))
+        return done(1000 + (int)e);
+      pst[0].push_back(dense_problem(KIND_DENSE_T, s, a.r, b.rc, -1.0, Omh + ss(pn.right, s), s,
+                                     h.H + a.offB, a.r, 1.0, yp, s));
+      pst[0].push_back(dense_problem(KIND_DENSE_T, s, b.r, a.rc, -1.0, Omh + ss(pn.left, s), s,
+                                     h.H + b.offB, b.r, 1.0, yp + (size_t)s * a.r, s));
+      pst[0].push_back(dense_problem(KIND_DENSE, s, a.rc, b.r, -1.0, Psh + ss(pn.right, s), s,
+                                     h.H + b.offB, b.r, 1.0, zp, s));
+      pst[0].push_back(dense_problem(KIND_DENSE, s, b.rc, a.r, -1.0, Psh + ss(pn.left, s), s,
+                                     h.H + a.offB, a.r, 1.0, zp + (size_t)s * a.rc, s));
+      yo += pn.wr;
+      zo += pn.wc;
+    }
+    if ((st = run_stages(pst, strm))) return done(st);
+  }
+  h.stats[0] = L;
+  h.stats[1] = rank_max;
+  h.stats[2] = rank_sum;
+  h.stats[3] = saturated;
+  h.stats[4] = storage;
+  h.stats[5] = s;
+  // product offsets: G (sum of row ranks) and F (sum of column ranks) per level
+  h.goff.assign(L + 1, {});
+  h.foff.assign(L + 1, {});
+  h.gsum.assign(L + 1, 0);
+  h.fsum.assign(L + 1, 0);
+  for (int l = 1; l <= L; ++l)
+    for (int id : h.lev[l]) {
+      h.goff[l].push_back(h.gsum[l]);
+      h.foff[l].push_back(h.fsum[l]);
+      h.gsum[l] += h.nodes[id].r;
+      h.fsum[l] += h.nodes[id].rc;
+    }
+  return done(0);
+}
+
+int product(const Hss& h, int m, double alpha, const double* a, int lda, double beta, double* c,
+            int ldc, hipStream_t strm) {
+  const int L = h.L;
+  std::vector<std::vector<BatchProblem>> stg;
+  if (L == 0) {
+    stg.push_back({gen_problem(h, h.kind, m, h.n, h.n, alpha, a, lda, 0, 0, beta, c, ldc)});
+    return run_stages(stg, strm);
+  }
+  long long gtot = 0, ftot = 0;
+  std::vector<long long> gbase(L + 1, 0), fbase(L + 1, 0);
+  for (int l = 1; l <= L; ++l) {
+    gbase[l] = gtot;
+    fbase[l] = ftot;
+    gtot += h.gsum[l];
+    ftot += h.fsum[l];
+  }
+  std::unique_lock<std::mutex> lk;
+  double* G = (double*)ws_lease(WS_HSS_WORK, sizeof(double) * (size_t)m * (gtot + ftot + 1), strm,
+                                lk);
+  if (!G) return 1002;
+  double* F = G + (size_t)m * gtot;
+  auto gcol = [&](int l, int i) { return G + (size_t)m * (gbase[l] + h.goff[l][i]); };
+  auto fcol = [&](int l, int i) { return F + (size_t)m * (fbase[l] + h.foff[l][i]); };
+  // up pass
+  stg.emplace_back();
+  for (size_t i = 0; i < h.lev[L].size(); ++i) {
+    const Node& nd = h.nodes[h.lev[L][i]];
+    stg.back().push_back(dense_problem(KIND_DENSE, m, nd.r, nd.n, 1.0, a + (size_t)nd.lo * lda,
+                                       lda, h.H + nd.offU, nd.wr, 0.0, gcol(L, (int)i), m));
+  }
+  for (int l = L - 1; l >= 1; --l) {
+    stg.emplace_back();
+    for (size_t i = 0; i < h.lev[l].size(); ++i) {
+      const Node& nd = h.nodes[h.lev[l][i]];
+      stg.back().push_back(dense_problem(KIND_DENSE, m, nd.r, nd.wr, 1.0, gcol(l + 1, 2 * (int)i),
+                                         m, h.H + nd.offU, nd.wr, 0.0, gcol(l, (int)i), m));
+    }
+  }
+  // couplings: F_b = G_a B_a for every sibling pair of every level
+  stg.emplace_back();
+  for (int l = 1; l <= L; ++l)
+    for (size_t i = 0; i < h.lev[l].size(); ++i) {
+      const Node& nd = h.nodes[h.lev[l][i]];
+      const int js = (int)(i ^ 1);
+      const Node& sb = h.nodes[nd.sib];
+      stg.back().push_back(dense_problem(KIND_DENSE, m, sb.rc, nd.r, 1.0, gcol(l, (int)i), m,
+                                         h.H + nd.offB, nd.r, 0.0, fcol(l, js), m));
+    }
+  // down pass: [F_a F_b] += F_tau V_tau^T
+  for (int l = 1; l < L; ++l) {
+    stg.emplace_back();
+    for (size_t i = 0; i < h.lev[l].size(); ++i) {
+      const Node& nd = h.nodes[h.lev[l][i]];
+      stg.back().push_back(dense_problem(KIND_DENSE_T, m, nd.wc, nd.rc, 1.0, fcol(l, (int)i), m,
+                                         h.H + nd.offV, nd.wc, 1.0, fcol(l + 1, 2 * (int)i), m));
+    }
+  }
+  // leaves: X(:, I) = alpha (A(:, I) D + F V^T) + beta X(:, I)
+  stg.emplace_back();
+  for (int id : h.lev[L]) {
+    const Node& nd = h.nodes[id];
+    stg.back().push_back(gen_problem(h, h.kind, m, nd.n, nd.n, alpha, a + (size_t)nd.lo * lda,
+                                     lda, nd.lo, nd.lo, beta, c + (size_t)nd.lo * ldc, ldc));
+  }
+  stg.emplace_back();
+  for (size_t i = 0; i < h.lev[L].size(); ++i) {
+    const Node& nd = h.nodes[h.lev[L][i]];
+    stg.back().push_back(dense_problem(KIND_DENSE_T, m, nd.n, nd.rc, alpha, fcol(L, (int)i), m,
+                                       h.H + nd.offV, nd.wc, 1.0, c + (size_t)nd.lo * ldc, ldc));
+  }
+  return run_stages(stg, strm);
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// C ABI.
+
+extern "C" int psmma_hss_build_dev(int kind, int n, const double* g0, const double* g1,
+                                   const double* g2, const double* g3, const double* g4, int leaf,
+                                   double tol, int samples, unsigned long long seed, void** hss,
+                                   long long* stats, void* stream) {
+  if (kind != PSMMA_KIND_CAUCHY && kind != PSMMA_KIND_CAUCHYEIG) return -1;
+  if (n < 1) return -2;
+  if (!g0 || !g1 || !g2 || !g3 || (kind == PSMMA_KIND_CAUCHYEIG && !g4)) return -3;
+  if (leaf < 1 || leaf > kHssMaxWidth || n % leaf != 0 || ((n / leaf) & (n / leaf - 1)) != 0)
+    return -8;
+  if (!(tol >= 0.0)) return -9;
+  if (samples < 0 || samples > kHssMaxSamples) return -10;
+  if (!hss) return -12;
+  *hss = nullptr;
+  Hss* h = new Hss;
+  h->kind = kind;
+  h->n = n;
+  h->leaf = leaf;
+  if (kind == PSMMA_KIND_CAUCHYEIG)
+    h->g = GenPtrs{g0, g1, g2, g3, g4, n, n, n};
+  else
+    h->g = GenPtrs{g0, g1, g2, g3, nullptr, 0, n, n};
+  // samples = 0: 64, then 96, then 128 while some node's rank reaches within the
+  // oversampling margin of the sample count
+  static const int kTries[3] = {64, 96, 128};
+  int st = 0;
+  for (int a = 0; a < 3; ++a) {
+    h->s = samples ? samples : kTries[a];
+    st = build(*h, tol, seed, (hipStream_t)stream);
+    if (st != 0 || samples || h->stats[3] == 0 || a == 2) break;
+    hipStreamSynchronize((hipStream_t)stream);
+    hipFree(h->H);
+    h->H = nullptr;
+  }
+  if (st != 0) {
+    psmma_hss_destroy(h);
+    return st;
+  }
+  if (stats)
+    for (int i = 0; i < 6; ++i) stats[i] = h->stats[i];
+  *hss = h;
+  return 0;
+}
+
+extern "C" int psmma_hss_product_dev(void* hss, int m, double alpha, const double* a, int lda,
+                                     double beta, double* c, int ldc, void* stream) {
+  const Hss* h = (const Hss*)hss;
+  if (!h) return -1;
+  if (m < 0) return -2;
+  if (lda < std::max(1, m)) return -5;
+  if (ldc < std::max(1, m)) return -8;
+  if (m == 0) return 0;
+  if (!a) return -4;
+  if (!c) return -7;
+  return product(*h, m, alpha, a, lda, beta, c, ldc, (hipStream_t)stream);
+}
+
+extern "C" int psmma_hss_layout(void* hss, int* nnodes, int* parent, int* lo, int* size,
+                                int* udim, int* vdim, int* bdim, long long* offsets,
+                                const double** h_out) {
+  const Hss* h = (const Hss*)hss;
+  if (!h || !nnodes) return -1;
+  const int nn = (int)h->nodes.size();
+  *nnodes = nn;
+  if (h_out) *h_out = h->H;
+  for (const Node& nd : h->nodes) {
+    const int p = nd.post;
+    if (parent) parent[p] = nd.parent < 0 ? -1 : h->nodes[nd.parent].post;
+    if (lo) lo[p] = nd.lo;
+    if (size) size[p] = nd.n;
+    const int rcs = nd.sib >= 0 ? h->nodes[nd.sib].rc : 0;
+    if (udim) {
+      udim[2 * p] = nd.level ? nd.wr : 0;
+      udim[2 * p + 1] = nd.r;
+    }
+    if (vdim) {
+      vdim[2 * p] = nd.level ? nd.wc : 0;
+      vdim[2 * p + 1] = nd.rc;
+    }
+    if (bdim) {
+      bdim[2 * p] = nd.level ? nd.r : 0;
+      bdim[2 * p + 1] = nd.level ? rcs : 0;
+    }
+    if (offsets) {
+      offsets[3 * p] = nd.offU;
+      offsets[3 * p + 1] = nd.offV;
+      offsets[3 * p + 2] = nd.offB;
+    }
+  }
+  return 0;
+}
+
+extern "C" int psmma_hss_destroy(void* hss) {
+  Hss* h = (Hss*)hss;
+  if (!h) return 0;
+  if (h->H) {
+    hipDeviceSynchronize();
+    hipFree(h->H);
+  }
+  delete h;
+  return 0;
+}

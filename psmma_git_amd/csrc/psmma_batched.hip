@@ -13,7 +13,7 @@
 
 namespace psmma {
 hipError_t launch_batch_f64(const BatchProblem* dprobs, const int* dtbeg, int nprob, int ntiles,
-                            unsigned* ctr, int nworkgroups, hipStream_t s);
+                            unsigned* ctr, int nworkgroups, hipStream_t s, int kset);
 void batch_tile_shape(int kind, int cls, int* bm, int* bn);
 void* gsplit_create(const BatchProblem* probs, int nprob, int t, size_t* ws_bytes);
 int gsplit_bind(void* h, void* ws);
@@ -87,8 +87,14 @@ struct F64Dev {
   unsigned* ctr = nullptr;  // work counter and exit counter of the persistent launch
 };
 
-void build_f64(const std::vector<BatchProblem>& in, int big_min, F64Table& T) {
-  T = F64Table{};
+}  // namespace
+
+namespace psmma {
+// Tile list of a persistent batch launch: problems with m, n > 0 get their tile
+// class (large when min(m, n) >= big_min), tile counts and first tile, ordered
+// by the cost of one tile, longest first.  Returns the number of tiles.
+int batch_tiles(const std::vector<BatchProblem>& in, int big_min, std::vector<BatchProblem>& out,
+                std::vector<int>& tbeg) {
   std::vector<BatchProblem> all;
   for (auto b : in) {
     if (b.m == 0 || b.n == 0) continue;
@@ -107,12 +113,24 @@ void build_f64(const std::vector<BatchProblem>& in, int big_min, F64Table& T) {
   std::stable_sort(all.begin(), all.end(), [&](const BatchProblem& x, const BatchProblem& y) {
     return cost(x) > cost(y);
   });
+  out.clear();
+  tbeg.clear();
+  int ntiles = 0;
   for (auto& b : all) {
-    b.tile_begin = T.ntiles;
-    T.probs.push_back(b);
-    T.tbeg.push_back(T.ntiles);
-    T.ntiles += b.ntm * b.ntn;
+    b.tile_begin = ntiles;
+    out.push_back(b);
+    tbeg.push_back(ntiles);
+    ntiles += b.ntm * b.ntn;
   }
+  return ntiles;
+}
+}  // namespace psmma
+
+namespace {
+
+void build_f64(const std::vector<BatchProblem>& in, int big_min, F64Table& T) {
+  T = F64Table{};
+  T.ntiles = batch_tiles(in, big_min, T.probs, T.tbeg);
 }
 
 void free_f64(F64Dev& D) {
@@ -142,7 +160,7 @@ int enqueue_f64(const F64Table& T, const F64Dev& D, hipStream_t s, int* nodes) {
   int n = 0;
   if (T.ntiles > 0) {
     hipError_t e = launch_batch_f64(D.probs, D.tbeg, (int)T.probs.size(), T.ntiles, D.ctr,
-                                    device_workgroups(2), s);
+                                    device_workgroups(2), s, 0);
     if (e != hipSuccess) return 1000 + (int)e;
     n = 1;
   }

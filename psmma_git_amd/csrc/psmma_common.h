@@ -25,6 +25,12 @@ enum Kind : int {
   KIND_TOEPLITZ = 1,   // H(i,j) = G(K + j - i)  (1-based, G length 2K-1) psmma_toeplitz
   KIND_HANKEL = 2,     // H(i,j) = G(i + j - 1)                           psmma_hankel
   KIND_CAUCHYEIG = 3,  // H(i,j) = U_i V_j / (F_i - lambda_j), stable form psmma_cauchyeig
+  // Internal kinds (not part of the C ABI), used by the HSS construction and
+  // product: the transposes of the Cauchy kinds and explicit matrices.
+  KIND_CAUCHYEIG_T = 4,  // H(i,j) = Ceig(j,i)
+  KIND_CAUCHY_T = 5,     // H(i,j) = Ccauchy(j,i)
+  KIND_DENSE = 6,        // H(i,j) = p0[j * korder + i]   (column-major, ld = korder)
+  KIND_DENSE_T = 7,      // H(i,j) = p0[i * korder + j]   (the transpose of a column-major matrix)
 };
 
 // Generators of one structured operand.  Pointers are device pointers.
@@ -77,6 +83,23 @@ __device__ __forceinline__ double gen_entry<KIND_CAUCHYEIG>(const GenPtrs& g, in
   const double* F = g.p0;
   double den = (r <= c) ? ((F[r] - F[c]) - g.p3[c]) : ((F[r] - F[c + 1]) - g.p4[c]);
   return (g.p1[r] * g.p2[c]) / den;
+}
+
+template <>
+__device__ __forceinline__ double gen_entry<KIND_CAUCHYEIG_T>(const GenPtrs& g, int r, int c) {
+  return gen_entry<KIND_CAUCHYEIG>(g, c, r);
+}
+template <>
+__device__ __forceinline__ double gen_entry<KIND_CAUCHY_T>(const GenPtrs& g, int r, int c) {
+  return gen_entry<KIND_CAUCHY>(g, c, r);
+}
+template <>
+__device__ __forceinline__ double gen_entry<KIND_DENSE>(const GenPtrs& g, int r, int c) {
+  return g.p0[(size_t)c * g.korder + r];
+}
+template <>
+__device__ __forceinline__ double gen_entry<KIND_DENSE_T>(const GenPtrs& g, int r, int c) {
+  return g.p0[(size_t)r * g.korder + c];
 }
 
 __device__ __forceinline__ double gen_entry_rt(int kind, const GenPtrs& g, int r, int c) {

@@ -60,7 +60,9 @@ __device__ __forceinline__ void right_tile(int M, int N, int K, double alpha,
   // slab of a contiguous block needs only a window of BK+BN-1 generator values.
   // Index-list blocks generate every entry, as the Cauchy kinds do.
   constexpr bool WIN = (KIND == KIND_TOEPLITZ || KIND == KIND_HANKEL) && !IDX;
-  constexpr bool GATHER = (KIND == KIND_TOEPLITZ || KIND == KIND_HANKEL) && IDX;
+  // every entry gathered through gen_entry: index-list Toeplitz / Hankel blocks and the
+  // internal transposed and explicit kinds
+  constexpr bool GATHER = ((KIND == KIND_TOEPLITZ || KIND == KIND_HANKEL) && IDX) || KIND >= 4;
   constexpr int WLEN = BK + BN - 1;
   constexpr int WREG = (WLEN + NT - 1) / NT;
   constexpr int HREG = WIN ? 1 : BK * BN / NT;  // generated entries per thread per slab
@@ -152,7 +154,7 @@ __device__ __forceinline__ void right_tile(int M, int N, int K, double alpha,
           int gr = r0 + gk;
           if constexpr (IDX) gr = g.ridx[gk] - 1;
           if constexpr (GATHER) {
-            // index-list Toeplitz / Hankel entries are gathered straight from g
+            // gathered kinds: each entry straight from gen_entry
             hreg[q] = colok ? gen_entry<KIND>(g, gr, gcol) : 0.0;
           } else if constexpr (KIND == KIND_CAUCHY) {
             rowa[q] = g.p0[gr];  // D_r
@@ -552,6 +554,9 @@ constexpr int kBatchLds =
               cmax(tile_lds<KIND_CAUCHY, 128, 128, 8>(), tile_lds<KIND_CAUCHYEIG, 128, 128, 8>())),
          cmax(cmax(tile_lds<KIND_CAUCHY, 64, 64, 16>(), tile_lds<KIND_CAUCHYEIG, 64, 64, 16>()),
               cmax(tile_lds<KIND_TOEPLITZ, 64, 64, 16>(), tile_lds<KIND_HANKEL, 64, 64, 16>())));
+static_assert(tile_lds<KIND_DENSE, 128, 128, 8>() <= kBatchLds &&
+                  tile_lds<KIND_DENSE, 64, 64, 16>() <= kBatchLds,
+              "the HSS kinds use the Cauchy kinds' tile shapes");
 
 template <int KIND, int BM, int BN, int BK, int WGM, int WGN>
 __device__ __forceinline__ void batch_tile(const BatchProblem& p, int local, double* lds) {
@@ -573,6 +578,12 @@ __device__ __forceinline__ void batch_tile_kind(const BatchProblem& p, int local
   }
 }
 
+// Kind sets of the persistent batch kernel: the public kinds (batched API), and
+// the kinds of the HSS construction and product (Cauchy kinds, their transposes
+// and explicit matrices).  One kernel per set keeps each set's register budget.
+enum BatchKindSet : int { KSET_PUBLIC = 0, KSET_HSS = 1 };
+
+template <int KSET>
 __global__ __launch_bounds__(256, 2) void sgemm_f64_batch_kernel(
     const BatchProblem* __restrict__ probs, const int* __restrict__ tbeg, int nprob, int ntiles,
     unsigned* __restrict__ ctr) {
@@ -590,11 +601,22 @@ __global__ __launch_bounds__(256, 2) void sgemm_f64_batch_kernel(
     }
     const BatchProblem& p = probs[lo];
     const int local = t - tbeg[lo];
-    switch (p.kind) {
-      case KIND_CAUCHY: batch_tile_kind<KIND_CAUCHY>(p, local, lds); break;
-      case KIND_TOEPLITZ: batch_tile_kind<KIND_TOEPLITZ>(p, local, lds); break;
-      case KIND_HANKEL: batch_tile_kind<KIND_HANKEL>(p, local, lds); break;
-      default: batch_tile_kind<KIND_CAUCHYEIG>(p, local, lds); break;
+    if constexpr (KSET == KSET_PUBLIC) {
+      switch (p.kind) {
+        case KIND_CAUCHY: batch_tile_kind<KIND_CAUCHY>(p, local, lds); break;
+        case KIND_TOEPLITZ: batch_tile_kind<KIND_TOEPLITZ>(p, local, lds); break;
+        case KIND_HANKEL: batch_tile_kind<KIND_HANKEL>(p, local, lds); break;
+        default: batch_tile_kind<KIND_CAUCHYEIG>(p, local, lds); break;
+      }
+    } else {
+      switch (p.kind) {
+        case KIND_CAUCHY: batch_tile_kind<KIND_CAUCHY>(p, local, lds); break;
+        case KIND_CAUCHYEIG: batch_tile_kind<KIND_CAUCHYEIG>(p, local, lds); break;
+        case KIND_CAUCHYEIG_T: batch_tile_kind<KIND_CAUCHYEIG_T>(p, local, lds); break;
+        case KIND_CAUCHY_T: batch_tile_kind<KIND_CAUCHY_T>(p, local, lds); break;
+        case KIND_DENSE: batch_tile_kind<KIND_DENSE>(p, local, lds); break;
+        default: batch_tile_kind<KIND_DENSE_T>(p, local, lds); break;
+      }
     }
   }
   if (threadIdx.x == 0) {
@@ -607,13 +629,18 @@ __global__ __launch_bounds__(256, 2) void sgemm_f64_batch_kernel(
   }
 }
 
-// One launch over the whole tile list; ctr: two zeroed unsigned counters.
+// One launch over the whole tile list; ctr: two zeroed unsigned counters; kset:
+// 0 for the public kinds, 1 for the HSS kinds.
 hipError_t launch_batch_f64(const BatchProblem* dprobs, const int* dtbeg, int nprob, int ntiles,
-                            unsigned* ctr, int nworkgroups, hipStream_t s) {
+                            unsigned* ctr, int nworkgroups, hipStream_t s, int kset) {
   if (ntiles <= 0) return hipSuccess;
   const int g = ntiles < nworkgroups ? ntiles : nworkgroups;
-  hipLaunchKernelGGL(sgemm_f64_batch_kernel, dim3(g), dim3(256), 0, s, dprobs, dtbeg, nprob, ntiles,
-                     ctr);
+  if (kset == KSET_HSS)
+    hipLaunchKernelGGL(sgemm_f64_batch_kernel<KSET_HSS>, dim3(g), dim3(256), 0, s, dprobs, dtbeg,
+                       nprob, ntiles, ctr);
+  else
+    hipLaunchKernelGGL(sgemm_f64_batch_kernel<KSET_PUBLIC>, dim3(g), dim3(256), 0, s, dprobs,
+                       dtbeg, nprob, ntiles, ctr);
   return hipGetLastError();
 }
 

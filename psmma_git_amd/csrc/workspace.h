@@ -13,6 +13,8 @@ enum WsSlot : int {
   WS_BLR_GEMM = 2,  // compressed products: rank-dependent GEMM operands
   WS_RING = 3,      // ring steps and redistributions: index lists, offset tables
   WS_TUNE = 4,      // split-depth measurement: sampled FP64 and split results
+  WS_HSS_TABLES = 5,  // HSS stages: problem tables of the persistent launches
+  WS_HSS_WORK = 6,    // HSS product: the up- and down-pass matrices G and F
 };
 
 // Returns a buffer of at least `bytes` for (current device, s, slot), or nullptr

@@ -3,6 +3,7 @@
 Argument-check paths of the Fortran entry points return before any HIP call,
 so the reference's INFO codes can be exercised on the CPU too.
 """
+import ctypes
 import re
 from pathlib import Path
 
@@ -151,3 +152,26 @@ def test_fortran_caller_on_a_larger_grid_is_refused():
     assert vals == {"toeplitz_status": "-2", "pzmdftl1_status": "-2", "pzmdftl_status": "-2",
                     "outputs_untouched": "1"}
     assert r.stderr.count("process grid 2 x 2") == 3
+
+
+def test_hss_build_argument_codes():
+    """Argument checks of psmma_hss_build_dev return before any device call (the
+    pointers below are never dereferenced)."""
+    L = api.lib()
+    p = ctypes.c_void_p(64)
+    h = ctypes.c_void_p()
+    st = (ctypes.c_longlong * 6)()
+
+    def call(kind=api.KIND_CAUCHYEIG, n=1024, gens=(p,) * 5, leaf=256, tol=1e-12, samples=0):
+        return L.psmma_hss_build_dev(kind, n, *gens, leaf, tol, samples, 0, ctypes.byref(h), st,
+                                     None)
+
+    assert call(kind=api.KIND_TOEPLITZ) == -1
+    assert call(n=0) == -2
+    assert call(gens=(p, p, p, p, None)) == -3  # eigen-Cauchy needs DIFR
+    assert call(leaf=300) == -8                  # leaf above 256
+    assert call(n=768) == -8                     # three leaves: not a power of two
+    assert call(tol=-1.0) == -9
+    assert call(samples=129) == -10
+    assert L.psmma_hss_product_dev(None, 4, 1.0, p, 4, 0.0, p, 4, None) == -1
+    assert L.psmma_hss_destroy(None) == 0
