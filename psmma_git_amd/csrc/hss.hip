@@ -48,7 +48,7 @@
 
 namespace psmma {
 int batch_tiles(const std::vector<BatchProblem>& in, int big_min, std::vector<BatchProblem>& out,
-                std::vector<int>& tbeg);
+                std::vector<int>& tbeg, int kset);
 hipError_t launch_batch_f64(const BatchProblem* dprobs, const int* dtbeg, int nprob, int ntiles,
                             unsigned* ctr, int nworkgroups, hipStream_t s, int kset);
 }  // namespace psmma
@@ -308,7 +308,7 @@ int run_stages(const std::vector<std::vector<BatchProblem>>& stages, hipStream_t
   auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
   size_t bytes = 0;
   for (size_t i = 0; i < stages.size(); ++i) {
-    ntiles[i] = batch_tiles(stages[i], 256, probs[i], tbeg[i]);
+    ntiles[i] = batch_tiles(stages[i], 256, probs[i], tbeg[i], kKsetHss);
     bytes += al(sizeof(BatchProblem) * probs[i].size()) + al(sizeof(int) * tbeg[i].size()) +
              al(2 * sizeof(unsigned));
   }

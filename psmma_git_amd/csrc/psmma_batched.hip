@@ -14,7 +14,7 @@
 namespace psmma {
 hipError_t launch_batch_f64(const BatchProblem* dprobs, const int* dtbeg, int nprob, int ntiles,
                             unsigned* ctr, int nworkgroups, hipStream_t s, int kset);
-void batch_tile_shape(int kind, int cls, int* bm, int* bn);
+void batch_tile_shape(int kset, int kind, int cls, int* bm, int* bn);
 void* gsplit_create(const BatchProblem* probs, int nprob, int t, size_t* ws_bytes);
 int gsplit_bind(void* h, void* ws);
 int gsplit_enqueue(void* h, hipStream_t s, int* nodes);
@@ -94,20 +94,20 @@ namespace psmma {
 // class (large when min(m, n) >= big_min), tile counts and first tile, ordered
 // by the cost of one tile, longest first.  Returns the number of tiles.
 int batch_tiles(const std::vector<BatchProblem>& in, int big_min, std::vector<BatchProblem>& out,
-                std::vector<int>& tbeg) {
+                std::vector<int>& tbeg, int kset) {
   std::vector<BatchProblem> all;
   for (auto b : in) {
     if (b.m == 0 || b.n == 0) continue;
     b.cls = std::min(b.m, b.n) >= big_min ? 1 : 0;
     int bm, bn;
-    batch_tile_shape(b.kind, b.cls, &bm, &bn);
+    batch_tile_shape(kset, b.kind, b.cls, &bm, &bn);
     b.ntm = (b.m + bm - 1) / bm;
     b.ntn = (b.n + bn - 1) / bn;
     all.push_back(b);
   }
-  auto cost = [](const BatchProblem& b) {
+  auto cost = [kset](const BatchProblem& b) {
     int bm, bn;
-    batch_tile_shape(b.kind, b.cls, &bm, &bn);
+    batch_tile_shape(kset, b.kind, b.cls, &bm, &bn);
     return (long long)std::max(b.k, 1) * bm * bn;
   };
   std::stable_sort(all.begin(), all.end(), [&](const BatchProblem& x, const BatchProblem& y) {
@@ -130,7 +130,7 @@ namespace {
 
 void build_f64(const std::vector<BatchProblem>& in, int big_min, F64Table& T) {
   T = F64Table{};
-  T.ntiles = batch_tiles(in, big_min, T.probs, T.tbeg);
+  T.ntiles = batch_tiles(in, big_min, T.probs, T.tbeg, 0);
 }
 
 void free_f64(F64Dev& D) {

@@ -533,8 +533,20 @@ __global__ __launch_bounds__(WGM* WGN * 64, 2) void dft_f64_left_kernel(
 // their uniform arguments in VGPRs).  Measured on MI355X, 512-problem batch:
 // 19.4 ms against 20.2 ms for one launch per (kind, class) and 22.5 ms with the
 // large bodies called.  All classes use 256 threads.
-void batch_tile_shape(int kind, int cls, int* bm, int* bn) {
-  if (cls == 1) {
+// Tile shapes of the HSS kind set (the Cauchy kinds, their transposes and
+// explicit matrices; short contractions of K = leaf or a rank): class 1 for
+// problems of at least 256 rows and columns, class 0 below.
+struct HssTiles {
+  static constexpr int BM1 = 128, BN1 = 128, BK1 = 8, WM1 = 2, WN1 = 2;
+  static constexpr int BM0 = 64, BN0 = 64, BK0 = 16, WM0 = 2, WN0 = 2;
+  static constexpr int MIN_BLOCKS = 2;  // workgroups per CU the register budget allows
+};
+
+void batch_tile_shape(int kset, int kind, int cls, int* bm, int* bn) {
+  if (kset == 1) {
+    *bm = cls == 1 ? HssTiles::BM1 : HssTiles::BM0;
+    *bn = cls == 1 ? HssTiles::BN1 : HssTiles::BN0;
+  } else if (cls == 1) {
     const bool win = kind == KIND_TOEPLITZ || kind == KIND_HANKEL;
     *bm = win ? 64 : 128;
     *bn = win ? 256 : 128;
@@ -554,9 +566,13 @@ constexpr int kBatchLds =
               cmax(tile_lds<KIND_CAUCHY, 128, 128, 8>(), tile_lds<KIND_CAUCHYEIG, 128, 128, 8>())),
          cmax(cmax(tile_lds<KIND_CAUCHY, 64, 64, 16>(), tile_lds<KIND_CAUCHYEIG, 64, 64, 16>()),
               cmax(tile_lds<KIND_TOEPLITZ, 64, 64, 16>(), tile_lds<KIND_HANKEL, 64, 64, 16>())));
-static_assert(tile_lds<KIND_DENSE, 128, 128, 8>() <= kBatchLds &&
-                  tile_lds<KIND_DENSE, 64, 64, 16>() <= kBatchLds,
-              "the HSS kinds use the Cauchy kinds' tile shapes");
+constexpr int kHssLds = cmax(
+    cmax(cmax(tile_lds<KIND_DENSE, HssTiles::BM1, HssTiles::BN1, HssTiles::BK1>(),
+              tile_lds<KIND_CAUCHYEIG, HssTiles::BM1, HssTiles::BN1, HssTiles::BK1>()),
+         tile_lds<KIND_CAUCHY, HssTiles::BM1, HssTiles::BN1, HssTiles::BK1>()),
+    cmax(cmax(tile_lds<KIND_DENSE, HssTiles::BM0, HssTiles::BN0, HssTiles::BK0>(),
+              tile_lds<KIND_CAUCHYEIG, HssTiles::BM0, HssTiles::BN0, HssTiles::BK0>()),
+         tile_lds<KIND_CAUCHY, HssTiles::BM0, HssTiles::BN0, HssTiles::BK0>()));
 
 template <int KIND, int BM, int BN, int BK, int WGM, int WGN>
 __device__ __forceinline__ void batch_tile(const BatchProblem& p, int local, double* lds) {
@@ -578,16 +594,26 @@ __device__ __forceinline__ void batch_tile_kind(const BatchProblem& p, int local
   }
 }
 
+template <int KIND>
+__device__ __forceinline__ void hss_tile_kind(const BatchProblem& p, int local, double* lds) {
+  using T = HssTiles;
+  if (p.cls == 1)
+    batch_tile<KIND, T::BM1, T::BN1, T::BK1, T::WM1, T::WN1>(p, local, lds);
+  else
+    batch_tile<KIND, T::BM0, T::BN0, T::BK0, T::WM0, T::WN0>(p, local, lds);
+}
+
 // Kind sets of the persistent batch kernel: the public kinds (batched API), and
 // the kinds of the HSS construction and product (Cauchy kinds, their transposes
 // and explicit matrices).  One kernel per set keeps each set's register budget.
 enum BatchKindSet : int { KSET_PUBLIC = 0, KSET_HSS = 1 };
 
 template <int KSET>
-__global__ __launch_bounds__(256, 2) void sgemm_f64_batch_kernel(
+__global__ __launch_bounds__(256, KSET == KSET_PUBLIC ? 2 : HssTiles::MIN_BLOCKS) void
+sgemm_f64_batch_kernel(
     const BatchProblem* __restrict__ probs, const int* __restrict__ tbeg, int nprob, int ntiles,
     unsigned* __restrict__ ctr) {
-  __shared__ double lds[kBatchLds];
+  __shared__ double lds[KSET == KSET_PUBLIC ? kBatchLds : kHssLds];
   __shared__ int tile_s;
   for (;;) {
     if (threadIdx.x == 0) tile_s = (int)atomicAdd(&ctr[0], 1u);
@@ -610,12 +636,12 @@ __global__ __launch_bounds__(256, 2) void sgemm_f64_batch_kernel(
       }
     } else {
       switch (p.kind) {
-        case KIND_CAUCHY: batch_tile_kind<KIND_CAUCHY>(p, local, lds); break;
-        case KIND_CAUCHYEIG: batch_tile_kind<KIND_CAUCHYEIG>(p, local, lds); break;
-        case KIND_CAUCHYEIG_T: batch_tile_kind<KIND_CAUCHYEIG_T>(p, local, lds); break;
-        case KIND_CAUCHY_T: batch_tile_kind<KIND_CAUCHY_T>(p, local, lds); break;
-        case KIND_DENSE: batch_tile_kind<KIND_DENSE>(p, local, lds); break;
-        default: batch_tile_kind<KIND_DENSE_T>(p, local, lds); break;
+        case KIND_CAUCHY: hss_tile_kind<KIND_CAUCHY>(p, local, lds); break;
+        case KIND_CAUCHYEIG: hss_tile_kind<KIND_CAUCHYEIG>(p, local, lds); break;
+        case KIND_CAUCHYEIG_T: hss_tile_kind<KIND_CAUCHYEIG_T>(p, local, lds); break;
+        case KIND_CAUCHY_T: hss_tile_kind<KIND_CAUCHY_T>(p, local, lds); break;
+        case KIND_DENSE: hss_tile_kind<KIND_DENSE>(p, local, lds); break;
+        default: hss_tile_kind<KIND_DENSE_T>(p, local, lds); break;
       }
     }
   }
