@@ -221,3 +221,58 @@ def test_faithful_eig_ring_matches_dense(ora, P, Q, nb):
     assert st[1] + st[2] > 0 and st[3] == 0
     assert ora.rel_fro(dense, a @ c) < 1e-14
     assert ora.rel_fro(faith, dense) < 1e-14
+
+
+# The swap pass (every prd-th step) repeats until no interpolation coefficient
+# |Z_i W_j / (D_i - D_j)| of the chosen rows exceeds rho.  When the rank is a
+# multiple of prd, the last step ran a pass and nothing changed the generators
+# after it, so the returned interpolation matrix is bounded by rho; at other
+# ranks it is not.  Over a seeded family this pins rho from both sides (no
+# coefficient above it; some between the two kinds' constants) and the period
+# (ranks that are a multiple of a shorter period are not bounded).
+RHO_CAUCHY = 1.12                   # SRC/Cauchylowrank.f90:70-71 (rho = 1.12D0, prd = 10)
+RHO_EIG = float(np.float32(1.2))    # SRC/CauchyHssEig.f90:98-99 (rho = 1.2E+0, prd = 5)
+
+
+def test_rrlu_interpolation_bounded_by_rho_after_the_swap_pass(ora):
+    rng = np.random.default_rng(0)
+    ranks, maxima = [], []
+    for _ in range(150):
+        m, n = 60, 50
+        d = np.sort(rng.random(m))
+        f = 1.0 + rng.uniform(0.01, 0.3) + np.sort(rng.random(n))
+        u, v = rng.standard_normal(m), rng.standard_normal(n)
+        for tol in (1e-4, 1e-6, 1e-8, 1e-10, 1e-12, 1e-14):
+            rk, x, _, _ = ora.compress_cauchy(d, f, u, v, tol)
+            if x is not None:
+                ranks.append(rk)
+                maxima.append(np.abs(x).max())
+    ranks, maxima = np.array(ranks), np.array(maxima)
+    after_pass = maxima[ranks % 10 == 0]
+    assert len(after_pass) >= 40
+    assert after_pass.max() <= RHO_CAUCHY * (1 + 1e-12)
+    assert after_pass.max() > 1.05                        # the bound is approached, not slack
+    assert np.sum(maxima[ranks % 10 == 5] > RHO_CAUCHY) >= 5  # no pass at the 5th step
+    assert maxima.max() > 1.3                              # unbounded between passes
+
+
+def test_eig_rrlu_interpolation_bounded_by_rho_after_the_swap_pass(ora):
+    ranks, maxima = [], []
+    for seed in range(12):
+        n = 240
+        f, dl, dr, zh, v = _eig(n, seed)
+        for r0, r1, c0, c1 in [(0, 80, 120, 240), (0, 100, 140, 240), (160, 240, 0, 100),
+                               (40, 120, 150, 240), (0, 60, 60, 180)]:
+            rows, cols = np.arange(r0, r1), np.arange(c0, c1)
+            for tol in (1e-4, 1e-6, 1e-8, 1e-10, 1e-12, 1e-14, ora.RRLU_TOL):
+                rk, q, _, _ = ora.compress_cauchyeig2(f, dl, dr, zh, v, rows, cols, tol)
+                if q is not None:
+                    ranks.append(rk)
+                    maxima.append(np.abs(q).max())
+    ranks, maxima = np.array(ranks), np.array(maxima)
+    after_pass = maxima[ranks % 5 == 0]
+    assert len(after_pass) >= 40
+    assert after_pass.max() <= RHO_EIG * (1 + 1e-12)
+    # some coefficients sit between the plain kind's 1.12 and 1.2: the constants differ
+    assert np.sum(after_pass > RHO_CAUCHY) >= 3
+    assert np.sum(maxima[ranks % 5 != 0] > RHO_EIG) >= 10  # unbounded between passes
