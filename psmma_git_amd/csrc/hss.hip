@@ -749,18 +749,32 @@ int product(const Hss& h, int m, double alpha, const double* a, int lda, double 
                                          h.H + nd.offV, nd.wc, 1.0, fcol(l + 1, 2 * (int)i), m));
     }
   }
-  // leaves: X(:, I) = alpha (A(:, I) D + F V^T) + beta X(:, I)
-  stg.emplace_back();
-  for (int id : h.lev[L]) {
-    const Node& nd = h.nodes[id];
-    stg.back().push_back(gen_problem(h, h.kind, m, nd.n, nd.n, alpha, a + (size_t)nd.lo * lda,
-                                     lda, nd.lo, nd.lo, beta, c + (size_t)nd.lo * ldc, ldc));
-  }
+  // leaves: X(:, I) = alpha (A(:, I) D + F V^T) + beta X(:, I) in one launch: the
+  // contraction runs over the leaf's generated diagonal block, then over the explicit
+  // tail F V^T (the leaf is a multiple of both tile classes' slab depths); otherwise
+  // in two launches
+  const bool fused = h.leaf % 16 == 0;
   stg.emplace_back();
   for (size_t i = 0; i < h.lev[L].size(); ++i) {
     const Node& nd = h.nodes[h.lev[L][i]];
-    stg.back().push_back(dense_problem(KIND_DENSE_T, m, nd.n, nd.rc, alpha, fcol(L, (int)i), m,
-                                       h.H + nd.offV, nd.wc, 1.0, c + (size_t)nd.lo * ldc, ldc));
+    BatchProblem b = gen_problem(h, h.kind, m, nd.n, nd.n, alpha, a + (size_t)nd.lo * lda, lda,
+                                 nd.lo, nd.lo, beta, c + (size_t)nd.lo * ldc, ldc);
+    if (fused && nd.rc > 0) {
+      b.A2 = fcol(L, (int)i);
+      b.lda2 = m;
+      b.H2 = h.H + nd.offV;
+      b.ldh2 = nd.wc;
+      b.k2 = nd.rc;
+    }
+    stg.back().push_back(b);
+  }
+  if (!fused) {
+    stg.emplace_back();
+    for (size_t i = 0; i < h.lev[L].size(); ++i) {
+      const Node& nd = h.nodes[h.lev[L][i]];
+      stg.back().push_back(dense_problem(KIND_DENSE_T, m, nd.n, nd.rc, alpha, fcol(L, (int)i), m,
+                                         h.H + nd.offV, nd.wc, 1.0, c + (size_t)nd.lo * ldc, ldc));
+    }
   }
   return run_stages(stg, strm);
 }

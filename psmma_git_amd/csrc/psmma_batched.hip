@@ -108,7 +108,7 @@ int batch_tiles(const std::vector<BatchProblem>& in, int big_min, std::vector<Ba
   auto cost = [kset](const BatchProblem& b) {
     int bm, bn;
     batch_tile_shape(kset, b.kind, b.cls, &bm, &bn);
-    return (long long)std::max(b.k, 1) * bm * bn;
+    return (long long)std::max(b.k + b.k2, 1) * bm * bn;
   };
   std::stable_sort(all.begin(), all.end(), [&](const BatchProblem& x, const BatchProblem& y) {
     return cost(x) > cost(y);

@@ -162,6 +162,23 @@ struct BatchProblem {
   int tile_begin;  // first index of this problem's tiles in the batch tile list
   int ntm, ntn;    // tiles along m and n
   int cls;         // tile class: 0 small (64 x 64), 1 large (shape by kind)
+  // Optional explicit tail of the contraction (generated kinds of the HSS kind set):
+  // C = alpha * (A * H + A2 * H2) + beta * C with A2 m x k2 (ld lda2) and
+  // H2(k, c) = H2[k * ldh2 + c]; k must then be a multiple of the tile's slab depth.
+  const double* A2;
+  int lda2;
+  const double* H2;
+  int ldh2;
+  int k2;
+};
+
+// The explicit tail segment as the tile body takes it.
+struct TailSeg {
+  const double* A = nullptr;
+  int lda = 1;
+  const double* H = nullptr;
+  int ldh = 1;
+  int K = 0;
 };
 
 }  // namespace psmma

@@ -45,12 +45,14 @@ struct RightLds {
 // H given by the 1-based index lists of g; r0, c0 unused).
 // As/Hs: LDS areas of 2 * RightLds::ASZ and 2 * RightLds::HSZ doubles.  Ends
 // with all waves past a barrier, so the caller may reuse the LDS at once.
-template <int KIND, int BM, int BN, int BK, int WGM, int WGN, bool IDX = false>
+// With TAIL (generated kinds, K a multiple of BK) the contraction continues past K
+// with the explicit segment tl: C = alpha * (A * H + tl.A * tl.H) + beta * C.
+template <int KIND, int BM, int BN, int BK, int WGM, int WGN, bool IDX = false, bool TAIL = false>
 __device__ __forceinline__ void right_tile(int M, int N, int K, double alpha,
                                            const double* __restrict__ A, int lda, const GenPtrs& g,
                                            int r0, int c0, double beta, double* __restrict__ C,
                                            int ldc, int m0, int n0, double* __restrict__ As_,
-                                           double* __restrict__ Hs_) {
+                                           double* __restrict__ Hs_, const TailSeg& tl = TailSeg{}) {
   constexpr int NT = WGM * WGN * 64;
   constexpr int WM = BM / WGM, WN = BN / WGN;
   constexpr int TM = WM / 16, TN = WN / 16;
@@ -71,6 +73,7 @@ __device__ __forceinline__ void right_tile(int M, int N, int K, double alpha,
   static_assert(BM * BK % NT == 0 && BK * BN % NT == 0, "tile/threads mismatch");
   static_assert(WIN || NT % BN == 0, "generated tiles assume a fixed column per thread");
   static_assert(WM % 16 == 0 && WN % 16 == 0 && BK % 4 == 0, "bad wave tile");
+  static_assert(!TAIL || (!WIN && !GATHER && !IDX), "the explicit tail follows a generated segment");
 
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
@@ -123,6 +126,19 @@ __device__ __forceinline__ void right_tile(int M, int N, int K, double alpha,
   }
 
   auto load_a = [&](int k0) {
+    if constexpr (TAIL) {
+      if (k0 >= K) {  // slab of the explicit tail (uniform)
+        const int t0 = k0 - K;
+#pragma unroll
+        for (int q = 0; q < AREG; ++q) {
+          int idx = tid + q * NT;
+          int row = idx % BM, col = idx / BM;
+          int gr = m0 + row, gc = t0 + col;
+          areg[q] = (gr < M && gc < tl.K) ? tl.A[(size_t)gc * tl.lda + gr] : 0.0;
+        }
+        return;
+      }
+    }
 #pragma unroll
     for (int q = 0; q < AREG; ++q) {
       int idx = tid + q * NT;
@@ -133,6 +149,16 @@ __device__ __forceinline__ void right_tile(int M, int N, int K, double alpha,
   };
   // Issue the generator loads of slab k0 (consumed after the MFMA loop).
   auto fetch_h = [&](int k0) {
+    if constexpr (TAIL) {
+      if (k0 >= K) {  // explicit tail: H2 rows k0 - K.., this thread's column
+#pragma unroll
+        for (int q = 0; q < HREG; ++q) {
+          int gk = k0 - K + tid / BN + q * (NT / BN);
+          hreg[q] = (gk < tl.K && colok) ? tl.H[(size_t)gk * tl.ldh + (n0 + hcol)] : 0.0;
+        }
+        return;
+      }
+    }
     if constexpr (WIN) {
       long long base;
       if constexpr (KIND == KIND_TOEPLITZ)
@@ -174,6 +200,9 @@ __device__ __forceinline__ void right_tile(int M, int N, int K, double alpha,
   };
   // Evaluate the prefetched slab (the FP64 divisions of the Cauchy kinds).
   auto finish_h = [&](int k0) {
+    if constexpr (TAIL) {
+      if (k0 >= K) return;  // the tail's entries were loaded as they are
+    }
     if constexpr (!WIN && !GATHER) {
 #pragma unroll
       for (int q = 0; q < HREG; ++q) {
@@ -215,7 +244,7 @@ __device__ __forceinline__ void right_tile(int M, int N, int K, double alpha,
     }
   };
 
-  const int nkt = (K + BK - 1) / BK;
+  const int nkt = (K + BK - 1) / BK + (TAIL ? (tl.K + BK - 1) / BK : 0);
   if (nkt > 0) {
     load_a(0);
     fetch_h(0);
@@ -574,12 +603,14 @@ constexpr int kHssLds = cmax(
               tile_lds<KIND_CAUCHYEIG, HssTiles::BM0, HssTiles::BN0, HssTiles::BK0>()),
          tile_lds<KIND_CAUCHY, HssTiles::BM0, HssTiles::BN0, HssTiles::BK0>()));
 
-template <int KIND, int BM, int BN, int BK, int WGM, int WGN>
+template <int KIND, int BM, int BN, int BK, int WGM, int WGN, bool TAIL = false>
 __device__ __forceinline__ void batch_tile(const BatchProblem& p, int local, double* lds) {
   using L = RightLds<KIND, BM, BN, BK>;
-  right_tile<KIND, BM, BN, BK, WGM, WGN>(p.m, p.n, p.k, p.alpha, p.A, p.lda, p.g, p.r0, p.c0,
-                                         p.beta, p.C, p.ldc, (local / p.ntn) * BM,
-                                         (local % p.ntn) * BN, lds, lds + 2 * L::ASZ);
+  TailSeg tl;
+  if constexpr (TAIL) tl = TailSeg{p.A2, p.lda2, p.H2, p.ldh2, p.k2};
+  right_tile<KIND, BM, BN, BK, WGM, WGN, false, TAIL>(
+      p.m, p.n, p.k, p.alpha, p.A, p.lda, p.g, p.r0, p.c0, p.beta, p.C, p.ldc,
+      (local / p.ntn) * BM, (local % p.ntn) * BN, lds, lds + 2 * L::ASZ, tl);
 }
 
 template <int KIND>
@@ -597,6 +628,15 @@ __device__ __forceinline__ void batch_tile_kind(const BatchProblem& p, int local
 template <int KIND>
 __device__ __forceinline__ void hss_tile_kind(const BatchProblem& p, int local, double* lds) {
   using T = HssTiles;
+  if constexpr (KIND == KIND_CAUCHY || KIND == KIND_CAUCHYEIG) {
+    if (p.k2 > 0) {  // generated diagonal block plus an explicit tail
+      if (p.cls == 1)
+        batch_tile<KIND, T::BM1, T::BN1, T::BK1, T::WM1, T::WN1, true>(p, local, lds);
+      else
+        batch_tile<KIND, T::BM0, T::BN0, T::BK0, T::WM0, T::WN0, true>(p, local, lds);
+      return;
+    }
+  }
   if (p.cls == 1)
     batch_tile<KIND, T::BM1, T::BN1, T::BK1, T::WM1, T::WN1>(p, local, lds);
   else
