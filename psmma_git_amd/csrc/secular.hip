@@ -17,7 +17,7 @@
 namespace {
 
 constexpr int kWaves = 4;      // roots per 256-thread workgroup
-constexpr int kMaxIter = 320;  // the safeguard halves the bracket at least every 2 steps
+constexpr int kMaxIter = 200;  // 24 model steps, then bisection to full precision
 
 // Sum over the wave; every lane returns lane 0's value so that wave-uniform
 // control flow sees one number (lanes would otherwise differ in the last bit).
@@ -44,10 +44,19 @@ __device__ __forceinline__ double lam_minus_d(const double* d, const double* dif
 // Root j of f(lam) = 1 + rho sum_i z_i^2 / (d_i - lam), d strictly increasing,
 // rho > 0: lam_j in (d_j, d_{j+1}), the last one in (d_n, d_n + rho z^T z).
 // The secular function at the interval midpoint picks the nearer pole as the
-// origin s; the root is found as tau = lam - s on the shifted poles d_i - s by
-// Newton's method inside a bisection bracket (f is increasing in tau between
-// the poles), so DIFL_j = lam_j - d_j and DIFR_j = lam_j - d_{j+1} keep a small
-// relative error however close the root is to a pole.
+// origin s; the root is found as tau = lam - s on the shifted poles d_i - s, so
+// DIFL_j = lam_j - d_j and DIFR_j = lam_j - d_{j+1} keep a small relative error
+// however close the root is to a pole.
+//
+// Iteration: the rational model of Bunch, Nielsen and Sorensen (Numer. Math. 31
+// (1978) 31-48) in the form LAPACK's DLAED4 uses: with psi / phi the sums over
+// the poles left / right of the interval and Dl, Dr the two interval poles minus
+// the current point, f is matched by c + s / (Dl - eta) + S / (Dr - eta) with the
+// derivatives of psi and phi, and the root of that model is the next step
+// (quadratic convergence; one pole for the last root).  f is increasing in tau
+// between the poles, so the sign of f keeps a bracket; a step leaving it, or a
+// run of 24 model steps without convergence, falls back to bisection, which
+// then halves the bracket to full precision.
 __global__ __launch_bounds__(256) void secular_roots_kernel(int n, const double* __restrict__ d,
                                                             const double* __restrict__ z,
                                                             double rho, double* __restrict__ difl,
@@ -58,36 +67,70 @@ __global__ __launch_bounds__(256) void secular_roots_kernel(int n, const double*
   double zz = 0.0;
   for (int i = lane; i < n; i += 64) zz += z[i] * z[i];
   zz = wave_sum(zz);
+  const bool last = j + 1 >= n;
   const double lo = d[j];
-  const double hi = (j + 1 < n) ? d[j + 1] : d[n - 1] + rho * zz;
+  const double hi = last ? d[n - 1] + rho * zz : d[j + 1];
   const double mid = 0.5 * (lo + hi);
   double s = 0.0;
   for (int i = lane; i < n; i += 64) s += z[i] * z[i] / (d[i] - mid);
   const bool left = 1.0 + rho * wave_sum(s) >= 0.0;  // root left of the midpoint
   const double org = left ? lo : hi;
+  const double pl = left ? 0.0 : lo - hi;  // the interval's poles, shifted
+  const double pr = left ? hi - lo : 0.0;
   double a = left ? 0.0 : lo - hi;
   double b = left ? mid - lo : 0.0;
   double tau = 0.5 * (a + b);
-  double w1 = b - a, w2 = b - a;  // bracket widths one and two steps back
   const double eps = 1.1102230246251565e-16;
   for (int it = 0; it < kMaxIter; ++it) {
-    double s1 = 0.0, s2 = 0.0;
+    double psi = 0.0, dpsi = 0.0, phi = 0.0, dphi = 0.0;
     for (int i = lane; i < n; i += 64) {
       const double zi = z[i];
       const double r = 1.0 / ((d[i] - org) - tau);
       const double w = zi * zi * r;
-      s1 += w;
-      s2 += w * r;
+      if (i <= j) {
+        psi += w;
+        dpsi += w * r;
+      } else {
+        phi += w;
+        dphi += w * r;
+      }
     }
-    const double g = 1.0 + rho * wave_sum(s1);
-    const double gp = rho * wave_sum(s2);
-    if (g == 0.0) break;
+    psi = wave_sum(psi);
+    dpsi = wave_sum(dpsi);
+    phi = wave_sum(phi);
+    dphi = wave_sum(dphi);
+    const double g = 1.0 + rho * (psi + phi);
+    // f is zero within its rounding error: psi sums negative terms and phi
+    // positive ones, so 1 + rho (phi - psi) bounds the magnitudes summed
+    if (fabs(g) <= 8.0 * eps * (1.0 + rho * (phi - psi))) break;
     if (g < 0.0) a = tau; else b = tau;
-    double nt = tau - g / gp;
+    const double dl = pl - tau, dr = pr - tau;  // interval poles minus the point
+    double eta;
+    if (last) {  // one pole: g ~ c + s / (dl - eta), s = dl^2 rho dpsi
+      const double sl = rho * dpsi * dl * dl;
+      const double c = g - rho * dpsi * dl;
+      eta = dl + sl / c;
+    } else {
+      const double A = (dl + dr) * g - dl * dr * rho * (dpsi + dphi);
+      const double B = dl * dr * g;
+      const double C = g - dl * rho * dpsi - dr * rho * dphi;
+      const double disc = sqrt(fabs(A * A - 4.0 * B * C));
+      if (C == 0.0)
+        eta = B / A;
+      else if (A <= 0.0)
+        eta = (A - disc) / (2.0 * C);
+      else
+        eta = 2.0 * B / (A + disc);
+    }
+    // converged: the model's step is below the resolution of tau (the step may
+    // land on a bracket end, which a bisection would throw away)
+    if (fabs(eta) <= 2.0 * eps * fabs(tau)) {
+      tau += eta;
+      break;
+    }
+    double nt = tau + eta;
+    if (!(nt > a && nt < b) || it >= 24) nt = 0.5 * (a + b);
     const double w = b - a;
-    if (!(nt > a && nt < b) || w > 0.5 * w2) nt = 0.5 * (a + b);
-    w2 = w1;
-    w1 = w;
     const bool done = fabs(nt - tau) <= 2.0 * eps * fabs(nt) ||
                       w <= 4.0 * eps * fmax(fabs(a), fabs(b));
     tau = nt;
