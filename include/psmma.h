@@ -475,7 +475,7 @@ int psmma_cauchy_blr_dev(int m, int n, double alpha, const double* a, int lda, c
  * HSSMM generators, SRC/aux_hss.f90:27-38), each node's bases from an
  * interpolative decomposition of `samples` (1..128) random samples of its
  * block row / column, truncated at |R_kk| <= tol |R_00|; samples = 0 chooses
- * the count: 64, then 96, then 128 while some node's rank comes within 10 of
+ * the count: 64 (96 from n = 16384), then 96, then 128 while some node's rank comes within 10 of
  * it (the oversampling margin).  `seed` fixes the samples.  The generators stay
  * on the device (the generator arrays must outlive the handle); the device is
  * synchronised.  stats (may be NULL, 6 entries): levels, largest rank, sum of

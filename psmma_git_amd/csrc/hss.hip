@@ -301,14 +301,16 @@ struct Hss {
 namespace {
 
 // Uploads the stage tables and runs each stage as one persistent launch (in order on s).
-int run_stages(const std::vector<std::vector<BatchProblem>>& stages, hipStream_t s) {
+// big_min: the smaller side from which a problem takes the 128 x 128 tile class.
+int run_stages(const std::vector<std::vector<BatchProblem>>& stages, hipStream_t s,
+               int big_min = 256) {
   std::vector<std::vector<BatchProblem>> probs(stages.size());
   std::vector<std::vector<int>> tbeg(stages.size());
   std::vector<int> ntiles(stages.size());
   auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
   size_t bytes = 0;
   for (size_t i = 0; i < stages.size(); ++i) {
-    ntiles[i] = batch_tiles(stages[i], 256, probs[i], tbeg[i], kKsetHss);
+    ntiles[i] = batch_tiles(stages[i], big_min, probs[i], tbeg[i], kKsetHss);
     bytes += al(sizeof(BatchProblem) * probs[i].size()) + al(sizeof(int) * tbeg[i].size()) +
              al(2 * sizeof(unsigned));
   }
@@ -508,7 +510,9 @@ int build(Hss& h, double tol, unsigned long long seed, hipStream_t strm) {
       stg[1].push_back(gen_problem(h, h.kind, s, nd.n, nd.n, -1.0, Ps + (size_t)nd.lo * s, s,
                                    nd.lo, nd.lo, 1.0, Z + (size_t)nd.lo * s, s));
     }
-    if ((st = run_stages(stg, strm))) return done(st);
+    // s rows only: the 128 x 128 class from 64 rows on (one tile row per problem, long
+    // contractions) beats the 64 x 64 one
+    if ((st = run_stages(stg, strm, 64))) return done(st);
   }
   // level inputs: the ID matrices (s x width) of every node of the level
   std::vector<const double*> Min(nid, nullptr), Zin(nid, nullptr);
@@ -806,10 +810,11 @@ extern "C" int psmma_hss_build_dev(int kind, int n, const double* g0, const doub
   else
     h->g = GenPtrs{g0, g1, g2, g3, nullptr, 0, n, n};
   // samples = 0: 64, then 96, then 128 while some node's rank reaches within the
-  // oversampling margin of the sample count
+  // oversampling margin of the sample count; from n = 16384 on the count starts at
+  // 96 (the eigenvector merge's top-level ranks reach the 70s there at tol 1e-12)
   static const int kTries[3] = {64, 96, 128};
   int st = 0;
-  for (int a = 0; a < 3; ++a) {
+  for (int a = (!samples && n >= 16384) ? 1 : 0; a < 3; ++a) {
     h->s = samples ? samples : kTries[a];
     st = build(*h, tol, seed, (hipStream_t)stream);
     if (st != 0 || samples || h->stats[3] == 0 || a == 2) break;

@@ -64,7 +64,11 @@ __device__ __forceinline__ void right_tile(int M, int N, int K, double alpha,
   constexpr bool WIN = (KIND == KIND_TOEPLITZ || KIND == KIND_HANKEL) && !IDX;
   // every entry gathered through gen_entry: index-list Toeplitz / Hankel blocks and the
   // internal transposed and explicit kinds
-  constexpr bool GATHER = ((KIND == KIND_TOEPLITZ || KIND == KIND_HANKEL) && IDX) || KIND >= 4;
+  constexpr bool GATHER = ((KIND == KIND_TOEPLITZ || KIND == KIND_HANKEL) && IDX) ||
+                         KIND == KIND_DENSE || KIND == KIND_DENSE_T;
+  // transposed Cauchy kinds: H(k, c) = C(c, k), the row generators are C's column ones
+  constexpr bool TRANS = KIND == KIND_CAUCHYEIG_T || KIND == KIND_CAUCHY_T;
+  static_assert(!(TRANS && IDX), "no index lists for the transposed kinds");
   constexpr int WLEN = BK + BN - 1;
   constexpr int WREG = (WLEN + NT - 1) / NT;
   constexpr int HREG = WIN ? 1 : BK * BN / NT;  // generated entries per thread per slab
@@ -91,6 +95,9 @@ __device__ __forceinline__ void right_tile(int M, int N, int K, double alpha,
   double rowa[HREG], rowb[HREG];  // row generators (Cauchy kinds)
   double hreg[HREG];
   int growv[HREG];  // global rows of the slab (eigen-Cauchy index lists)
+  // transposed eigen-Cauchy: F_{k+1}, DIFL_k, DIFR_k of the slab's rows
+  double rowc[KIND == KIND_CAUCHYEIG_T ? HREG : 1], rowd[KIND == KIND_CAUCHYEIG_T ? HREG : 1],
+      rowe[KIND == KIND_CAUCHYEIG_T ? HREG : 1];
 
   // Generator index range of the whole block: window slots outside it are only
   // used by padded (row, col) pairs and are set to 0 so 0 * value stays 0.
@@ -114,6 +121,16 @@ __device__ __forceinline__ void right_tile(int M, int N, int K, double alpha,
     if (colok) {
       cv0 = g.p1[gcol];  // F_c
       cv1 = g.p3[gcol];  // V_c
+    }
+  } else if constexpr (KIND == KIND_CAUCHY_T) {
+    if (colok) {
+      cv0 = g.p0[gcol];  // D of C's row c
+      cv1 = g.p2[gcol];  // U of C's row c
+    }
+  } else if constexpr (KIND == KIND_CAUCHYEIG_T) {
+    if (colok) {
+      cv0 = g.p0[gcol];  // F of C's row c
+      cv1 = g.p1[gcol];  // U of C's row c
     }
   } else if constexpr (KIND == KIND_CAUCHYEIG) {
     if (colok) {
@@ -185,6 +202,15 @@ __device__ __forceinline__ void right_tile(int M, int N, int K, double alpha,
           } else if constexpr (KIND == KIND_CAUCHY) {
             rowa[q] = g.p0[gr];  // D_r
             rowb[q] = g.p2[gr];  // U_r
+          } else if constexpr (KIND == KIND_CAUCHY_T) {
+            rowa[q] = g.p1[gr];  // F of C's column k
+            rowb[q] = g.p3[gr];  // V of C's column k
+          } else if constexpr (KIND == KIND_CAUCHYEIG_T) {
+            rowa[q] = g.p0[gr];                                  // F_k
+            rowb[q] = g.p2[gr];                                  // V_k
+            rowc[q] = (gr + 1 < g.korder) ? g.p0[gr + 1] : 0.0;  // F_{k+1}
+            rowd[q] = g.p3[gr];                                  // DIFL_k
+            rowe[q] = g.p4[gr];                                  // DIFR_k
           } else {
             rowa[q] = g.p0[gr];  // F_r
             rowb[q] = g.p1[gr];  // U_r
@@ -212,6 +238,14 @@ __device__ __forceinline__ void right_tile(int M, int N, int K, double alpha,
         if (gk < K && colok) {
           if constexpr (KIND == KIND_CAUCHY) {
             v = (rowb[q] * cv1) / (rowa[q] - cv0);
+          } else if constexpr (KIND == KIND_CAUCHY_T) {
+            v = (cv1 * rowb[q]) / (cv0 - rowa[q]);  // C(c, k) = U_c V_k / (D_c - F_k)
+          } else if constexpr (KIND == KIND_CAUCHYEIG_T) {
+            // C(c, k), c = gcol, k = gr: the eigen-Cauchy form with its roles swapped
+            const int gr = r0 + gk;
+            const double den = (gcol <= gr) ? ((cv0 - rowa[q]) - rowd[q])
+                                            : ((cv0 - rowc[q]) - rowe[q]);
+            v = (cv1 * rowb[q]) / den;
           } else {
             // SRC/CauchyHssEig.f90:1764-1768: d_r - lambda_c from stored differences
             int gr = IDX ? growv[q] : r0 + gk;
