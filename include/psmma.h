@@ -470,8 +470,9 @@ int psmma_cauchy_blr_dev(int m, int n, double alpha, const double* a, int lda, c
  * (kind PSMMA_KIND_CAUCHYEIG, g0..g4 = fpole, u, v, difl, difr as in
  * psmma_cauchyeig_dev) or plain Cauchy matrix (PSMMA_KIND_CAUCHY, g0..g3 = d, f,
  * u, v), the counterpart of Cauchy2hssEig_omp (SRC/CauchyHssEig.f90:1955-2259):
- * a complete binary tree over n / leaf leaves (leaf <= 256, n / leaf a power of
- * two), nested row and column bases U, V and skeleton couplings B per node (the
+ * a complete binary tree whose node ranges halve (left half = floor) until they
+ * hold at most `leaf` indices (2 <= leaf <= 256, any n; all leaves at one depth),
+ * nested row and column bases U, V and skeleton couplings B per node (the
  * HSSMM generators, SRC/aux_hss.f90:27-38), each node's bases from an
  * interpolative decomposition of `samples` (1..128) random samples of its
  * block row / column, truncated at |R_kk| <= tol |R_00|; samples = 0 chooses

@@ -3,9 +3,10 @@
 // Cauchy2hssEig_omp / ComprcauchyEig (SRC/CauchyHssEig.f90:1955-2449) with the
 // HSSMM layout of its generators (SRC/aux_hss.f90:27-48, hssexpmm2 :418-492).
 //
-// Tree: a complete binary tree over n / leaf leaves (a power of two); node tau
-// covers rows and columns I_tau.  Generators, as HSSMM keeps them per node:
-//   U_tau  (w x r):  nested row basis; leaves w = leaf, parents w = r_a + r_b;
+// Tree: a complete binary tree whose node ranges halve until they hold at most
+// `leaf` indices (all leaves at one depth); node tau covers rows and columns I_tau.
+// Generators, as HSSMM keeps them per node:
+//   U_tau  (w x r):  nested row basis; leaves w = |I_tau|, parents w = r_a + r_b;
 //   V_tau  (w' x r'): nested column basis (C(rows outside, I) ~ C(., J') V^T);
 //   B_tau  (r x r'_sibling) = C(J_tau, J'_sibling), the skeleton coupling;
 //   D_tau  = C(I_tau, I_tau) for leaves, never stored: the product generates it
@@ -386,21 +387,29 @@ BatchProblem gen_problem(const Hss& h, int kind, int m, int n, int k, double alp
   return b;
 }
 
+// Complete binary tree: node ranges halve (the left half takes the floor) until
+// every node holds at most `leaf` indices, so all leaves sit at depth L =
+// min { l : ceil(n / 2^l) <= leaf } and sizes within a level differ by at most one.
 void build_tree(Hss& h) {
-  const int n = h.n, nl = n / h.leaf, L = ilog2(nl);
+  const int n = h.n;
+  int L = 0;
+  while ((n + (1 << L) - 1) >> L > h.leaf) ++L;
   h.L = L;
-  h.nodes.assign(2 * nl - 1, Node{});
+  h.nodes.assign((2 << L) - 1, Node{});
   h.lev.assign(L + 1, {});
+  h.nodes[0].n = n;
   for (int l = 0; l <= L; ++l)
     for (int i = 0; i < (1 << l); ++i) {
       const int id = (1 << l) - 1 + i;
       Node& nd = h.nodes[id];
       nd.level = l;
-      nd.n = n >> l;
-      nd.lo = i * nd.n;
       if (l > 0) {
         nd.parent = (1 << (l - 1)) - 1 + i / 2;
         nd.sib = (1 << l) - 1 + (i ^ 1);
+        const Node& p = h.nodes[nd.parent];
+        const int half = p.n / 2;
+        nd.lo = (i & 1) ? p.lo + half : p.lo;
+        nd.n = (i & 1) ? p.n - half : half;
       }
       if (l < L) {
         nd.left = (1 << (l + 1)) - 1 + 2 * i;
@@ -427,7 +436,7 @@ void build_tree(Hss& h) {
 
 int build(Hss& h, double tol, unsigned long long seed, hipStream_t strm) {
   build_tree(h);
-  const int n = h.n, s = h.s, nl = n / h.leaf, L = h.L;
+  const int n = h.n, s = h.s, nl = 1 << h.L, L = h.L;
   // generator storage: fixed per-node regions sized for the widest ID
   long long off = 0;
   for (auto& nd : h.nodes) {
@@ -757,7 +766,8 @@ int product(const Hss& h, int m, double alpha, const double* a, int lda, double 
   // contraction runs over the leaf's generated diagonal block, then over the explicit
   // tail F V^T (the leaf is a multiple of both tile classes' slab depths); otherwise
   // in two launches
-  const bool fused = h.leaf % 16 == 0;
+  bool fused = true;
+  for (int id : h.lev[L]) fused = fused && h.nodes[id].n % 16 == 0;
   stg.emplace_back();
   for (size_t i = 0; i < h.lev[L].size(); ++i) {
     const Node& nd = h.nodes[h.lev[L][i]];
@@ -795,8 +805,7 @@ extern "C" int psmma_hss_build_dev(int kind, int n, const double* g0, const doub
   if (kind != PSMMA_KIND_CAUCHY && kind != PSMMA_KIND_CAUCHYEIG) return -1;
   if (n < 1) return -2;
   if (!g0 || !g1 || !g2 || !g3 || (kind == PSMMA_KIND_CAUCHYEIG && !g4)) return -3;
-  if (leaf < 1 || leaf > kHssMaxWidth || n % leaf != 0 || ((n / leaf) & (n / leaf - 1)) != 0)
-    return -8;
+  if (leaf < 2 || leaf > kHssMaxWidth) return -8;
   if (!(tol >= 0.0)) return -9;
   if (samples < 0 || samples > kHssMaxSamples) return -10;
   if (!hss) return -12;

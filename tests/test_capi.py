@@ -170,7 +170,7 @@ def test_hss_build_argument_codes():
     assert call(n=0) == -2
     assert call(gens=(p, p, p, p, None)) == -3  # eigen-Cauchy needs DIFR
     assert call(leaf=300) == -8                  # leaf above 256
-    assert call(n=768) == -8                     # three leaves: not a power of two
+    assert call(leaf=1) == -8                    # a leaf of one index
     assert call(tol=-1.0) == -9
     assert call(samples=129) == -10
     assert L.psmma_hss_product_dev(None, 4, 1.0, p, 4, 0.0, p, 4, None) == -1

@@ -87,8 +87,12 @@ def rebuild(lay, c_exact):
     return r
 
 
-@pytest.mark.parametrize("kind,n", [("eig", 2048), ("eig", 4096), ("cauchy", 2048)])
+@pytest.mark.parametrize("kind,n", [("eig", 2048), ("eig", 4096), ("cauchy", 2048),
+                                    ("eig", 3000), ("cauchy", 1000)])
 def test_hss_product_and_representation(gpu, kind, n):
+    """n = 3000 and 1000: leaves of 187 / 188 and 250 indices (ranges halved until they fit
+    the leaf bound), so the leaf stage runs unfused and the tree is not a power-of-two split
+    of a multiple of the leaf."""
     import torch
 
     print(f"case hss {kind} N={n} leaf={LEAF} tol={TOL}", flush=True)
@@ -96,7 +100,10 @@ def test_hss_product_and_representation(gpu, kind, n):
     h = api.Hss(okind, gens, leaf=LEAF, tol=TOL, seed=11)
     st = h.stats
     print(f"  stats {st}", flush=True)
-    assert st["levels"] == int(np.log2(n // LEAF)) and st["samples"] == 64
+    assert st["levels"] == int(np.ceil(np.log2(n / LEAF))) and st["samples"] == 64
+    lay = h.layout()
+    leaves = lay["size"][[i for i in range(len(lay["parent"])) if i not in set(lay["parent"])]]
+    assert leaves.max() <= LEAF and leaves.max() - leaves.min() <= 1 and leaves.sum() == n
     assert st["saturated"] == 0 and 0 < st["rank_max"] < st["samples"]
     rng = np.random.default_rng(n)
     m = 1024
@@ -108,7 +115,6 @@ def test_hss_product_and_representation(gpu, kind, n):
     got = x.cpu().numpy()
     want = 1.3 * (a @ c) + 0.5 * x0
     err = ora.rel_fro(got, want)
-    lay = h.layout()
     r = rebuild(lay, c)
     erep = ora.rel_fro(r, c)
     epass = ora.rel_fro(got, 1.3 * (a @ r) + 0.5 * x0)
