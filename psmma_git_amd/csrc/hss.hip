@@ -66,6 +66,7 @@ constexpr int kHssOversample = 10;
 constexpr int kHssMaxWidth = 256;
 constexpr int kHssThreads = 256;
 constexpr int kKsetHss = 1;
+constexpr int kKsetLeaf = 2;  // the leaves' generated diagonal blocks with their F V^T tail
 
 // ---------------------------------------------------------------------------
 // Device kernels.
@@ -302,16 +303,18 @@ struct Hss {
 namespace {
 
 // Uploads the stage tables and runs each stage as one persistent launch (in order on s).
-// big_min: the smaller side from which a problem takes the 128 x 128 tile class.
+// big_min: the smaller side from which a problem takes the 128 x 128 tile class;
+// ksets: the batch kernel's kind set per stage (default: the HSS set for all).
 int run_stages(const std::vector<std::vector<BatchProblem>>& stages, hipStream_t s,
-               int big_min = 256) {
+               int big_min = 256, const std::vector<int>& ksets = {}) {
+  auto kset_of = [&](size_t i) { return i < ksets.size() ? ksets[i] : kKsetHss; };
   std::vector<std::vector<BatchProblem>> probs(stages.size());
   std::vector<std::vector<int>> tbeg(stages.size());
   std::vector<int> ntiles(stages.size());
   auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
   size_t bytes = 0;
   for (size_t i = 0; i < stages.size(); ++i) {
-    ntiles[i] = batch_tiles(stages[i], big_min, probs[i], tbeg[i], kKsetHss);
+    ntiles[i] = batch_tiles(stages[i], big_min, probs[i], tbeg[i], kset_of(i));
     bytes += al(sizeof(BatchProblem) * probs[i].size()) + al(sizeof(int) * tbeg[i].size()) +
              al(2 * sizeof(unsigned));
   }
@@ -340,8 +343,8 @@ int run_stages(const std::vector<std::vector<BatchProblem>>& stages, hipStream_t
   for (size_t i = 0; i < stages.size(); ++i) {
     if (ntiles[i] == 0) continue;
     e = launch_batch_f64((const BatchProblem*)(d + offp[i]), (const int*)(d + offt[i]),
-                         (int)probs[i].size(), ntiles[i], (unsigned*)(d + offc[i]), 2 * cus, s,
-                         kKsetHss);
+                         (int)probs[i].size(), ntiles[i], (unsigned*)(d + offc[i]),
+                         kset_of(i) == kKsetLeaf ? cus : 2 * cus, s, kset_of(i));
     if (e != hipSuccess) return 1000 + (int)e;
   }
   // the tables must outlive the launches: they stay leased until the stream has them
@@ -782,6 +785,8 @@ int product(const Hss& h, int m, double alpha, const double* a, int lda, double 
     }
     stg.back().push_back(b);
   }
+  std::vector<int> ksets(stg.size(), kKsetHss);
+  if (fused) ksets.back() = kKsetLeaf;
   if (!fused) {
     stg.emplace_back();
     for (size_t i = 0; i < h.lev[L].size(); ++i) {
@@ -790,7 +795,7 @@ int product(const Hss& h, int m, double alpha, const double* a, int lda, double 
                                          h.H + nd.offV, nd.wc, 1.0, c + (size_t)nd.lo * ldc, ldc));
     }
   }
-  return run_stages(stg, strm);
+  return run_stages(stg, strm, 256, ksets);
 }
 
 }  // namespace

@@ -605,8 +605,19 @@ struct HssTiles {
   static constexpr int MIN_BLOCKS = 2;  // workgroups per CU the register budget allows
 };
 
+// Tile shape of the leaf kind set (the HSS product's last stage: generated diagonal
+// blocks with the explicit F V^T tail): the single-problem kernel's eigen-Cauchy
+// shape, 256 x 128 with 8 waves of 64 x 64 (one workgroup per CU), for every problem
+// of the set.
+struct LeafTiles {
+  static constexpr int BM = 256, BN = 128, BK = 16, WM = 4, WN = 2;
+};
+
 void batch_tile_shape(int kset, int kind, int cls, int* bm, int* bn) {
-  if (kset == 1) {
+  if (kset == 2) {
+    *bm = LeafTiles::BM;
+    *bn = LeafTiles::BN;
+  } else if (kset == 1) {
     *bm = cls == 1 ? HssTiles::BM1 : HssTiles::BM0;
     *bn = cls == 1 ? HssTiles::BN1 : HssTiles::BN0;
   } else if (cls == 1) {
@@ -659,6 +670,13 @@ __device__ __forceinline__ void batch_tile_kind(const BatchProblem& p, int local
   }
 }
 
+// (The tail form serves k2 = 0 too: it adds no slabs.)
+template <int KIND>
+__device__ __forceinline__ void leaf_tile_kind(const BatchProblem& p, int local, double* lds) {
+  using T = LeafTiles;
+  batch_tile<KIND, T::BM, T::BN, T::BK, T::WM, T::WN, true>(p, local, lds);
+}
+
 template <int KIND>
 __device__ __forceinline__ void hss_tile_kind(const BatchProblem& p, int local, double* lds) {
   using T = HssTiles;
@@ -680,14 +698,17 @@ __device__ __forceinline__ void hss_tile_kind(const BatchProblem& p, int local, 
 // Kind sets of the persistent batch kernel: the public kinds (batched API), and
 // the kinds of the HSS construction and product (Cauchy kinds, their transposes
 // and explicit matrices).  One kernel per set keeps each set's register budget.
-enum BatchKindSet : int { KSET_PUBLIC = 0, KSET_HSS = 1 };
+enum BatchKindSet : int { KSET_PUBLIC = 0, KSET_HSS = 1, KSET_LEAF = 2 };
+
+constexpr int kLeafLds = tile_lds<KIND_CAUCHYEIG, LeafTiles::BM, LeafTiles::BN, LeafTiles::BK>();
 
 template <int KSET>
-__global__ __launch_bounds__(256, KSET == KSET_PUBLIC ? 2 : HssTiles::MIN_BLOCKS) void
+__global__ __launch_bounds__(KSET == KSET_LEAF ? 512 : 256,
+                             KSET == KSET_PUBLIC ? 2 : (KSET == KSET_HSS ? HssTiles::MIN_BLOCKS : 1)) void
 sgemm_f64_batch_kernel(
     const BatchProblem* __restrict__ probs, const int* __restrict__ tbeg, int nprob, int ntiles,
     unsigned* __restrict__ ctr) {
-  __shared__ double lds[KSET == KSET_PUBLIC ? kBatchLds : kHssLds];
+  __shared__ double lds[KSET == KSET_PUBLIC ? kBatchLds : (KSET == KSET_HSS ? kHssLds : kLeafLds)];
   __shared__ int tile_s;
   for (;;) {
     if (threadIdx.x == 0) tile_s = (int)atomicAdd(&ctr[0], 1u);
@@ -708,6 +729,11 @@ sgemm_f64_batch_kernel(
         case KIND_HANKEL: batch_tile_kind<KIND_HANKEL>(p, local, lds); break;
         default: batch_tile_kind<KIND_CAUCHYEIG>(p, local, lds); break;
       }
+    } else if constexpr (KSET == KSET_LEAF) {
+      if (p.kind == KIND_CAUCHY)
+        leaf_tile_kind<KIND_CAUCHY>(p, local, lds);
+      else
+        leaf_tile_kind<KIND_CAUCHYEIG>(p, local, lds);
     } else {
       switch (p.kind) {
         case KIND_CAUCHY: hss_tile_kind<KIND_CAUCHY>(p, local, lds); break;
@@ -730,12 +756,15 @@ sgemm_f64_batch_kernel(
 }
 
 // One launch over the whole tile list; ctr: two zeroed unsigned counters; kset:
-// 0 for the public kinds, 1 for the HSS kinds.
+// 0 for the public kinds, 1 for the HSS kinds, 2 for the HSS product's leaves.
 hipError_t launch_batch_f64(const BatchProblem* dprobs, const int* dtbeg, int nprob, int ntiles,
                             unsigned* ctr, int nworkgroups, hipStream_t s, int kset) {
   if (ntiles <= 0) return hipSuccess;
   const int g = ntiles < nworkgroups ? ntiles : nworkgroups;
-  if (kset == KSET_HSS)
+  if (kset == KSET_LEAF)
+    hipLaunchKernelGGL(sgemm_f64_batch_kernel<KSET_LEAF>, dim3(g), dim3(512), 0, s, dprobs, dtbeg,
+                       nprob, ntiles, ctr);
+  else if (kset == KSET_HSS)
     hipLaunchKernelGGL(sgemm_f64_batch_kernel<KSET_HSS>, dim3(g), dim3(256), 0, s, dprobs, dtbeg,
                        nprob, ntiles, ctr);
   else
