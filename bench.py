@@ -477,6 +477,9 @@ def profile_leg(leg, seed, torch, api, inputs):
         else:
             ms = ev_time(lambda: api.toeplitz_dev(1.0, a, g, n, n, c, prec=api.PREC_SPLIT_BF16,
                                                   slices=3), 2, torch)
+            ref = api.colmajor_empty(n, n)
+            api.toeplitz_dev(1.0, a, g, n, n, ref)
+            return {"leg": leg, "n": n, "ms": round(ms, 3), "rel_err_vs_fp64": rel(c, ref)}
         return {"leg": leg, "n": n, "ms": round(ms, 3)}
     if leg == "cauchy2048":  # plain Cauchy x dense, N = 2048, FP64
         n = 2048

@@ -50,6 +50,10 @@ constexpr int kSplitI8Kmax = 131071;
 // form), or this many bf16 slices (the count that stays <= 1e-13 in the tests).
 constexpr int kAutoSplitBits = 45;
 constexpr int kAutoBf16Slices = 6;
+// bf16 forms with at most this many slices run every kept slice pair into one
+// FP32 accumulator (their ~2^(-8s) slicing error is above FP32 accumulation
+// rounding); more slices sum each class exactly per K-chunk and flush to FP64.
+constexpr int kBf16Fp32AccSlices = 3;
 // Exponent of an all-zero row/column (below any FP64 exponent; sums stay in int).
 constexpr int kNoExp = -10000;
 // Pairwise coprime (distinct prime factors), all <= 256 so a symmetric
@@ -1535,6 +1539,163 @@ __global__ __launch_bounds__(256) void gemm_bf16_slices_kernel(
     }
 }
 
+// bf16 slice GEMM with one FP32 accumulator (slices <= 3): the kept slice pairs
+// (p, q), p + q <= s - 1, are concatenated along k into one contraction of
+// npair * kp bf16 products, lowest class first, and accumulated in FP32 on
+// v_mfma_f32_16x16x32_bf16.  Staging is the int8 direct-to-LDS design: 256 x 256
+// tiles, eight waves of 128 x 64, k steps of 64 bytes (32 bf16) through a ring
+// of kDmaStages LDS stages filled by global_load_lds, the same chunk swizzle.
+// The bf16 operand bytes of a lane (A: row l % 16, bf16 k 8 (l / 16) .. +7)
+// are the int8 form's (row l % 16, bytes 16 (l / 16) .. +15), so the stage
+// layout and the fragment reads carry over unchanged.  Rows past M (columns
+// past N) load the last valid row; their outputs are not stored.
+// pairs: pair s of the contraction is (A plane (pairs >> 4s) & 3, H plane
+// (pairs >> (4s + 2)) & 3).
+__global__ __launch_bounds__(512, 1) void gemm_bf16_dma_kernel(
+    int M, int N, int kp, const unsigned short* __restrict__ As,
+    const unsigned short* __restrict__ Hs, size_t aplane, size_t hplane, int npair, int pairs,
+    const int* __restrict__ ea, const int* __restrict__ eh, double alpha, double beta,
+    double* __restrict__ C, int ldc) {
+  constexpr int BM = 256, BN = 256, BK = 64, NS = kDmaStages;
+  constexpr int WGN = BN / 64, NW = 2 * WGN, WM = 128, WN = 64;
+  constexpr int TM = WM / 16, TN = WN / 16;
+  constexpr int GPWA = 16 / NW, GPWB = (BN / 16) / NW;
+  constexpr int SA = BM * BK, SB = BN * BK;
+  constexpr int DPS = GPWA + GPWB;
+  static_assert(NS * (SA + SB) <= 160 * 1024, "LDS of one CU");
+  __shared__ __align__(1024) signed char lds[NS * (SA + SB)];
+  signed char(*Ast)[SA] = (signed char(*)[SA])lds;
+  signed char(*Bst)[SB] = (signed char(*)[SB])(lds + NS * SA);
+
+  const int nbn = (N + BN - 1) / BN, nbm = (M + BM - 1) / BM;
+  int bmi, bni;
+  patch_remap<4, 8>(blockIdx.x, nbm, nbn, bmi, bni);
+  const int m0 = bmi * BM, n0 = bni * BN;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm0 = (wave / WGN) * WM, wn0 = (wave % WGN) * WN;
+  const int lr = lane & 15, lg = lane >> 4;
+  const unsigned rowb = 2u * (unsigned)kp;  // bytes per row of a plane
+
+  const int rg = lane >> 2;
+  const int gch = (lane & 3) ^ dma_swz(rg);
+  unsigned aoff[GPWA], boff[GPWB];
+#pragma unroll
+  for (int q = 0; q < GPWA; ++q) {
+    const int r = 16 * (GPWA * wave + q) + rg;
+    aoff[q] = (unsigned)min(m0 + r, M - 1) * rowb + 16u * gch;
+  }
+#pragma unroll
+  for (int q = 0; q < GPWB; ++q) {
+    const int r = 16 * (GPWB * wave + q) + rg;
+    boff[q] = (unsigned)min(n0 + r, N - 1) * rowb + 16u * gch;
+  }
+  const char* Ab = (const char*)As;
+  const char* Hb = (const char*)Hs;
+  auto dma16 = [&](const char* gbase, unsigned voff, const signed char* ldst) {
+    const unsigned la = __builtin_amdgcn_readfirstlane((unsigned)(size_t)ldst);
+    asm volatile("s_mov_b32 m0, %0\n\ts_nop 0\n\tglobal_load_lds_dwordx4 %1, %2"
+                 :
+                 : "s"(la), "v"(voff), "s"(gbase)
+                 : "memory", "m0");
+  };
+  // the issue stream walks the k steps of segment after segment; past the last
+  // step it re-reads that step into the free stage (results unused)
+  const int nks = kp / 32, nkt = npair * nks;
+  int iseg = 0, ik = 0;
+  auto issue_next = [&](int st) {
+    const int pr = (pairs >> (4 * iseg)) & 15;
+    const char* ak = Ab + 2 * (size_t)(pr & 3) * aplane + (size_t)ik * BK;
+    const char* bk = Hb + 2 * (size_t)(pr >> 2) * hplane + (size_t)ik * BK;
+#pragma unroll
+    for (int q = 0; q < GPWA; ++q) dma16(ak, aoff[q], &Ast[st][(GPWA * wave + q) * 1024]);
+#pragma unroll
+    for (int q = 0; q < GPWB; ++q) dma16(bk, boff[q], &Bst[st][(GPWB * wave + q) * 1024]);
+    if (++ik == nks) {
+      if (iseg + 1 < npair) {
+        ++iseg;
+        ik = 0;
+      } else {
+        ik = nks - 1;
+      }
+    }
+  };
+
+  v4f acc[TM][TN];
+#pragma unroll
+  for (int i = 0; i < TM; ++i)
+#pragma unroll
+    for (int j = 0; j < TN; ++j) acc[i][j] = (v4f){0.f, 0.f, 0.f, 0.f};
+
+  const int lofs = lr * 64 + ((lg ^ dma_swz(lr)) * 16);
+  v2l af[2][TM], bfd[2][TN];
+  auto frags = [&](int kt, int slot) {
+    const signed char* sa = Ast[kt % NS];
+    const signed char* sb = Bst[kt % NS];
+#pragma unroll
+    for (int i = 0; i < TM; ++i) af[slot][i] = *(const v2l*)(sa + (wm0 + 16 * i) * 64 + lofs);
+#pragma unroll
+    for (int j = 0; j < TN; ++j) bfd[slot][j] = *(const v2l*)(sb + (wn0 + 16 * j) * 64 + lofs);
+  };
+
+#pragma unroll
+  for (int q = 0; q < NS - 1; ++q) issue_next(q);
+  wait_vmcnt<(NS - 2) * DPS>();
+  asm volatile("" ::: "memory");
+  __builtin_amdgcn_s_barrier();
+  asm volatile("" ::: "memory");
+  frags(0, 0);
+
+  // the int8 kernel's five-stage schedule: one barrier per two k steps
+  auto step = [&](int kt, auto slotc) {
+    constexpr int slot = decltype(slotc)::value;
+    static_assert(NS == 5, "barrier schedule for five stages");
+    if constexpr (slot == 0) {
+      wait_vmcnt<DPS>();
+      asm volatile("" ::: "memory");
+      __builtin_amdgcn_s_barrier();
+      asm volatile("" ::: "memory");
+    }
+    issue_next((kt + NS - 1) % NS);
+    frags(kt + 1, slot ^ 1);
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+      for (int j = 0; j < TN; ++j)
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+            __builtin_bit_cast(v8bf, af[slot][i]), __builtin_bit_cast(v8bf, bfd[slot][j]),
+            acc[i][j], 0, 0, 0);
+    __builtin_amdgcn_sched_barrier(0);
+  };
+  for (int kt = 0; kt < nkt; kt += 2) {  // nks is a multiple of 8: nkt is even, >= 8
+    step(kt, std::integral_constant<int, 0>{});
+    step(kt + 1, std::integral_constant<int, 1>{});
+  }
+  wait_vmcnt<0>();
+
+  // C = alpha * acc * 2^(ea_i + eh_j) + beta * C: four consecutive rows of one
+  // column per lane and block (32 contiguous bytes)
+#pragma unroll
+  for (int j = 0; j < TN; ++j) {
+    const int gc = n0 + wn0 + 16 * j + lr;
+    if (gc >= N) continue;
+    const int ec = eh[gc];
+    double* cc = C + (size_t)gc * ldc;
+#pragma unroll
+    for (int i = 0; i < TM; ++i) {
+      const int gr0 = m0 + wm0 + 16 * i + 4 * lg;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int gr = gr0 + r;
+        if (gr >= M) continue;
+        double v = alpha * ldexp((double)acc[i][j][r], ea[gr] + ec);
+        if (beta != 0.0) v += beta * cc[gr];
+        cc[gr] = v;
+      }
+    }
+  }
+}
+
 template <int KIND>
 static int run_split_bf16(int s, int M, int N, int K, double alpha, const double* A, int lda,
                           const GenPtrs& g, int r0, int c0, double beta, double* C, int ldc,
@@ -1561,8 +1722,20 @@ static int run_split_bf16(int s, int M, int N, int K, double alpha, const double
                      lda, ea, s, as, kp, aplane);
   hipLaunchKernelGGL(split_h_bf16_kernel<KIND>, dim3((N + 15) / 16, kp / 256), dim3(256), 0, st, K,
                      N, g, r0, c0, eh, s, hs, kp, hplane);
+  if (s <= kBf16Fp32AccSlices && 2 * (size_t)M * kp < (1ull << 32) &&
+      2 * (size_t)N * kp < (1ull << 32)) {
+    // kept pairs p + q <= s - 1, lowest class first
+    int npair = 0, pairs = 0;
+    for (int d = s - 1; d >= 0; --d)
+      for (int p = 0; p <= d; ++p) pairs |= (p | ((d - p) << 2)) << (4 * npair++);
+    const int nb256 = ((M + 255) / 256) * ((N + 255) / 256);
+    hipLaunchKernelGGL(gemm_bf16_dma_kernel, dim3(nb256), dim3(512), 0, st, M, N, kp, as, hs,
+                       aplane, hplane, npair, pairs, ea, eh, alpha, beta, C, ldc);
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? 0 : 1000 + (int)e;
+  }
   int nb = ((M + 63) / 64) * ((N + 63) / 64);
-#define PSMMA_BF16_CASE(SS)                                                                   \
+#define PSMMA_BF16_CASE(SS)                                                                \
   case SS:                                                                                    \
     hipLaunchKernelGGL(gemm_bf16_slices_kernel<SS>, dim3(nb), dim3(256), 0, st, M, N, kp, as, \
                        hs, aplane, hplane, ea, eh, alpha, beta, C, ldc);                     \

@@ -135,3 +135,31 @@ def test_split_bf16_parity(gpu, ora, kind, m, n, k):
     print(f"errs={errs}", flush=True)
     assert errs[3] <= 1e-6
     assert errs[6] <= 1e-12
+
+
+@pytest.mark.parametrize("kind", ["cauchy", "toeplitz", "hankel", "cauchyeig"])
+def test_split_bf16_fp32acc_ragged(gpu, ora, kind):
+    """1-3 slices run every kept slice pair into one FP32 accumulator on the
+    direct-to-LDS bf16 kernel: 1100 x 1037 x 1300 gives 5 x 5 ragged 256 x 256
+    tiles and K % 32 != 0.  Error falls by about 2^-8 per slice down to the FP32
+    accumulation floor; the sliced operands are exact, so any kept pair that was
+    dropped or doubled would show up as an error far above these bounds."""
+    import torch
+
+    m, n, k = 1100, 1037, 1300
+    print(f"case bf16 fp32acc kind={kind} m={m} n={n} k={k}", flush=True)
+    rng = np.random.default_rng(case_seed("bf", kind, m, n, k))
+    gens, dev = make_gens(kind, max(k, n) + 5, rng)
+    a = rng.uniform(-1, 1, (m, k))
+    c = rng.standard_normal((m, n))
+    want = ora.structured_gemm_np(ORA_KIND[kind], 0.5, a, gens, 3, 2, n, beta=-1.0, c=c)
+    errs = {}
+    for s in (1, 2, 3):
+        da = dev_colmajor(a, ld=m + 3)
+        dc = dev_colmajor(c, ld=m + 1)
+        run_dev(kind, 0.5, da, dev, gens, n, dc, -1.0, 3, 2, prec=api.PREC_SPLIT_BF16, slices=s)
+        torch.cuda.synchronize()
+        errs[s] = ora.rel_fro(host(dc), want)
+    print(f"errs={errs}", flush=True)
+    assert errs[1] <= 2e-2 and errs[2] <= 1e-4 and errs[3] <= 1e-6
+    assert errs[3] < errs[2] < errs[1]
