@@ -437,6 +437,7 @@ def batched_config(seed, torch, api, inputs):
     ms_loop_graph = ev_time(graph.replay, 5, torch)
     del graph
     plans = api.BatchedPlan(probs, prec=api.PREC_SPLIT_I8, slices=0)
+    split_t, _ = plans.info()
     mss = ev_time(plans.launch, 5, torch)
     es = [rel(c, r) for (_, _, _, c), r in zip(keep, refs)]
     plan64.close()
@@ -447,7 +448,9 @@ def batched_config(seed, torch, api, inputs):
             "fp64_err_max": max(e64),
             "fp64_loop_ms": round(ms_loop, 3), "fp64_loop_graph_ms": round(ms_loop_graph, 3),
             "fp64_loop_err_max": eloop,
-            "split_i8_ms": round(mss, 3), "split_tflops": round(flops / mss / 1e9, 2),
+            "split_i8_ms": round(mss, 3), "split_moduli": split_t,
+            "split_form": "fused" if os.environ.get("PSMMA_GSPLIT_FUSED", "1") != "0" else "residue planes",
+            "split_tflops": round(flops / mss / 1e9, 2),
             "split_speedup_vs_fp64": round(ms64 / mss, 3),
             "split_speedup_vs_fp64_loop": round(min(ms_loop, ms_loop_graph) / mss, 3),
             "split_err": {"median": q(es, 0.5), "p90": q(es, 0.9), "max": max(es)}}

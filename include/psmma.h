@@ -324,8 +324,9 @@ int psmma_redistribute_dev(int m, int n, int nprow, int npcol, int src_mb, int s
  * stream's workspace).  Synchronises `stream`.  Returns 0, 1 when no depth
  * reaches the target (the largest is returned), -(argument number).  Batched
  * split plans created with slices = 0 take the largest depth this chooses at
- * 1e-13 (a tenth of the 1e-12 goal, a margin for the problems and columns not
- * sampled) over the longest-K problem of each kind. */
+ * 2e-13 (a fifth of the 1e-12 goal, a margin for the columns not sampled and
+ * the fused form's rounding) over the longest-K problem of each kind, whose
+ * contraction keeps the fewest bits per operand. */
 int psmma_autotune_f64(int kind, int m, int n, int k, int* cfg, float* ms);
 int psmma_split_slices_dev(const psmma_problem* p, int prec, double target, int* slices,
                            double* err, void* stream);

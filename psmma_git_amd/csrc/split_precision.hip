@@ -1778,7 +1778,9 @@ enum {
   PH_FILL = 0, PH_ROWEXP, PH_COLEXP, PH_EHFILL, PH_SPLITA, PH_SPLITB,
   PH_GEMM_D, PH_GEMM_T, PH_GEMM_H,     // 128 x 128 tiles
   PH_GEMM_DB, PH_GEMM_TB, PH_GEMM_HB,  // 256 x 256 tiles (large problems)
-  PH_CRT, NPH
+  PH_CRT,
+  PH_FUSED,  // the fused form: 128 x 128 tiles, all moduli and the recombination
+  NPH
 };
 
 struct SplitProblem {
@@ -1834,6 +1836,10 @@ __global__ __launch_bounds__(256) void gsplit_phase_kernel(const SplitProblem* _
         gen_range_exp_kernel_body(q.g.p0, q.glo, q.ghi, q.e1, bx, 0);
       } else if (q.kind == KIND_CAUCHY) {
         col_exp_kernel_body<KIND_CAUCHY>(q.K, q.N, q.g, q.r0, q.c0, 256, q.eh, bx, by);
+      } else if (q.kind == KIND_TOEPLITZ) {
+        col_exp_kernel_body<KIND_TOEPLITZ>(q.K, q.N, q.g, q.r0, q.c0, 256, q.eh, bx, by);
+      } else if (q.kind == KIND_HANKEL) {
+        col_exp_kernel_body<KIND_HANKEL>(q.K, q.N, q.g, q.r0, q.c0, 256, q.eh, bx, by);
       } else {
         col_exp_kernel_body<KIND_CAUCHYEIG>(q.K, q.N, q.g, q.r0, q.c0, 256, q.eh, bx, by);
       }
@@ -1850,6 +1856,12 @@ __global__ __launch_bounds__(256) void gsplit_phase_kernel(const SplitProblem* _
                                 cp, q.bres, q.bplane, q.lpad, bx, 0);
       } else if (q.kind == KIND_CAUCHY) {
         split_h_kernel_body<KIND_CAUCHY>(q.K, q.N, q.g, q.r0, q.c0, q.eh, q.abits, cp, q.bres,
+                                         q.kp, q.bplane, bx, by);
+      } else if (q.kind == KIND_TOEPLITZ) {
+        split_h_kernel_body<KIND_TOEPLITZ>(q.K, q.N, q.g, q.r0, q.c0, q.eh, q.abits, cp, q.bres,
+                                           q.kp, q.bplane, bx, by);
+      } else if (q.kind == KIND_HANKEL) {
+        split_h_kernel_body<KIND_HANKEL>(q.K, q.N, q.g, q.r0, q.c0, q.eh, q.abits, cp, q.bres,
                                          q.kp, q.bplane, bx, by);
       } else {
         split_h_kernel_body<KIND_CAUCHYEIG>(q.K, q.N, q.g, q.r0, q.c0, q.eh, q.abits, cp, q.bres,
@@ -1906,21 +1918,339 @@ __global__ __launch_bounds__(256) void gsplit_crt_kernel(const SplitProblem* __r
                 q.beta, q.C, q.ldc, e0 + (size_t)h * 256);
 }
 
+// ---------------------------------------------------------------------------
+// int8 modular GEMM with the recombination in registers (fused form).
+//
+// One workgroup owns one 128 x 128 output tile and runs the t residue GEMMs of
+// that tile back to back as one k loop of t * kp / 64 steps (the DMA ring runs
+// on across the modulus boundaries).  When modulus l is done, every int32
+// accumulator x is reduced to its symmetric residue r (|r| <= 128) and folded
+// into a 64-bit fixed-point fraction
+//     S += (r + 128) * W_l  (mod 2^64),   W_l = round(2^65 y_l / m_l),
+// y_l = (M / m_l)^-1 mod m_l (M = prod m_l).  By the Chinese remainder theorem
+// X / M = sum_l r_l y_l / m_l (mod 1), so after the last modulus
+//     S - sum_l 128 W_l = 2^65 X / M  (mod 2^64),
+// and that is X / M itself: abits keeps |X| <= K 2^(2 abits) < M / 4, so
+// |2^65 X / M| < 2^63 fits a signed 64-bit integer.  The rounding of the W_l
+// adds at most t * 128 * 1/2 units of 2^-65 to X / M: |dX| <= t 2^-59 M, about
+// 2^-45 of a typical |X| at K = 2048, below the 2^-abits operand rounding.
+// C = alpha * X * 2^-(sa_i + sh_j) + beta * C is then written directly: no
+// residue planes of the result and no recombination pass.
+//
+// Four waves of 64 x 64 (16 accumulators and 32 fraction words each) and two
+// workgroups per CU, so one workgroup's residue folds and epilogue overlap the
+// other's MFMAs.  k steps of 64 bytes go through a five-stage ring filled by
+// global_load_lds three steps ahead (one spare stage: the fragments of a step
+// are read at the step itself, and one barrier per two steps keeps the ring
+// safe).  Dense B stages 128 columns x 64 bytes like A; the generator-window
+// kinds stage the t windows of the tile once, at the start, in 16-byte chunks.
+struct FusedCrt {
+  int t;
+  int m[kMaxModuli];
+  float rmf[kMaxModuli];             // fl(1 / m_l), for mod_acc
+  unsigned long long w[kMaxModuli];  // round(2^65 y_l / m_l) mod 2^64
+  unsigned long long k0;             // sum_l 128 w_l mod 2^64 (the residue offset)
+  double scale;                      // M * 2^-65
+};
+
+static FusedCrt make_fused(int t) {
+  FusedCrt f{};
+  f.t = t;
+  unsigned __int128 M = 1;
+  for (int l = 0; l < t; ++l) M *= (unsigned)kModuli[l];
+  for (int l = 0; l < t; ++l) {
+    const unsigned m = (unsigned)kModuli[l];
+    const long long y = inv_mod((long long)((M / m) % m), m);
+    const unsigned __int128 num = ((unsigned __int128)y << 65) + m / 2;
+    f.m[l] = (int)m;
+    f.rmf[l] = 1.0f / (float)m;
+    f.w[l] = (unsigned long long)(num / m);  // the quotient's low 64 bits
+    f.k0 += 128ull * f.w[l];
+  }
+  const unsigned long long top = (unsigned long long)(M >> 64), bot = (unsigned long long)M;
+  const long double Ml = (long double)top * 18446744073709551616.0L + (long double)bot;
+  f.scale = (double)ldexpl(Ml, -65);
+  return f;
+}
+
+constexpr int kFusedWinKmax = 2048;  // longest padded contraction of the fused window form
+constexpr int kFusedWinStride = kFusedWinKmax + 128 + 32;  // LDS bytes per modulus window
+constexpr int kFusedLds = 5 * 128 * 64 * 2;  // five stages of A and B (dense form)
+static_assert(5 * 128 * 64 + kMaxModuli * kFusedWinStride <= kFusedLds, "window form fits");
+
+template <int BMODE>
+__device__ __forceinline__ void gemm_i8_fused_body(
+    signed char* __restrict__ lds, int M, int N, int kp, int lpad,
+    const signed char* __restrict__ Ares, const signed char* __restrict__ Bres, size_t aplane,
+    size_t bplane, const FusedCrt& fc, const int* __restrict__ ea, const int* __restrict__ eh,
+    int abits, double alpha, double beta, double* __restrict__ C, int ldc, int bmi, int bni) {
+  constexpr bool WIN = BMODE != BMODE_DENSE;
+  constexpr int BM = 128, BN = 128, BK = 64, NS = 5, D = 3;
+  constexpr int WM = 64, WN = 64, TM = 4, TN = 4, NT = 256;
+  constexpr int GPW = 2;  // 16-row DMA groups per wave and operand
+  constexpr int SA = BM * BK, SB = BN * BK;
+  constexpr int DPS = WIN ? GPW : 2 * GPW;  // DMA instructions per wave per k step
+  signed char(*Ast)[SA] = (signed char(*)[SA])lds;
+  signed char(*Bst)[SB] = (signed char(*)[SB])(lds + NS * SA);
+  unsigned* Win = (unsigned*)(lds + NS * SA);
+
+  const int t = fc.t;
+  const int m0 = bmi * BM, n0 = bni * BN;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm0 = (wave >> 1) * WM, wn0 = (wave & 1) * WN;
+  const int lr = lane & 15, lg = lane >> 4;
+
+  // window form: window byte x of modulus l holds R_l[ws + x], ws = the tile's
+  // first generator position rounded down to 16 (chunks of 16 bytes are either
+  // inside [0, lpad) or outside it: lpad is a multiple of 16)
+  int cj[TN];
+  if constexpr (WIN) {
+    const int ubase = (BMODE == BMODE_TOE) ? (N - n0 - BN) : n0;
+    const int ws = (ubase >> 4) << 4;
+    const int nch = (kp + BN + 32) / 16;
+    for (int x = tid; x < t * nch; x += NT) {
+      const int l = x / nch, c = x - l * nch;
+      const int u = ws + 16 * c;
+      uint4 v = make_uint4(0, 0, 0, 0);
+      if (u >= 0 && u < lpad) v = *(const uint4*)(Bres + l * bplane + u);
+      *(uint4*)((char*)Win + l * kFusedWinStride + 16 * c) = v;
+    }
+#pragma unroll
+    for (int j = 0; j < TN; ++j)
+      cj[j] = ubase - ws +
+              ((BMODE == BMODE_TOE) ? (BN - 1 - wn0 - 16 * j - lr) : (wn0 + 16 * j + lr));
+  }
+
+  const int rg = lane >> 2;
+  const int gch = (lane & 3) ^ dma_swz(rg);
+  unsigned aoff[GPW], boff[GPW];
+#pragma unroll
+  for (int q = 0; q < GPW; ++q) {
+    const int r = 16 * (GPW * wave + q) + rg;
+    aoff[q] = (unsigned)min(m0 + r, M - 1) * (unsigned)kp + 16u * gch;
+    boff[q] = WIN ? 0u : (unsigned)min(n0 + r, N - 1) * (unsigned)kp + 16u * gch;
+  }
+  auto dma16 = [&](const signed char* gbase, unsigned voff, const signed char* ldst) {
+    const unsigned la = __builtin_amdgcn_readfirstlane((unsigned)(size_t)ldst);
+    asm volatile("s_mov_b32 m0, %0\n\ts_nop 0\n\tglobal_load_lds_dwordx4 %1, %2"
+                 :
+                 : "s"(la), "v"(voff), "s"(gbase)
+                 : "memory", "m0");
+  };
+  // the issue stream walks modulus after modulus; past the last step it
+  // re-reads that step into the spare stage (never read)
+  // (running source pointers and uniform LDS offsets keep the per-step
+  // address work in a few scalar instructions)
+  const int nks = kp / BK;
+  const int wv = __builtin_amdgcn_readfirstlane(wave);
+  int il = 0, ik = 0;
+  const signed char* ap = Ares;
+  const signed char* bp = Bres;
+  auto issue_next = [&](int st) {
+#pragma unroll
+    for (int q = 0; q < GPW; ++q) dma16(ap, aoff[q], &Ast[st][(GPW * wv + q) * 1024]);
+    if constexpr (!WIN) {
+#pragma unroll
+      for (int q = 0; q < GPW; ++q) dma16(bp, boff[q], &Bst[st][(GPW * wv + q) * 1024]);
+    }
+    if (++ik < nks) {
+      ap += BK;
+      bp += BK;
+    } else if (il + 1 < t) {  // first step of the next modulus's planes
+      ++il;
+      ik = 0;
+      ap += (ptrdiff_t)aplane - (kp - BK);
+      bp += (ptrdiff_t)bplane - (kp - BK);
+    } else {
+      ik = nks - 1;  // past the end: the last step again
+    }
+  };
+
+  v4i acc[TM][TN];
+  unsigned long long S[TM][TN][4];
+#pragma unroll
+  for (int i = 0; i < TM; ++i)
+#pragma unroll
+    for (int j = 0; j < TN; ++j) {
+      acc[i][j] = (v4i){0, 0, 0, 0};
+#pragma unroll
+      for (int r = 0; r < 4; ++r) S[i][j][r] = 0ull;
+    }
+
+  const int lofs = lr * 64 + ((lg ^ dma_swz(lr)) * 16);
+  // one k step: fragments of stage st (k step kk of modulus l), then 16 MFMAs
+  // (A fragments are read one row block at a time, just before their MFMAs:
+  // the 64 fraction words leave no room for all of them at once)
+  auto mfma_step = [&](int st, int kk, int l) {
+    v2l bf[TN];
+    if constexpr (WIN) {
+      const unsigned* wl = Win + l * (kFusedWinStride / 4);
+#pragma unroll
+      for (int j = 0; j < TN; ++j) {
+        const int x0 = kk * BK + lg * 16 + cj[j];
+        const unsigned* wp = wl + (x0 >> 2);
+        const int sh = x0 & 3;
+        unsigned w[5];
+#pragma unroll
+        for (int d = 0; d < 5; ++d) w[d] = wp[d];
+        const unsigned o0 = __builtin_amdgcn_alignbyte(w[1], w[0], sh);
+        const unsigned o1 = __builtin_amdgcn_alignbyte(w[2], w[1], sh);
+        const unsigned o2 = __builtin_amdgcn_alignbyte(w[3], w[2], sh);
+        const unsigned o3 = __builtin_amdgcn_alignbyte(w[4], w[3], sh);
+        bf[j][0] = (long)(((unsigned long)o1 << 32) | o0);
+        bf[j][1] = (long)(((unsigned long)o3 << 32) | o2);
+      }
+    } else {
+#pragma unroll
+      for (int j = 0; j < TN; ++j) bf[j] = *(const v2l*)(Bst[st] + (wn0 + 16 * j) * 64 + lofs);
+    }
+    // the next row block's fragment is read while this one's MFMAs run
+    v2l af = *(const v2l*)(Ast[st] + wm0 * 64 + lofs);
+#pragma unroll
+    for (int i = 0; i < TM; ++i) {
+      v2l an = af;
+      if (i + 1 < TM) an = *(const v2l*)(Ast[st] + (wm0 + 16 * (i + 1)) * 64 + lofs);
+#pragma unroll
+      for (int j = 0; j < TN; ++j)
+        acc[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(af, bf[j], acc[i][j], 0, 0, 0);
+      __builtin_amdgcn_sched_barrier(0);
+      af = an;
+    }
+  };
+  // end of modulus l: accumulators -> symmetric residues -> fraction words
+  // (mod_acc: kp <= kFastModK keeps |x| <= 2^28; the host routes longer
+  // contractions elsewhere)
+  auto fold = [&](int l) {
+    const int m = fc.m[l];
+    const float rmf = fc.rmf[l];
+    const unsigned long long w = fc.w[l];
+    const unsigned hm = (unsigned)(m + 1) >> 1;
+    const unsigned neg = 128u - (unsigned)m;  // r - m + 128 (mod 2^32)
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+      for (int j = 0; j < TN; ++j) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const unsigned rr = mod_acc(acc[i][j][r], m, rmf);
+          const unsigned u = rr + (rr >= hm ? neg : 128u);  // in [0, 256)
+          S[i][j][r] += (unsigned long long)u * w;
+        }
+        acc[i][j] = (v4i){0, 0, 0, 0};
+      }
+  };
+
+  if constexpr (WIN) __syncthreads();  // the windows were written with LDS stores
+#pragma unroll
+  for (int q = 0; q < D; ++q) issue_next(q);
+  int sc = 0;  // stage of the current step: kt % NS
+  auto next_stage = [](int s) { return s == NS - 1 ? 0 : s + 1; };
+  for (int l = 0; l < t; ++l) {
+    for (int kk = 0; kk < nks; kk += 2) {  // nks is a multiple of 4
+      // steps kt and kt+1 have landed once only step kt+2 may be in flight;
+      // after the barrier every wave has read the stages of steps <= kt-1,
+      // which the DMA of steps kt+3 and kt+4 overwrite
+      wait_vmcnt<DPS>();
+      asm volatile("" ::: "memory");
+      __builtin_amdgcn_s_barrier();
+      asm volatile("" ::: "memory");
+      const int s1 = next_stage(sc);
+      const int s3 = next_stage(next_stage(s1));
+      issue_next(s3);
+      mfma_step(sc, kk, l);
+      issue_next(next_stage(s3));
+      mfma_step(s1, kk + 1, l);
+      sc = next_stage(s1);
+    }
+    fold(l);
+  }
+  wait_vmcnt<0>();  // no DMA outlives the loop
+
+  // C = alpha * X * 2^-(sa_i + sh_j) + beta * C, X = (S - k0) * M * 2^-65
+  const unsigned long long k0 = fc.k0;
+  const double scale = fc.scale;
+#pragma unroll
+  for (int j = 0; j < TN; ++j) {
+    const int gc = n0 + wn0 + 16 * j + lr;
+    if (gc >= N) continue;
+    const int sj = abits - eh[gc];
+    double* cc = C + (size_t)gc * ldc;
+#pragma unroll
+    for (int i = 0; i < TM; ++i) {
+      const int gr0 = m0 + wm0 + 16 * i + 4 * lg;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int gr = gr0 + r;
+        if (gr >= M) continue;
+        const double x = (double)(long long)(S[i][j][r] - k0) * scale;
+        double v = alpha * ldexp(x, -(abits - ea[gr] + sj));
+        if (beta != 0.0) v += beta * cc[gr];
+        cc[gr] = v;
+      }
+    }
+  }
+}
+
+// Grouped fused GEMM: one launch over the 128 x 128 tiles of every problem,
+// all kinds.  The host lists each problem's tiles in kFusedPatch^2 patches;
+// each run of that many consecutive units goes to one XCD, so the workgroups
+// of a patch run there together and share its row and column panels in L2.
+constexpr int kFusedPatch = 4;
+__global__ __launch_bounds__(256, 2) void gsplit_fused_kernel(const SplitProblem* __restrict__ P,
+                                                              const UnitOwner* __restrict__ own,
+                                                              FusedCrt fc, int nunits) {
+  __shared__ __align__(1024) signed char lds[kFusedLds];
+  constexpr int NXCD = 8, G = kFusedPatch * kFusedPatch;
+  int u = blockIdx.x;
+  const int full = nunits / (NXCD * G) * (NXCD * G);
+  if (u < full) u = (u / (NXCD * G)) * (NXCD * G) + (u % NXCD) * G + (u / NXCD) % G;
+  const UnitOwner o = own[u];
+  const SplitProblem& q = P[o.x];
+  const int ntn = (q.N + 127) / 128;
+  const int bmi = o.y / ntn, bni = o.y - bmi * ntn;
+  if (q.bmode == BMODE_TOE)
+    gemm_i8_fused_body<BMODE_TOE>(lds, q.M, q.N, q.kp, q.lpad, q.ares, q.bres, q.aplane, q.bplane,
+                                  fc, q.ea, q.eh, q.abits, q.alpha, q.beta, q.C, q.ldc, bmi, bni);
+  else if (q.bmode == BMODE_HANK)
+    gemm_i8_fused_body<BMODE_HANK>(lds, q.M, q.N, q.kp, q.lpad, q.ares, q.bres, q.aplane,
+                                   q.bplane, fc, q.ea, q.eh, q.abits, q.alpha, q.beta, q.C, q.ldc,
+                                   bmi, bni);
+  else
+    gemm_i8_fused_body<BMODE_DENSE>(lds, q.M, q.N, q.kp, q.lpad, q.ares, q.bres, q.aplane,
+                                    q.bplane, fc, q.ea, q.eh, q.abits, q.alpha, q.beta, q.C, q.ldc,
+                                    bmi, bni);
+}
+
 // Host plan of a grouped split batch (table, unit totals, workspace size).
 struct SplitBatch {
   int t = 0;
+  bool fused = false;  // the fused GEMM + recombination form
   CrtParams cp{};
+  FusedCrt fc{};
   std::vector<SplitProblem> tab;
   int total[NPH] = {0};
   size_t ws_bytes = 0;
 };
 
+// The fused form addresses residue planes with 32-bit offsets: every problem's
+// M * kp (and N * kp where B is a dense plane) must stay below 2^32; its
+// residue reduction needs kp <= kFastModK.
+static bool fused_fits(const BatchProblem& b) {
+  if (b.m == 0 || b.n == 0 || b.k == 0) return true;
+  const size_t kp = ((size_t)b.k + 255) / 256 * 256;
+  if (kp > (size_t)kFastModK) return false;
+  const bool win = (b.kind == KIND_TOEPLITZ || b.kind == KIND_HANKEL) && kp <= kFusedWinKmax;
+  return (size_t)b.m * kp < (1ull << 32) && (win || (size_t)b.n * kp < (1ull << 32));
+}
+
 // Lays out the batch in a workspace starting at `ws` (nullptr: sizing only).
-// Problems must be validated; empty ones are skipped.
+// Problems must be validated; empty ones are skipped.  sb.fused selects the
+// fused form (set by the caller before the first layout).
 void split_batch_layout(const BatchProblem* probs, int nprob, int t, char* ws, SplitBatch& sb) {
   auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
   sb.t = t;
   sb.cp = make_crt(t);
+  sb.fc = make_fused(t);
   sb.tab.clear();
   for (int i = 0; i < NPH; ++i) sb.total[i] = 0;
   size_t off = 0;
@@ -1935,7 +2265,8 @@ void split_batch_layout(const BatchProblem* probs, int nprob, int t, char* ws, S
     q.bmode = (b.kind == KIND_TOEPLITZ) ? BMODE_TOE : (b.kind == KIND_HANKEL) ? BMODE_HANK : BMODE_DENSE;
     q.M = b.m; q.N = b.n; q.K = b.k;
     q.kp = ((b.k + 255) / 256) * 256;
-    if (q.bmode != BMODE_DENSE && q.kp > kBatchKmax) q.bmode = BMODE_DENSE;
+    if (q.bmode != BMODE_DENSE && q.kp > (sb.fused ? kFusedWinKmax : kBatchKmax))
+      q.bmode = BMODE_DENSE;
     q.abits = choose_abits(t, b.k);
     q.alpha = b.alpha; q.beta = b.beta; q.A = b.A; q.lda = b.lda; q.g = b.g;
     q.r0 = b.r0; q.c0 = b.c0; q.C = b.C; q.ldc = b.ldc;
@@ -1959,7 +2290,7 @@ void split_batch_layout(const BatchProblem* probs, int nprob, int t, char* ws, S
     q.e1 = (int*)take(sizeof(int));
     q.ares = (signed char*)take(t * q.aplane);
     q.bres = (signed char*)take(t * q.bplane);
-    q.ores = (unsigned char*)take(t * q.oplane);
+    q.ores = sb.fused ? nullptr : (unsigned char*)take(t * q.oplane);
     const bool dense = q.bmode == BMODE_DENSE;
     const int kc = (q.K + 255) / 256;
     int gx[NPH], gy[NPH];
@@ -1977,12 +2308,16 @@ void split_batch_layout(const BatchProblem* probs, int nprob, int t, char* ws, S
     const bool big = std::min(q.M, q.N) >= kBigTile && q.K >= kBigTile &&
                      (size_t)q.M * q.kp < (1ull << 32) &&
                      (q.bmode != BMODE_DENSE || (size_t)q.N * q.kp < (1ull << 32));
-    int gph = dense ? PH_GEMM_D : (q.bmode == BMODE_TOE ? PH_GEMM_T : PH_GEMM_H);
-    if (big) gph += PH_GEMM_DB - PH_GEMM_D;
-    const int tile = big ? 256 : 128;
-    gx[gph] = ((q.M + tile - 1) / tile) * ((q.N + tile - 1) / tile) * t; gy[gph] = 1;
-    gx[PH_CRT] = (int)(((size_t)q.ldo * q.N / 4 + 256 * kCrtQuads - 1) / (256 * kCrtQuads));
-    gy[PH_CRT] = 1;
+    if (sb.fused) {
+      gx[PH_FUSED] = ((q.M + 127) / 128) * ((q.N + 127) / 128); gy[PH_FUSED] = 1;
+    } else {
+      int gph = dense ? PH_GEMM_D : (q.bmode == BMODE_TOE ? PH_GEMM_T : PH_GEMM_H);
+      if (big) gph += PH_GEMM_DB - PH_GEMM_D;
+      const int tile = big ? 256 : 128;
+      gx[gph] = ((q.M + tile - 1) / tile) * ((q.N + tile - 1) / tile) * t; gy[gph] = 1;
+      gx[PH_CRT] = (int)(((size_t)q.ldo * q.N / 4 + 256 * kCrtQuads - 1) / (256 * kCrtQuads));
+      gy[PH_CRT] = 1;
+    }
     for (int ph = 0; ph < NPH; ++ph) {
       q.gx[ph] = gx[ph];
       q.ub[ph] = sb.total[ph];
@@ -2027,6 +2362,16 @@ int split_batch_enqueue(const SplitBatch& sb, const SplitProblem* dtab, hipStrea
     hipLaunchKernelGGL(kern, dim3(sb.total[ph]), dim3(256), 0, s, dtab,
                        phase_owners(sb, dtab, ph), sb.cp);
     ++n;
+  }
+  if (sb.fused) {
+    if (sb.total[PH_FUSED]) {
+      hipLaunchKernelGGL(gsplit_fused_kernel, dim3(sb.total[PH_FUSED]), dim3(256), 0, s, dtab,
+                         phase_owners(sb, dtab, PH_FUSED), sb.fc, sb.total[PH_FUSED]);
+      ++n;
+    }
+    if (nodes) *nodes = n;
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? 0 : 1000 + (int)e;
   }
   if (sb.total[PH_GEMM_D]) {
     hipLaunchKernelGGL(gsplit_gemm_kernel<BMODE_DENSE>, dim3(sb.total[PH_GEMM_D]), dim3(256), 0, s,
@@ -2098,6 +2443,10 @@ struct GSplit {
 void* gsplit_create(const BatchProblem* probs, int nprob, int t, size_t* ws_bytes) {
   GSplit* h = new GSplit;
   h->probs.assign(probs, probs + nprob);
+  // PSMMA_GSPLIT_FUSED=0 keeps the residue-plane pipeline (read per plan)
+  const char* fe = getenv("PSMMA_GSPLIT_FUSED");
+  const bool fused_on = !(fe && atoi(fe) == 0);
+  h->sb.fused = fused_on && std::all_of(probs, probs + nprob, fused_fits);
   split_batch_layout(h->probs.data(), nprob, t, nullptr, h->sb);
   if (ws_bytes) *ws_bytes = h->sb.ws_bytes;
   return h;
@@ -2117,6 +2466,20 @@ int gsplit_bind(void* hv, void* ws) {
     for (size_t i = 0; i < np; ++i) {
       const int u0 = sb.tab[i].ub[ph];
       const int u1 = (i + 1 < np) ? sb.tab[i + 1].ub[ph] : sb.total[ph];
+      if (ph == PH_FUSED && u1 > u0) {  // (no units unless the batch is fused)
+        // tiles in patch order: P x P patches of the tile grid, row-major inside
+        // a patch, so the P^2 consecutive units one XCD takes share P row and P
+        // column panels in its L2
+        constexpr int P = kFusedPatch;
+        const int ntm = (sb.tab[i].M + 127) / 128, ntn = (sb.tab[i].N + 127) / 128;
+        int u = u0;
+        for (int pr = 0; pr < ntm; pr += P)
+          for (int pc = 0; pc < ntn; pc += P)
+            for (int r = pr; r < std::min(ntm, pr + P); ++r)
+              for (int c = pc; c < std::min(ntn, pc + P); ++c)
+                own[off + u++] = UnitOwner{(int)i, r * ntn + c};
+        continue;
+      }
       for (int u = u0; u < u1; ++u) own[off + u] = UnitOwner{(int)i, u - u0};
     }
     off += (size_t)sb.total[ph];

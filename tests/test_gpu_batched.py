@@ -151,3 +151,81 @@ def test_batched_split_large_tiles(gpu):
     print("split large-tile errs max", max(errs))
     assert max(errs) <= 1e-12
     plan.close()
+
+
+def make_rect_batch(rng, shapes, beta):
+    """Problems of shape (kind, m, n, k): rectangular H, block offsets, beta."""
+    import torch
+
+    probs, keep = [], []
+    for kind, m, n, k in shapes:
+        nglob = max(n, k) + 5
+        r0, c0 = 3, 2
+        if kind == "cauchy":
+            b = inputs.cauchy_generators(nglob, rng)
+            gens = dict(d=b[:, 0].copy(), f=b[:, 1].copy(), u=b[:, 2].copy(), v=b[:, 3].copy())
+            order, lg, korder = ["d", "f", "u", "v"], 0, 0
+        elif kind in ("toeplitz", "hankel"):
+            gens = dict(g=rng.standard_normal(2 * nglob - 1), korder=nglob)
+            order, lg = ["g"], 2 * nglob - 1
+            korder = nglob if kind == "toeplitz" else 0
+        else:
+            e = inputs.eig_merge_inputs(nglob, seed=int(rng.integers(1 << 30)), with_q=False)
+            gens = dict(f=e["f"], u=e["u"], v=e["v"], difl=e["difl"], difr=e["difr"])
+            order, lg, korder = ["f", "u", "v", "difl", "difr"], nglob, 0
+        dgens = [torch.from_numpy(np.ascontiguousarray(gens[q])).cuda() for q in order]
+        a = rng.uniform(-1, 1, (m, k))
+        c = rng.standard_normal((m, n))
+        da = dev_colmajor(a, ld=m + 3)
+        dc = dev_colmajor(c, ld=m + 1)
+        p = api.make_problem(ORA_KIND[kind], -0.75, da, dgens, n, dc, beta=beta, r0=r0, c0=c0,
+                             lg=lg, korder=korder)
+        want = ora_want_k(kind, a, gens, r0, c0, n, beta, c)
+        probs.append(p)
+        keep.append((dc, want, dgens, da, c))
+    return probs, keep
+
+
+def ora_want_k(kind, a, gens, r0, c0, n, beta, c):
+    from oracle import oracle as ora
+
+    return ora.structured_gemm_np(ORA_KIND[kind], -0.75, a, gens, r0, c0, n, beta=beta, c=c)
+
+
+@pytest.mark.parametrize("beta", [0.0, 0.5])
+def test_batched_split_fused_matches_residue_pipeline(gpu, monkeypatch, beta):
+    """The fused split form (all moduli of a 128 x 128 tile in one workgroup, the
+    recombination as a 64-bit fraction in registers) against the oracle and against the
+    residue-plane pipeline on the same batch: ragged tiles, K != N, block offsets, the
+    generator-window Toeplitz / Hankel form (kp <= 2048) and its dense fallback
+    (kp = 2304), every kind, beta = 0 and beta != 0."""
+    import torch
+    from oracle import oracle as ora
+
+    rng = np.random.default_rng(31 if beta == 0.0 else 32)
+    shapes = [("cauchy", 300, 257, 700), ("toeplitz", 129, 390, 1030), ("hankel", 517, 130, 64),
+              ("cauchyeig", 250, 260, 300), ("toeplitz", 700, 650, 2100),
+              ("hankel", 333, 200, 2049), ("cauchy", 1, 1, 1), ("toeplitz", 60, 70, 50),
+              ("hankel", 1030, 1100, 2000), ("cauchy", 1000, 900, 1500)]
+    probs, keep = make_rect_batch(rng, shapes, beta)
+    outs = {}
+    for fused in ("1", "0"):
+        monkeypatch.setenv("PSMMA_GSPLIT_FUSED", fused)
+        plan = api.BatchedPlan(probs, prec=api.PREC_SPLIT_I8, slices=14)
+        for rep in range(2):  # replays: C is reset in between (beta != 0 reads it)
+            for dc, _, _, _, c in keep:
+                dc.copy_(dev_colmajor(c, ld=dc.stride(1)))
+            torch.cuda.synchronize()
+            plan.launch()
+            torch.cuda.synchronize()
+        plan.close()
+        outs[fused] = [host(dc) for dc, _, _, _, _ in keep]
+    for i, (s, (_, want, _, _, _)) in enumerate(zip(shapes, keep)):
+        ef = ora.rel_fro(outs["1"][i], want)
+        eu = ora.rel_fro(outs["0"][i], want)
+        d = ora.rel_fro(outs["1"][i], outs["0"][i])
+        print(f"{s}: fused {ef:.2e} residue-plane {eu:.2e} between {d:.2e}", flush=True)
+        assert ef <= 1e-12 and eu <= 1e-12
+        # the fraction's absolute error (about 2^-54 M) stays within a small factor of
+        # the operand rounding at 14 moduli
+        assert ef <= 4.0 * eu + 1e-15
