@@ -4,26 +4,28 @@
 // blocks; every off-diagonal block is compressed by the batched truncated RRQR
 // (Compress_RRQR semantics, tolerance tol) straight from the generators, and
 // the product runs as
-//   diagonal blocks:       X(:,J) = alpha A(:,J) C(J,J) + beta X(:,J)  (explicit C(J,J), one
-//                          strided-batched DGEMM)
-//   incompressible blocks: X(:,J) += A(:,I) C(I,J)   (structured FP64 MFMA)
 //   low-rank blocks C(I,J) ~ Q_IJ R_IJ:  T_I = A(:,I) [Q_I1 .. Q_In]
-//                                        X(:,J) += [T_1J .. T_nJ] [R_1J; ..; R_nJ]
-//   (each stage one strided-batched DGEMM over the block rows / columns, their widths
-//   padded to the widest with zero columns)
+//   diagonal blocks with the low-rank columns:
+//       X(:,J) = alpha (A(:,J) C(J,J) + [T_1J .. T_nJ] [R_1J; ..; R_nJ]) + beta X(:,J)
+//   incompressible blocks: X(:,J) += alpha A(:,I) C(I,J)   (structured FP64 MFMA)
+// Both GEMM stages are single launches of the persistent FP64 batch kernel on its
+// 256 x 128 leaf tiles: the first over the block rows (A(:, I) U_I as the explicit
+// segment of an otherwise empty contraction), the second over the block columns,
+// whose diagonal block is generated in LDS and whose low-rank columns follow as an
+// explicit tail of the same contraction (T'_J R_J, each block column at its own
+// width).
 // The T columns are regrouped from (I, J) to (J, I) order between the two
-// GEMM stages, so each stage is one plain GEMM per block row / block column.
+// stages, so each block column's tail is one contiguous panel of T'.
 // This is the reference's off-diagonal ring step (PSTOEPLITZ_COMPUTE's
 // RRQR branch, SRC/pstoeplitz_compute.f90:263-288, which compresses every
 // off-diagonal tile with Compress_RRQR and multiplies A by [Q | R P^T] with two
-// DGEMMs) applied on one GPU to a grid of leaf-sized tiles, with the plain
-// GEMMs grouped per block row and column.
+// DGEMMs) applied on one GPU to a grid of leaf-sized tiles, with the GEMMs
+// grouped per block row and column.
 // The compressor is the batched RRQR (Compress_RRQR) or, as an alternative,
 // the generator-only RRLU (Compress_cauchyEig2, SRC/CauchyHssEig.f90:2640-2829,
 // the reference's compressor of off-diagonal eigen-Cauchy ring tiles), whose
 // row ID C(I,J) ~ Q Y fills the same Q and R slots.
 #include <hip/hip_runtime.h>
-#include <rocblas/rocblas.h>
 
 #include <algorithm>
 #include <cstdio>
@@ -32,6 +34,7 @@
 #include <vector>
 
 #include "../../include/psmma.h"
+#include "batch_stages.h"
 #include "psmma_common.h"
 #include "workspace.h"
 
@@ -57,12 +60,13 @@ constexpr int kKmax = 32;
 struct PackEntry {
   int slot;   // block index of the RRQR outputs
   int r;      // rank
-  int offa;   // column offset in U (block-row order (I, J))
+  int offa;   // column offset in U and T (block-row order (I, J))
   int offb;   // column offset in T' and row offset in R (block-column order (J, I))
 };
 
-// U(:, offa + c) = Q_slot(:, c) and R(offb + c, :) = (R P^T)_slot(c, :) (R has S
-// rows: its leading dimension), one workgroup per block.
+// U(:, offa + c) = Q_slot(:, c) and R(offb + c, :) = (R P^T)_slot(c, :), both row-major
+// (U: leaf x S, R: rows of leaf entries), the layout of the batch kernel's explicit
+// tail operand; one workgroup per block.
 __global__ __launch_bounds__(256) void blr_pack_kernel(const PackEntry* __restrict__ tab, int leaf,
                                                        const double* __restrict__ qs,
                                                        const double* __restrict__ rs,
@@ -73,9 +77,9 @@ __global__ __launch_bounds__(256) void blr_pack_kernel(const PackEntry* __restri
   const double* rp = rs + (size_t)e.slot * kKmax * leaf;
   for (int x = threadIdx.x; x < leaf * e.r; x += 256) {
     const int i = x % leaf, c = x / leaf;
-    U[(size_t)(e.offa + c) * leaf + i] = q[(size_t)c * leaf + i];
+    U[(size_t)i * S + e.offa + c] = q[(size_t)c * leaf + i];
     // R P^T is stored column-major with leading dimension kKmax
-    R[(size_t)i * S + e.offb + c] = rp[(size_t)i * kKmax + c];
+    R[(size_t)(e.offb + c) * leaf + i] = rp[(size_t)i * kKmax + c];
   }
 }
 
@@ -111,30 +115,6 @@ __global__ __launch_bounds__(256) void blr_index_kernel(const int* __restrict__ 
     pl[(size_t)b * leaf + t] = r0[b] + t + 1;
     pu[(size_t)b * leaf + t] = c0[b] + t + 1;
   }
-}
-
-// The diagonal blocks C(J,J) explicitly, column-major leaf x leaf each: grid (nb, leaf),
-// thread i of workgroup (J, j) writes entry (i, j) of block J.
-template <int KIND>
-__global__ __launch_bounds__(256) void blr_diag_kernel(GenPtrs g, int leaf, double* __restrict__ H) {
-  const int J = blockIdx.x, j = blockIdx.y, i = threadIdx.x;
-  if (i < leaf)
-    H[((size_t)J * leaf + j) * leaf + i] = gen_entry<KIND>(g, J * leaf + i, J * leaf + j);
-}
-
-std::mutex g_mu;
-std::unordered_map<int, rocblas_handle> g_blas;
-
-rocblas_handle blas() {
-  int dev = 0;
-  hipGetDevice(&dev);
-  std::lock_guard<std::mutex> lk(g_mu);
-  auto it = g_blas.find(dev);
-  if (it != g_blas.end()) return it->second;
-  rocblas_handle h = nullptr;
-  if (rocblas_create_handle(&h) != rocblas_status_success) return nullptr;
-  g_blas[dev] = h;
-  return h;
 }
 
 size_t align_up(size_t x) { return (x + 255) & ~size_t(255); }
@@ -237,10 +217,9 @@ int blr_product(int kind, const GenPtrs& g, int m, int nglob, double alpha, cons
       ranksum += r;
     }
   }
-  // Column slots: block row I owns wA columns of U and T (its blocks' ranks, then
-  // zero padding), block column J owns wB columns of T' and rows of R, with wA / wB
-  // the widest block row / column, so that each GEMM stage is one strided-batched
-  // call of uniform shape (the padding adds zero columns, not error).
+  // Column slots: block row I owns wA columns of U and T (its blocks' ranks first),
+  // block column J owns wB columns of T' and rows of R, with wA / wB the widest block
+  // row / column; each stage problem takes only its own row's or column's width.
   std::vector<PackEntry> tab(nblk);
   std::vector<int> wrow(nb, 0), wcol(nb, 0);
   for (int I = 0; I < nb; ++I)
@@ -278,15 +257,14 @@ int blr_product(int kind, const GenPtrs& g, int m, int nglob, double alpha, cons
     stats[4] = ranksum;
   }
 
-  // second workspace part: pack table, U (leaf x S), R (SR x leaf), T (m x S), T' (m x SR),
-  // the explicit diagonal blocks (nb x leaf x leaf)
+  // second workspace part: pack table, U (leaf x S, row-major), R (SR x leaf, row-major),
+  // T (m x S), T' (m x SR)
   const size_t off_tab = 0;
   const size_t off_u = align_up(off_tab + sizeof(PackEntry) * std::max<size_t>(1, packed.size()));
   const size_t off_rr = align_up(off_u + sizeof(double) * (size_t)leaf * S);
   const size_t off_t = align_up(off_rr + sizeof(double) * (size_t)SR * leaf);
   const size_t off_tp = align_up(off_t + sizeof(double) * (size_t)m * S);
-  const size_t off_h = align_up(off_tp + sizeof(double) * (size_t)m * SR);
-  const size_t total = align_up(off_h + sizeof(double) * (size_t)nb * leaf * leaf);
+  const size_t total = align_up(off_tp + sizeof(double) * (size_t)m * SR);
   char* ws2 = (char*)ws_lease(WS_BLR_GEMM, total, s, lk_gemm);
   if (!ws2) return 1002;
   PackEntry* dtab = (PackEntry*)(ws2 + off_tab);
@@ -300,69 +278,80 @@ int blr_product(int kind, const GenPtrs& g, int m, int nglob, double alpha, cons
     if (e != hipSuccess) return 1000 + (int)e;
   }
 
-  rocblas_handle h = blas();
-  if (!h) return 1004;
-  rocblas_set_stream(h, s);
-  // diagonal blocks first (they carry beta; disjoint outputs): generated explicitly, then
-  // one strided-batched DGEMM X(:,J) = alpha A(:,J) C(J,J) + beta X(:,J) (at this short
-  // contraction rocBLAS's batched kernel runs at ~64 TF/s, the fused-generator grouped
-  // kernel at ~37); then incompressible blocks one by one (they accumulate)
-  {
-    double* H = (double*)(ws2 + off_h);
-    if (eig)
-      blr_diag_kernel<KIND_CAUCHYEIG><<<dim3(nb, leaf), 256, 0, s>>>(g, leaf, H);
-    else
-      blr_diag_kernel<KIND_CAUCHY><<<dim3(nb, leaf), 256, 0, s>>>(g, leaf, H);
-    if ((e = hipGetLastError()) != hipSuccess) return 1000 + (int)e;
-    if (rocblas_dgemm_strided_batched(h, rocblas_operation_none, rocblas_operation_none, m, leaf,
-                                      leaf, &alpha, a, lda, (rocblas_stride)leaf * lda, H, leaf,
-                                      (rocblas_stride)leaf * leaf, &beta, c, ldc,
-                                      (rocblas_stride)leaf * ldc, nb) != rocblas_status_success)
-      return 1005;
+  // stage problems: T_I = A(:, I) U_I per block row; X(:, J) = alpha (A(:, J) C(J, J) +
+  // T'_J R_J) + beta X(:, J) per block column (generated diagonal block, explicit tail)
+  const bool lowrank = !packed.empty();
+  std::vector<std::vector<BatchProblem>> stg(2);
+  auto problem = [&](int kd, int n, int k, double al, const double* A, int ld, double be,
+                     double* C, int ldC) {
+    BatchProblem b{};
+    b.kind = kd;
+    b.m = m;
+    b.n = n;
+    b.k = k;
+    b.alpha = al;
+    b.beta = be;
+    b.A = A;
+    b.lda = ld;
+    b.C = C;
+    b.ldc = ldC;
+    return b;
+  };
+  // (each block row and column takes its own width: the padding is never multiplied).
+  // T_I is a plain product: an empty generated segment and A(:, I) U_I as the tail.
+  if (lowrank)
+    for (int I = 0; I < nb; ++I) {
+      BatchProblem b = problem(kind, wrow[I], 0, 1.0, a, lda, 0.0, T + (size_t)I * m * wA, m);
+      b.g = g;
+      b.A2 = a + (size_t)I * leaf * lda;
+      b.lda2 = lda;
+      b.H2 = U + (size_t)I * wA;
+      b.ldh2 = (int)S;
+      b.k2 = leaf;
+      stg[0].push_back(b);
+    }
+  for (int J = 0; J < nb; ++J) {
+    BatchProblem b = problem(kind, leaf, leaf, alpha, a + (size_t)J * leaf * lda, lda, beta,
+                             c + (size_t)J * leaf * ldc, ldc);
+    b.g = g;
+    b.g.nrows = (J + 1) * leaf;
+    b.g.ncols = (J + 1) * leaf;
+    b.r0 = b.c0 = J * leaf;
+    if (lowrank) {
+      b.A2 = Tp + (size_t)J * m * wB;
+      b.lda2 = m;
+      b.H2 = R + (size_t)J * wB * leaf;
+      b.ldh2 = leaf;
+      b.k2 = wcol[J];
+    }
+    stg[1].push_back(b);
   }
+  BatchStages bs;
+  int st = bs.upload(stg, {BKSET_LEAF, BKSET_LEAF}, 256, WS_BLR_TABLES, s);
+  if (st) return st;
+  if (lowrank) {
+    {
+      blr_pack_kernel<<<(unsigned)packed.size(), 256, 0, s>>>(dtab, leaf, dq, drp, U, R, S);
+      e = hipGetLastError();
+    }
+    if (e != hipSuccess) {
+      fprintf(stderr, "compressed structured product: %s\n", hipGetErrorString(e));
+      return 1000 + (int)e;
+    }
+    if ((st = bs.launch(0))) return st;  // T_I = A(:, I) U_I
+    dim3 grid((unsigned)packed.size(), (unsigned)std::min(64, (m / 2 + 255) / 256 + 1));
+    blr_regroup_kernel<<<grid, 256, 0, s>>>(dtab, m, T, Tp, m);
+    if ((e = hipGetLastError()) != hipSuccess) return 1000 + (int)e;
+  }
+  if ((st = bs.launch(1))) return st;  // the diagonal blocks carry beta and the tails
+  // incompressible blocks one by one (they accumulate)
   for (size_t x = 0; x < dense.size() && e == hipSuccess; ++x) {
     const int b = dense[x], I = hr0[b] / leaf, J = hc0[b] / leaf;
     e = launch_right_f64(kind, -1, m, leaf, leaf, alpha, a + (size_t)I * leaf * lda,
                          lda, g, I * leaf, J * leaf, 1.0, c + (size_t)J * leaf * ldc, ldc, s);
   }
-  if (e == hipSuccess && !packed.empty()) {
-    // padding of U and R must be zero (the stages multiply through it)
-    e = hipMemsetAsync(U, 0, sizeof(double) * (size_t)leaf * S, s);
-    if (e == hipSuccess) e = hipMemsetAsync(R, 0, sizeof(double) * (size_t)SR * leaf, s);
-    if (e == hipSuccess) {
-      blr_pack_kernel<<<(unsigned)packed.size(), 256, 0, s>>>(dtab, leaf, dq, drp, U, R, SR);
-      e = hipGetLastError();
-    }
-    // T' columns past each block column's ranks: zero (R's matching rows are zero too, but
-    // stale memory could hold non-finite values)
-    for (int J = 0; J < nb && e == hipSuccess; ++J)
-      if (wcol[J] < wB)
-        e = hipMemsetAsync(Tp + ((size_t)J * wB + wcol[J]) * m, 0,
-                           sizeof(double) * (size_t)m * (wB - wcol[J]), s);
-  }
-  if (e != hipSuccess) {
-    fprintf(stderr, "compressed structured product: %s\n", hipGetErrorString(e));
-    return 1000 + (int)e;
-  }
-  if (!packed.empty()) {
-    const double one = 1.0, zero = 0.0;
-    // T_I = A(:, I) U_I for every block row I
-    if (rocblas_dgemm_strided_batched(h, rocblas_operation_none, rocblas_operation_none, m, wA,
-                                      leaf, &one, a, lda, (rocblas_stride)leaf * lda, U, leaf,
-                                      (rocblas_stride)leaf * wA, &zero, T, m,
-                                      (rocblas_stride)m * wA, nb) != rocblas_status_success)
-      return 1005;
-    dim3 grid((unsigned)packed.size(), (unsigned)std::min(64, (m / 2 + 255) / 256 + 1));
-    blr_regroup_kernel<<<grid, 256, 0, s>>>(dtab, m, T, Tp, m);
-    if ((e = hipGetLastError()) != hipSuccess) return 1000 + (int)e;
-    // X(:, J) += alpha T'_J R_J for every block column J
-    if (rocblas_dgemm_strided_batched(h, rocblas_operation_none, rocblas_operation_none, m, leaf,
-                                      wB, &alpha, Tp, m, (rocblas_stride)m * wB, R, (int)SR,
-                                      (rocblas_stride)wB, &one, c, ldc, (rocblas_stride)leaf * ldc,
-                                      nb) != rocblas_status_success)
-      return 1005;
-  }
-  return 0;
+  if (e != hipSuccess) return 1000 + (int)e;
+  return bs.finish();
 }
 
 }  // namespace

@@ -44,6 +44,7 @@
 #include <vector>
 
 #include "../../include/psmma.h"
+#include "batch_stages.h"
 #include "psmma_common.h"
 #include "workspace.h"
 
@@ -307,49 +308,11 @@ namespace {
 // ksets: the batch kernel's kind set per stage (default: the HSS set for all).
 int run_stages(const std::vector<std::vector<BatchProblem>>& stages, hipStream_t s,
                int big_min = 256, const std::vector<int>& ksets = {}) {
-  auto kset_of = [&](size_t i) { return i < ksets.size() ? ksets[i] : kKsetHss; };
-  std::vector<std::vector<BatchProblem>> probs(stages.size());
-  std::vector<std::vector<int>> tbeg(stages.size());
-  std::vector<int> ntiles(stages.size());
-  auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
-  size_t bytes = 0;
-  for (size_t i = 0; i < stages.size(); ++i) {
-    ntiles[i] = batch_tiles(stages[i], big_min, probs[i], tbeg[i], kset_of(i));
-    bytes += al(sizeof(BatchProblem) * probs[i].size()) + al(sizeof(int) * tbeg[i].size()) +
-             al(2 * sizeof(unsigned));
-  }
-  std::vector<char> host(bytes, 0);
-  std::vector<size_t> offp(stages.size()), offt(stages.size()), offc(stages.size());
-  size_t o = 0;
-  for (size_t i = 0; i < stages.size(); ++i) {
-    offp[i] = o;
-    if (!probs[i].empty())
-      memcpy(host.data() + o, probs[i].data(), sizeof(BatchProblem) * probs[i].size());
-    o += al(sizeof(BatchProblem) * probs[i].size());
-    offt[i] = o;
-    if (!tbeg[i].empty()) memcpy(host.data() + o, tbeg[i].data(), sizeof(int) * tbeg[i].size());
-    o += al(sizeof(int) * tbeg[i].size());
-    offc[i] = o;  // counters: zero
-    o += al(2 * sizeof(unsigned));
-  }
-  std::unique_lock<std::mutex> lk;
-  char* d = (char*)ws_lease(WS_HSS_TABLES, bytes, s, lk);
-  if (!d) return 1002;
-  hipError_t e = hipMemcpyAsync(d, host.data(), bytes, hipMemcpyHostToDevice, s);
-  if (e != hipSuccess) return 1000 + (int)e;
-  int dev = 0, cus = 256;
-  hipGetDevice(&dev);
-  hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-  for (size_t i = 0; i < stages.size(); ++i) {
-    if (ntiles[i] == 0) continue;
-    e = launch_batch_f64((const BatchProblem*)(d + offp[i]), (const int*)(d + offt[i]),
-                         (int)probs[i].size(), ntiles[i], (unsigned*)(d + offc[i]),
-                         kset_of(i) == kKsetLeaf ? cus : 2 * cus, s, kset_of(i));
-    if (e != hipSuccess) return 1000 + (int)e;
-  }
-  // the tables must outlive the launches: they stay leased until the stream has them
-  e = hipStreamSynchronize(s);
-  return e == hipSuccess ? 0 : 1000 + (int)e;
+  BatchStages bs;
+  int st = bs.upload(stages, ksets, big_min, WS_HSS_TABLES, s);
+  for (size_t i = 0; i < stages.size() && st == 0; ++i) st = bs.launch(i);
+  const int fin = bs.finish();  // the tables must outlive the launches
+  return st ? st : fin;
 }
 
 BatchProblem dense_problem(int kind, int m, int n, int k, double alpha, const double* a, int lda,

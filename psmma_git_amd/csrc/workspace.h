@@ -15,6 +15,7 @@ enum WsSlot : int {
   WS_TUNE = 4,      // split-depth measurement: sampled FP64 and split results
   WS_HSS_TABLES = 5,  // HSS stages: problem tables of the persistent launches
   WS_HSS_WORK = 6,    // HSS product: the up- and down-pass matrices G and F
+  WS_BLR_TABLES = 7,  // compressed products: problem tables of the batch launches
 };
 
 // Returns a buffer of at least `bytes` for (current device, s, slot), or nullptr
