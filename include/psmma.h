@@ -41,9 +41,10 @@ extern "C" {
  *                       (exact int32 accumulation) and rebuilt by the Chinese
  *                       remainder theorem in double-double FP64.  13 moduli keep
  *                       43 bits per operand at K = 8192, 14 keep 47.
- *   PSMMA_PREC_SPLIT_BF16 the same with bf16 slices on bf16 MFMA, FP32
- *                       accumulation over exactly representable K-chunks and
- *                       FP64 recombination.
+ *   PSMMA_PREC_SPLIT_BF16 the same with bf16 slices on bf16 MFMA: 1..3 slices
+ *                       run every kept slice pair into one FP32 accumulator
+ *                       (3 slices: ~1e-7); 4..7 slices sum each class exactly
+ *                       in FP32 over K-chunks and recombine in FP64.
  * The pole differences of the Cauchy kinds are always formed in FP64.
  *
  * slices = 0 asks for the default of the form: the fewest moduli that keep 45
@@ -242,9 +243,11 @@ typedef struct psmma_problem {
 /* All problems in one go on `stream`.  prec: PSMMA_PREC_FP64 (one persistent
  * launch over the tiles of every problem, all kinds and tile classes, longest
  * tiles first, taken from an atomic work counter) or PSMMA_PREC_SPLIT_I8 (each
- * phase of the int8 modular pipeline is one grouped launch over all problems;
- * slices = number of moduli, 0 = the fewest that keep 45 bits per operand for
- * the longest contraction of the batch).  The problem table is copied to the
+ * per-element phase of the int8 modular pipeline is one grouped launch over all
+ * problems, then one launch runs every 128 x 128 tile's residue GEMMs for all
+ * moduli with the recombination in registers; slices = number of moduli, 0 =
+ * the fewest that keep 45 bits per operand for the longest contraction of the
+ * batch).  The problem table is copied to the
  * device before the call returns; split calls on one device share a workspace
  * and must not overlap. */
 int psmma_batched_dev(int nprob, const psmma_problem* probs, int prec, int slices, void* stream);
