@@ -289,36 +289,53 @@ __global__ void col_exp_kernel(int K, int N, GenPtrs g, int r0, int c0, int kchu
   col_exp_kernel_body<KIND>(K, N, g, r0, c0, kchunk, eh, blockIdx.x, blockIdx.y);
 }
 
-// Residues of A: reads a 64 x 64 tile of column-major A, writes t row-major
-// int8 planes Ares[l][i * kp + k] (zero for k >= K; kp multiple of 128).
+// Residues of A: a unit is 64 rows x kSplitAK k of column-major A, taken as 64 x 64
+// tiles transposed through LDS; the next tile's global loads are in flight while
+// the current tile's residues are formed and stored.  Writes t row-major int8
+// planes Ares[l][i * kp + k] (zero for k >= K; kp a multiple of kSplitAK).
+constexpr int kSplitAK = 256;
 __device__ __forceinline__ void split_a_kernel_body(int M, int K, const double* __restrict__ A,
                                                       int lda, const int* __restrict__ ea,
                                                       int abits, CrtParams cp,
                                                       signed char* __restrict__ res, int kp,
                                                       size_t plane, int bx_, int by_) {
   __shared__ double tile[64][65];
-  const int i0 = bx_ * 64, k0 = by_ * 64;
+  const int i0 = bx_ * 64, kbase = by_ * kSplitAK;
   const int t = threadIdx.x;
-#pragma unroll
-  for (int q = 0; q < 16; ++q) {
-    int ii = t % 64, kk = t / 64 + 4 * q;
-    int gi = i0 + ii, gk = k0 + kk;
-    tile[kk][ii] = (gi < M && gk < K) ? gld(A + (size_t)gk * lda + gi) : 0.0;
-  }
-  __syncthreads();
-  // thread -> row i0 + t/4, 16 consecutive k
+  // thread -> row i0 + t/4, 16 consecutive k of each tile
   const int ii = t / 4, kb = (t % 4) * 16;
   const int gi = i0 + ii;
-  if (gi >= M) return;
-  const int sh = abits - gld(ea + gi);
-  double v[16];
+  const bool rowok = gi < M;
+  const int sh = rowok ? abits - gld(ea + gi) : 0;
+  double pre[16];
+  auto load = [&](int k0) {
 #pragma unroll
-  for (int e = 0; e < 16; ++e) v[e] = rint(ldexp(tile[kb + e][ii], sh));
-  for (int l = 0; l < cp.t; ++l) {
-    const int m = cp.m[l];
-    const double rm = cp.rm[l];
-    v4u* dst = (v4u*)(res + l * plane + (size_t)gi * kp + k0 + kb);
-    gst(dst, m > 253 ? residue_words<true>(v, m, rm) : residue_words<false>(v, m, rm));
+    for (int q = 0; q < 16; ++q) {
+      const int iq = t % 64, kk = t / 64 + 4 * q;
+      const int gr = i0 + iq, gk = k0 + kk;
+      pre[q] = (gr < M && gk < K) ? gld(A + (size_t)gk * lda + gr) : 0.0;
+    }
+  };
+  load(kbase);
+#pragma unroll 1
+  for (int st = 0; st < kSplitAK / 64; ++st) {
+    const int k0 = kbase + st * 64;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) tile[t / 64 + 4 * q][t % 64] = pre[q];
+    __syncthreads();
+    if (st + 1 < kSplitAK / 64) load(k0 + 64);
+    if (rowok) {
+      double v[16];
+#pragma unroll
+      for (int e = 0; e < 16; ++e) v[e] = rint(ldexp(tile[kb + e][ii], sh));
+      for (int l = 0; l < cp.t; ++l) {
+        const int m = cp.m[l];
+        const double rm = cp.rm[l];
+        v4u* dst = (v4u*)(res + l * plane + (size_t)gi * kp + k0 + kb);
+        gst(dst, m > 253 ? residue_words<true>(v, m, rm) : residue_words<false>(v, m, rm));
+      }
+    }
+    __syncthreads();  // every row view is read before the next tile is stored
   }
 }
 
@@ -1271,7 +1288,7 @@ static int enqueue_split_i8(int t, int M, int N, int K, double alpha, const doub
   const int kchunk = 256;
   hipLaunchKernelGGL(row_exp_kernel, dim3((M + 255) / 256, (K + kchunk - 1) / kchunk), dim3(256), 0,
                      s, M, K, A, lda, kchunk, ea);
-  hipLaunchKernelGGL(split_a_kernel, dim3((M + 63) / 64, kp / 64), dim3(256), 0, s, M, K, A, lda,
+  hipLaunchKernelGGL(split_a_kernel, dim3((M + 63) / 64, kp / kSplitAK), dim3(256), 0, s, M, K, A, lda,
                      ea, abits, cp, ares, kp, L.aplane);
   if constexpr (KIND == KIND_TOEPLITZ || KIND == KIND_HANKEL) {
     if (L.window) {
@@ -2300,7 +2317,7 @@ void split_batch_layout(const BatchProblem* probs, int nprob, int t, char* ws, S
     if (dense) { gx[PH_COLEXP] = (q.N + 255) / 256; gy[PH_COLEXP] = kc; }
     else { gx[PH_COLEXP] = (int)((L + 255) / 256); gy[PH_COLEXP] = 1; }
     if (!dense) { gx[PH_EHFILL] = (q.N + 255) / 256; gy[PH_EHFILL] = 1; }
-    gx[PH_SPLITA] = (q.M + 63) / 64; gy[PH_SPLITA] = q.kp / 64;
+    gx[PH_SPLITA] = (q.M + 63) / 64; gy[PH_SPLITA] = q.kp / kSplitAK;
     if (dense) { gx[PH_SPLITB] = (q.N + 15) / 16; gy[PH_SPLITB] = q.kp / 256; }
     else { gx[PH_SPLITB] = (q.lpad + 255) / 256; gy[PH_SPLITB] = 1; }
     // the large class: the direct-to-LDS tiles, whose residue planes must be
