@@ -229,3 +229,25 @@ def test_batched_split_fused_matches_residue_pipeline(gpu, monkeypatch, beta):
         # the fraction's absolute error (about 2^-54 M) stays within a small factor of
         # the operand rounding at 14 moduli
         assert ef <= 4.0 * eu + 1e-15
+
+
+def test_batched_split_long_contractions_take_residue_planes(gpu):
+    """K > 16384 leaves the fused form (its residue reduction needs kp <= 16384) and the
+    generator-window form (the window holds at most 16384 steps): the whole batch runs
+    the residue-plane pipeline, with the Toeplitz / Hankel operands as dense residue planes
+    and their column exponents from the generators."""
+    import torch
+    from oracle import oracle as ora
+
+    rng = np.random.default_rng(33)
+    shapes = [("toeplitz", 70, 50, 16385), ("hankel", 40, 60, 16390), ("cauchy", 30, 20, 16400),
+              ("cauchyeig", 50, 40, 300)]
+    probs, keep = make_rect_batch(rng, shapes, 0.0)
+    plan = api.BatchedPlan(probs, prec=api.PREC_SPLIT_I8, slices=15)
+    plan.launch()
+    torch.cuda.synchronize()
+    plan.close()
+    for s, (dc, want, _, _, _) in zip(shapes, keep):
+        err = ora.rel_fro(host(dc), want)
+        print(f"{s}: {err:.2e}", flush=True)
+        assert err <= 1e-12

@@ -43,7 +43,9 @@ def test_window_kinds_n8192(gpu, kind):
     rows = np.arange(n)[:, None]
     h = g_np[(n - 1) + cols[None, :] - rows] if kind == "toeplitz" else g_np[rows + cols[None, :]]
     want = a_np @ h
-    for prec, slices, bar in ((api.PREC_FP64, 0, 1e-13), (api.PREC_SPLIT_I8, 13, 1e-12)):
+    # the configuration's 3-slice bf16 form (FP32 accumulation) stays near 1e-7
+    for prec, slices, bar in ((api.PREC_FP64, 0, 1e-13), (api.PREC_SPLIT_I8, 13, 1e-12),
+                              (api.PREC_SPLIT_BF16, 3, 5e-7)):
         if kind == "toeplitz":
             api.toeplitz_dev(1.0, a, g, n, n, c, prec=prec, slices=slices)
         else:
