@@ -19,6 +19,14 @@ enum BatchKset : int { BKSET_PUBLIC = 0, BKSET_HSS = 1, BKSET_LEAF = 2 };
 
 class BatchStages {
  public:
+  BatchStages() = default;
+  BatchStages(const BatchStages&) = delete;
+  BatchStages& operator=(const BatchStages&) = delete;
+  // A caller that returns early (an error after upload) still must not free the
+  // tables' host copy or the lease while the stream may read them.
+  ~BatchStages() {
+    if (lk_.owns_lock()) (void)hipStreamSynchronize(s_);
+  }
   // Tiles every stage (problems with min(m, n) >= big_min take the large tile
   // class of their set) and copies all tables to a buffer leased from `slot` of
   // the stream's workspaces.  Returns 0, 1002 (no memory) or 1000 + hipError.
