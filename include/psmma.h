@@ -425,9 +425,11 @@ int psmma_rrlu_cauchy_batched_dev(int nblk, int m, int n, const int* pl, const i
  * C is split into leaf x leaf blocks (leaf <= 256, n % leaf == 0); every
  * off-diagonal block is compressed by the batched truncated RRQR at relative
  * tolerance tol (psmma_rrqr_batched_dev semantics, rank cap 32) from the
- * generators, and multiplied in compressed form; the diagonal blocks are
- * evaluated and multiplied by one strided-batched DGEMM, incompressible blocks
- * use the structured FP64 kernel.  The reference's off-diagonal ring
+ * generators, and multiplied in compressed form (two launches of the persistent
+ * FP64 batch kernel: A's block rows times the Q factors, then per block column
+ * the diagonal block generated in LDS with the low-rank columns as the explicit
+ * tail of its contraction); incompressible blocks use the structured FP64
+ * kernel.  The reference's off-diagonal ring
  * step (SRC/pstoeplitz_compute.f90:263-288: Compress_RRQR per tile, then two
  * DGEMMs) on a grid of leaf tiles.  Error: each block within ~tol of its largest
  * R entry.  stats (host, may be NULL, 5 entries): blocks, low-rank blocks,
@@ -466,6 +468,25 @@ int psmma_cauchy_blr_dev(int m, int n, double alpha, const double* a, int lda, c
                          const double* f, const double* u, const double* v, int leaf, double tol,
                          int compressor, double beta, double* c, int ldc, long long* stats,
                          void* stream);
+
+/* Compressed Toeplitz and Hankel products X = alpha * A * H + beta * X, H the n x n
+ * block of psmma_toeplitz_dev (H(r,c) = g[korder - 1 + c - r]) or psmma_hankel_dev
+ * (H(r,c) = g[r + c]), 0 <= r, c < n: every off-diagonal leaf block is compressed by
+ * the batched truncated RRQR straight from the generator (ConstToeplitzlowrank ->
+ * Compress_RRQR, SRC/Auxil_2p5D.f90:968-1189; tol as there, the reference uses
+ * 1e-13) and multiplied in compressed form, blocks of rank above 32 or full rank
+ * take the structured FP64 kernel (the reference regenerates the tile,
+ * SRC/Auxil_2p5D.f90:1054-1056); the diagonal blocks run on the window tiles.  The
+ * reference's off-diagonal Toeplitz ring step (SRC/pstoeplitz_compute.f90:263-295)
+ * on a grid of leaf tiles; the reference's Hankel ring never compresses
+ * (SRC/pshankel_compute.f90:238-265), this is its analogue.  stats and
+ * synchronisation as psmma_cauchyeig_blr_dev.  Returns 0 or -(argument number). */
+int psmma_toeplitz_blr_dev(int m, int n, double alpha, const double* a, int lda, const double* g,
+                           int lg, int korder, int leaf, double tol, double beta, double* c,
+                           int ldc, long long* stats, void* stream);
+int psmma_hankel_blr_dev(int m, int n, double alpha, const double* a, int lda, const double* g,
+                         int lg, int leaf, double tol, double beta, double* c, int ldc,
+                         long long* stats, void* stream);
 
 /* ===========================================================================
  * 6b. Nested HSS representation and HSS x dense product (device pointers).

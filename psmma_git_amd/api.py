@@ -78,6 +78,8 @@ _SIGS = {
                                       _I),
     "psmma_cauchy_blr_dev": ([_I, _I, _D, _P, _I, _P, _P, _P, _P, _I, _D, _I, _D, _P, _I, _P, _P],
                              _I),
+    "psmma_toeplitz_blr_dev": ([_I, _I, _D, _P, _I, _P, _I, _I, _I, _D, _D, _P, _I, _P, _P], _I),
+    "psmma_hankel_blr_dev": ([_I, _I, _D, _P, _I, _P, _I, _I, _D, _D, _P, _I, _P, _P], _I),
     "psmma_cauchyeig_blr_ex_dev": ([_I, _I, _D, _P, _I, _P, _P, _P, _P, _P, _I, _D, _I, _D, _P,
                                     _I, _P, _P], _I),
     "psmma_cauchyeig_blr_dev": ([_I, _I, _D, _P, _I, _P, _P, _P, _P, _P, _I, _D, _D, _P, _I, _P,
@@ -459,6 +461,34 @@ def cauchy_blr_dev(alpha, a, d, f, u, v, c, leaf=256, tol=1e-12, beta=0.0, strea
     _check(lib().psmma_cauchy_blr_dev(m, n, alpha, pa, lda, *gp, leaf, float(tol),
                                       int(compressor), beta, pc, ldc, st, _stream_ptr(stream)),
            "psmma_cauchy_blr_dev")
+    return dict(zip(("blocks", "lowrank", "dense", "zero", "rank_sum"), list(st)))
+
+
+def toeplitz_blr_dev(alpha, a, g, korder, c, leaf=256, tol=1e-13, beta=0.0, stream=None):
+    """X = alpha * A * H + beta * X, H(r, c) = g[korder - 1 + c - r] (n x n, n = a's
+    columns), off-diagonal leaf blocks compressed by the batched RRQR from the generator
+    (ConstToeplitzlowrank -> Compress_RRQR, SRC/Auxil_2p5D.f90:968-1189).  a, c:
+    column-major (m x n) device views; g: float64 device vector.  Returns the stats dict."""
+    pa, m, n, lda = _colmajor(a, "a")
+    pc, ldc = _out(c, m, n)
+    pg = _vec(g, "g", korder - 1 + n)
+    st = (ctypes.c_longlong * 5)()
+    _check(lib().psmma_toeplitz_blr_dev(m, n, alpha, pa, lda, pg, int(g.numel()), int(korder),
+                                        leaf, float(tol), beta, pc, ldc, st,
+                                        _stream_ptr(stream)), "psmma_toeplitz_blr_dev")
+    return dict(zip(("blocks", "lowrank", "dense", "zero", "rank_sum"), list(st)))
+
+
+def hankel_blr_dev(alpha, a, g, c, leaf=256, tol=1e-13, beta=0.0, stream=None):
+    """X = alpha * A * H + beta * X, H(r, c) = g[r + c] (n x n), off-diagonal leaf blocks
+    compressed by the batched RRQR from the generator.  Returns the stats dict."""
+    pa, m, n, lda = _colmajor(a, "a")
+    pc, ldc = _out(c, m, n)
+    pg = _vec(g, "g", 2 * n - 1)
+    st = (ctypes.c_longlong * 5)()
+    _check(lib().psmma_hankel_blr_dev(m, n, alpha, pa, lda, pg, int(g.numel()), leaf,
+                                      float(tol), beta, pc, ldc, st, _stream_ptr(stream)),
+           "psmma_hankel_blr_dev")
     return dict(zip(("blocks", "lowrank", "dense", "zero", "rank_sum"), list(st)))
 
 

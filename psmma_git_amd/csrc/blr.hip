@@ -298,11 +298,16 @@ int blr_product(int kind, const GenPtrs& g, int m, int nglob, double alpha, cons
     return b;
   };
   // (each block row and column takes its own width: the padding is never multiplied).
-  // T_I is a plain product: an empty generated segment and A(:, I) U_I as the tail.
+  // T_I is a plain product on the leaf tiles: an empty generated segment (whose
+  // generators are then never read; the kind only picks the tile code) and
+  // A(:, I) U_I as the tail.
+  const bool window = kind == KIND_TOEPLITZ || kind == KIND_HANKEL;
+  const int tail_kind = window ? KIND_CAUCHY : kind;
   if (lowrank)
     for (int I = 0; I < nb; ++I) {
-      BatchProblem b = problem(kind, wrow[I], 0, 1.0, a, lda, 0.0, T + (size_t)I * m * wA, m);
-      b.g = g;
+      BatchProblem b = problem(tail_kind, wrow[I], 0, 1.0, a, lda, 0.0,
+                               T + (size_t)I * m * wA, m);
+      if (!window) b.g = g;
       b.A2 = a + (size_t)I * leaf * lda;
       b.lda2 = lda;
       b.H2 = U + (size_t)I * wA;
@@ -310,6 +315,10 @@ int blr_product(int kind, const GenPtrs& g, int m, int nglob, double alpha, cons
       b.k2 = leaf;
       stg[0].push_back(b);
     }
+  // The diagonal blocks: the Cauchy kinds take the low-rank columns as the tail of
+  // the same contraction; the window kinds (Toeplitz / Hankel tiles take no tail)
+  // run them on their own window tiles, then the tails as a third stage.
+  stg.resize(window ? 3 : 2);
   for (int J = 0; J < nb; ++J) {
     BatchProblem b = problem(kind, leaf, leaf, alpha, a + (size_t)J * leaf * lda, lda, beta,
                              c + (size_t)J * leaf * ldc, ldc);
@@ -317,17 +326,22 @@ int blr_product(int kind, const GenPtrs& g, int m, int nglob, double alpha, cons
     b.g.nrows = (J + 1) * leaf;
     b.g.ncols = (J + 1) * leaf;
     b.r0 = b.c0 = J * leaf;
+    BatchProblem tl = problem(tail_kind, leaf, 0, alpha, a, lda, 1.0, b.C, ldc);
+    BatchProblem& tb = window ? tl : b;
     if (lowrank) {
-      b.A2 = Tp + (size_t)J * m * wB;
-      b.lda2 = m;
-      b.H2 = R + (size_t)J * wB * leaf;
-      b.ldh2 = leaf;
-      b.k2 = wcol[J];
+      tb.A2 = Tp + (size_t)J * m * wB;
+      tb.lda2 = m;
+      tb.H2 = R + (size_t)J * wB * leaf;
+      tb.ldh2 = leaf;
+      tb.k2 = wcol[J];
     }
     stg[1].push_back(b);
+    if (window && lowrank && wcol[J] > 0) stg[2].push_back(tl);
   }
   BatchStages bs;
-  int st = bs.upload(stg, {BKSET_LEAF, BKSET_LEAF}, 256, WS_BLR_TABLES, s);
+  int st = window
+               ? bs.upload(stg, {BKSET_LEAF, BKSET_PUBLIC, BKSET_LEAF}, 256, WS_BLR_TABLES, s)
+               : bs.upload(stg, {BKSET_LEAF, BKSET_LEAF}, 256, WS_BLR_TABLES, s);
   if (st) return st;
   if (lowrank) {
     {
@@ -343,7 +357,8 @@ int blr_product(int kind, const GenPtrs& g, int m, int nglob, double alpha, cons
     blr_regroup_kernel<<<grid, 256, 0, s>>>(dtab, m, T, Tp, m);
     if ((e = hipGetLastError()) != hipSuccess) return 1000 + (int)e;
   }
-  if ((st = bs.launch(1))) return st;  // the diagonal blocks carry beta and the tails
+  if ((st = bs.launch(1))) return st;  // the diagonal blocks carry beta (and the tails)
+  if (window && (st = bs.launch(2))) return st;  // the window kinds' tails, after beta
   // incompressible blocks one by one (they accumulate)
   for (size_t x = 0; x < dense.size() && e == hipSuccess; ++x) {
     const int b = dense[x], I = hr0[b] / leaf, J = hc0[b] / leaf;
@@ -403,4 +418,45 @@ extern "C" int psmma_cauchyeig_blr_dev(int m, int nglob, double alpha, const dou
   const int st = psmma_cauchyeig_blr_ex_dev(m, nglob, alpha, a, lda, fpole, u, v, difl, difr, leaf,
                                             tol, PSMMA_BLR_RRQR, beta, c, ldc, stats, stream);
   return st == -16 ? -15 : st;  // argument numbers of this entry point
+}
+
+// Compressed Toeplitz / Hankel products: the reference's off-diagonal Toeplitz ring
+// step (ConstToeplitzlowrank -> Compress_RRQR -> two DGEMMs, or the regenerated tile
+// at full rank, SRC/pstoeplitz_compute.f90:263-295, SRC/Auxil_2p5D.f90:968-1189) on
+// a grid of leaf tiles of the n x n block H(0:n, 0:n).
+extern "C" int psmma_toeplitz_blr_dev(int m, int n, double alpha, const double* a, int lda,
+                                      const double* g, int lg, int korder, int leaf, double tol,
+                                      double beta, double* c, int ldc, long long* stats,
+                                      void* stream) {
+  if (m < 0) return -1;
+  if (n < 0) return -2;
+  if (lda < std::max(1, m)) return -5;
+  if (leaf < 1 || leaf > 256 || (n % leaf) != 0) return -9;
+  if (!(tol >= 0.0)) return -10;
+  if (ldc < std::max(1, m)) return -13;
+  if (m == 0 || n == 0) return 0;
+  if (!a || !c) return -4;
+  // H(r, c) = g[korder - 1 + c - r] over 0 <= r, c < n
+  if (!g || korder - n < 0 || (long long)korder - 1 + n - 1 >= lg) return -6;
+  GenPtrs gp{g, nullptr, nullptr, nullptr, nullptr, korder, n, n};
+  return blr_product(KIND_TOEPLITZ, gp, m, n, alpha, a, lda, leaf, tol, PSMMA_BLR_RRQR, beta, c,
+                     ldc, stats, (hipStream_t)stream);
+}
+
+extern "C" int psmma_hankel_blr_dev(int m, int n, double alpha, const double* a, int lda,
+                                    const double* g, int lg, int leaf, double tol, double beta,
+                                    double* c, int ldc, long long* stats, void* stream) {
+  if (m < 0) return -1;
+  if (n < 0) return -2;
+  if (lda < std::max(1, m)) return -5;
+  if (leaf < 1 || leaf > 256 || (n % leaf) != 0) return -8;
+  if (!(tol >= 0.0)) return -9;
+  if (ldc < std::max(1, m)) return -12;
+  if (m == 0 || n == 0) return 0;
+  if (!a || !c) return -4;
+  // H(r, c) = g[r + c] over 0 <= r, c < n
+  if (!g || 2LL * n - 2 >= lg) return -6;
+  GenPtrs gp{g, nullptr, nullptr, nullptr, nullptr, 0, n, n};
+  return blr_product(KIND_HANKEL, gp, m, n, alpha, a, lda, leaf, tol, PSMMA_BLR_RRQR, beta, c,
+                     ldc, stats, (hipStream_t)stream);
 }

@@ -119,24 +119,27 @@ __device__ __forceinline__ void right_tile(int M, int N, int K, double alpha,
   const bool colok = (n0 + hcol) < N;
   int gcol = c0 + n0 + hcol;
   if constexpr (IDX) gcol = colok ? g.cidx[n0 + hcol] - 1 : 0;
+  // (a problem with an empty generated segment, K = 0, is a plain product on its
+  // explicit tail: its generators are never read)
+  const bool genok = colok && K > 0;
   double cv0 = 0.0, cv1 = 0.0, cv2 = 0.0, cv3 = 0.0, cv4 = 0.0;
   if constexpr (KIND == KIND_CAUCHY) {
-    if (colok) {
+    if (genok) {
       cv0 = g.p1[gcol];  // F_c
       cv1 = g.p3[gcol];  // V_c
     }
   } else if constexpr (KIND == KIND_CAUCHY_T) {
-    if (colok) {
+    if (genok) {
       cv0 = g.p0[gcol];  // D of C's row c
       cv1 = g.p2[gcol];  // U of C's row c
     }
   } else if constexpr (KIND == KIND_CAUCHYEIG_T) {
-    if (colok) {
+    if (genok) {
       cv0 = g.p0[gcol];  // F of C's row c
       cv1 = g.p1[gcol];  // U of C's row c
     }
   } else if constexpr (KIND == KIND_CAUCHYEIG) {
-    if (colok) {
+    if (genok) {
       cv0 = g.p0[gcol];                                   // F_c
       cv1 = (gcol + 1 < g.korder) ? g.p0[gcol + 1] : 0.0;  // F_{c+1}
       cv2 = g.p2[gcol];                                   // V_c
