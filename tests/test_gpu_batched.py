@@ -251,3 +251,26 @@ def test_batched_split_long_contractions_take_residue_planes(gpu):
         err = ora.rel_fro(host(dc), want)
         print(f"{s}: {err:.2e}", flush=True)
         assert err <= 1e-12
+
+
+def test_batched_split_fused_random_shapes(gpu):
+    """Forty problems of random kind and shape (m, n, k in [1, 700], block offsets,
+    beta = 0 or 1.25) through the fused split plan at the measured depth, each against
+    the oracle."""
+    import torch
+    from oracle import oracle as ora
+
+    rng = np.random.default_rng(34)
+    kinds = ["cauchy", "toeplitz", "hankel", "cauchyeig"]
+    shapes = [(kinds[int(rng.integers(4))], int(rng.integers(1, 701)), int(rng.integers(1, 701)),
+               int(rng.integers(1, 701))) for _ in range(40)]
+    for beta in (0.0, 1.25):
+        probs, keep = make_rect_batch(rng, shapes, beta)
+        plan = api.BatchedPlan(probs, prec=api.PREC_SPLIT_I8, slices=0)
+        t, _ = plan.info()
+        plan.launch()
+        torch.cuda.synchronize()
+        plan.close()
+        errs = [ora.rel_fro(host(dc), want) for dc, want, _, _, _ in keep]
+        print(f"beta {beta}: depth {t}, max err {max(errs):.2e}", flush=True)
+        assert max(errs) <= 1e-12
