@@ -40,7 +40,9 @@ extern "C" {
  *                       (2..16) pairwise-coprime moduli <= 256 on int8 MFMA
  *                       (exact int32 accumulation) and rebuilt by the Chinese
  *                       remainder theorem in double-double FP64.  13 moduli keep
- *                       43 bits per operand at K = 8192, 14 keep 47.
+ *                       43 bits per operand at K = 8192, 14 keep 47; the Cauchy
+ *                       kinds give the structured operand two of A's bits (its
+ *                       columns carry their norm in the entries near the pole).
  *   PSMMA_PREC_SPLIT_BF16 the same with bf16 slices on bf16 MFMA: 1..3 slices
  *                       run every kept slice pair into one FP32 accumulator
  *                       (3 slices: ~1e-7); 4..7 slices sum each class exactly
@@ -327,9 +329,10 @@ int psmma_redistribute_dev(int m, int n, int nprow, int npcol, int src_mb, int s
  * stream's workspace).  Synchronises `stream`.  Returns 0, 1 when no depth
  * reaches the target (the largest is returned), -(argument number).  Batched
  * split plans created with slices = 0 take the largest depth this chooses at
- * 2e-13 (a fifth of the 1e-12 goal, a margin for the columns not sampled and
- * the fused form's rounding) over the longest-K problem of each kind, whose
- * contraction keeps the fewest bits per operand. */
+ * 4e-13 (the rest of the 1e-12 goal is a margin for the columns not sampled,
+ * shorter problems with larger operand rounding and the fused form's fraction
+ * rounding) over the longest-K problem of each kind, whose contraction keeps
+ * the fewest bits per operand. */
 int psmma_autotune_f64(int kind, int m, int n, int k, int* cfg, float* ms);
 int psmma_split_slices_dev(const psmma_problem* p, int prec, double target, int* slices,
                            double* err, void* stream);

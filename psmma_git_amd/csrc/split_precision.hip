@@ -8,8 +8,9 @@
 // "Ozaki scheme II" family):
 //
 //  1. scale each row i of A by 2^sa_i and each column j of H by 2^sh_j so the
-//     largest entry is below 2^abits, and round to integers A', H'
-//     (|A'|, |H'| <= 2^abits; this rounding is the only approximation);
+//     largest entry is below 2^a (rows of A) or 2^h (columns of H), and round
+//     to integers A', H' (this rounding is the only approximation; a = h =
+//     abits, except for the Cauchy kinds, whose H takes two bits from A);
 //  2. for t pairwise-coprime moduli m_l <= 256 form the symmetric residues
 //     A'_l = A' mod m_l, H'_l = H' mod m_l as int8 matrices;
 //  3. compute C'_l = A'_l * H'_l exactly on v_mfma_i32_16x16x64_i8 (int32
@@ -17,7 +18,7 @@
 //     C'_l mod m_l;
 //  4. rebuild X = A' * H' from its residues (Chinese remainder theorem, Garner's
 //     mixed-radix form with signed digits, evaluated in double-double FP64); X is
-//     determined exactly because M = prod m_l > 2 K 2^(2 abits) > 2 |X|; scale
+//     determined exactly because M = prod m_l > 2 K 2^(a + h) > 2 |X|; scale
 //     back: C = alpha * X * 2^-(sa_i + sh_j) + beta * C.
 //
 // The structured operand is evaluated from its generators (FP64, divisions of
@@ -171,6 +172,27 @@ static int choose_abits(int t, int K) {
   int ab = ((int)std::floor(lg) - 2 - lk) / 2;
   if (ab > 52) ab = 52;
   return ab;
+}
+
+// Bits of the two operands (A rows to a bits, H columns to h bits), a + h <=
+// 2 choose_abits, so |X| <= K 2^(a + h) < M / 4 as above.  The Cauchy kinds give H
+// kCauchyHShift bits more and A as many fewer: a column of U_k V_j / (D_k - F_j)
+// carries its norm in the few entries near the pole, while the rounding error of
+// every entry is up to 2^-h of that column's largest, so H's rounding dominates
+// (summed over K entries against a norm of a few); A is dense and balanced.
+// Measured with an exact model of the rounding on the 512-problem batch at t = 13
+// (max over its Cauchy problems 1.24e-12 -> 3.3e-13; Toeplitz / Hankel unchanged
+// at 1.3e-13 with no shift) and on the eigenvector merge at N = 16384, t = 14
+// (2.6e-13 -> 8.6e-14; shifts of 1 and 3 bits lie between).
+constexpr int kCauchyHShift = 2;
+struct OpBits {
+  int a, h;
+};
+static OpBits choose_bits(int t, int K, int kind) {
+  const int ab = choose_abits(t, K);
+  const int d = (kind == KIND_CAUCHY || kind == KIND_CAUCHYEIG) ? kCauchyHShift : 0;
+  const int h = std::min(52, ab + d);
+  return {ab - (h - ab), h};
 }
 
 // ---------------------------------------------------------------------------
@@ -1208,14 +1230,14 @@ static void launch_gemm_i8(int t, int M, int N, int kp, const signed char* ares,
 }
 
 static int finish_crt(int t, int M, int N, const unsigned char* ores, size_t oplane, int ldo,
-                      const CrtParams& cp, const int* ea, const int* eh, int abits, double alpha,
-                      double beta, double* C, int ldc, hipStream_t s) {
+                      const CrtParams& cp, const int* ea, const int* eh, int abits, int hbits,
+                      double alpha, double beta, double* C, int ldc, hipStream_t s) {
   const size_t nquad = (size_t)ldo * N / 4;
   dim3 cg((unsigned)((nquad + 255) / 256));
 #define PSMMA_CRT_CASE(TT)                                                                     \
   case TT:                                                                                     \
     hipLaunchKernelGGL(crt_kernel<TT>, cg, dim3(256), 0, s, M, N, ores, oplane, ldo, cp, ea,    \
-                       eh, abits, abits, alpha, beta, C, ldc);                                 \
+                       eh, abits, hbits, alpha, beta, C, ldc);                                 \
     break;
   switch (t) {
     PSMMA_CRT_CASE(2) PSMMA_CRT_CASE(3) PSMMA_CRT_CASE(4) PSMMA_CRT_CASE(5) PSMMA_CRT_CASE(6)
@@ -1275,7 +1297,8 @@ static int enqueue_split_i8(int t, int M, int N, int K, double alpha, const doub
                             const GenPtrs& g, int r0, int c0, double beta, double* C, int ldc,
                             const I8Layout& L, char* ws, hipStream_t s) {
   const CrtParams cp = make_crt(t);
-  const int abits = choose_abits(t, K);
+  const OpBits bits = choose_bits(t, K, KIND);
+  const int abits = bits.a, hbits = bits.h;
   const int kp = L.kp, ldo = L.ldo;
   int* ea = (int*)ws;
   int* eh = (int*)(ws + L.off_eh);
@@ -1301,20 +1324,22 @@ static int enqueue_split_i8(int t, int M, int N, int K, double alpha, const doub
                          g.p0, L.glo, L.ghi, e1);
       hipLaunchKernelGGL(fill_from_kernel, dim3((N + 255) / 256), dim3(256), 0, s, eh, N, e1);
       hipLaunchKernelGGL(gen_residue_kernel, dim3((L.lpad + 255) / 256), dim3(256), 0, s, g.p0,
-                         L.glo, L.ghi, TOE ? 1 : 0, e1, abits, cp, hres, (size_t)L.lpad, L.lpad);
+                         L.glo, L.ghi, TOE ? 1 : 0, e1, hbits, cp, hres, (size_t)L.lpad, L.lpad);
       launch_gemm_i8<TOE ? BMODE_TOE : BMODE_HANK>(t, M, N, kp, ares, hres, L.aplane,
                                                      (size_t)L.lpad, L.lpad, cp, ores, L.oplane,
                                                      ldo, s);
-      return finish_crt(t, M, N, ores, L.oplane, ldo, cp, ea, eh, abits, alpha, beta, C, ldc, s);
+      return finish_crt(t, M, N, ores, L.oplane, ldo, cp, ea, eh, abits, hbits, alpha, beta, C,
+                        ldc, s);
     }
   }
   hipLaunchKernelGGL(col_exp_kernel<KIND>, dim3((N + 255) / 256, (K + kchunk - 1) / kchunk),
                      dim3(256), 0, s, K, N, g, r0, c0, kchunk, eh);
   hipLaunchKernelGGL(split_h_kernel<KIND>, dim3((N + 15) / 16, (kp + 255) / 256), dim3(256), 0, s,
-                     K, N, g, r0, c0, eh, abits, cp, hres, kp, L.hplane);
+                     K, N, g, r0, c0, eh, hbits, cp, hres, kp, L.hplane);
   launch_gemm_i8<BMODE_DENSE>(t, M, N, kp, ares, hres, L.aplane, L.hplane, 0, cp, ores, L.oplane,
                               ldo, s);
-  return finish_crt(t, M, N, ores, L.oplane, ldo, cp, ea, eh, abits, alpha, beta, C, ldc, s);
+  return finish_crt(t, M, N, ores, L.oplane, ldo, cp, ea, eh, abits, hbits, alpha, beta, C, ldc,
+                    s);
 }
 
 static int enqueue_split_i8_kind(int kind, int t, int M, int N, int K, double alpha,
@@ -1801,7 +1826,7 @@ enum {
 };
 
 struct SplitProblem {
-  int kind, bmode, M, N, K, kp, lpad, abits;
+  int kind, bmode, M, N, K, kp, lpad, abits, hbits;
   double alpha, beta;
   const double* A;
   int lda;
@@ -1869,19 +1894,19 @@ __global__ __launch_bounds__(256) void gsplit_phase_kernel(const SplitProblem* _
       break;
     case PH_SPLITB:
       if (!dense) {
-        gen_residue_kernel_body(q.g.p0, q.glo, q.ghi, q.bmode == BMODE_TOE ? 1 : 0, q.e1, q.abits,
+        gen_residue_kernel_body(q.g.p0, q.glo, q.ghi, q.bmode == BMODE_TOE ? 1 : 0, q.e1, q.hbits,
                                 cp, q.bres, q.bplane, q.lpad, bx, 0);
       } else if (q.kind == KIND_CAUCHY) {
-        split_h_kernel_body<KIND_CAUCHY>(q.K, q.N, q.g, q.r0, q.c0, q.eh, q.abits, cp, q.bres,
+        split_h_kernel_body<KIND_CAUCHY>(q.K, q.N, q.g, q.r0, q.c0, q.eh, q.hbits, cp, q.bres,
                                          q.kp, q.bplane, bx, by);
       } else if (q.kind == KIND_TOEPLITZ) {
-        split_h_kernel_body<KIND_TOEPLITZ>(q.K, q.N, q.g, q.r0, q.c0, q.eh, q.abits, cp, q.bres,
+        split_h_kernel_body<KIND_TOEPLITZ>(q.K, q.N, q.g, q.r0, q.c0, q.eh, q.hbits, cp, q.bres,
                                            q.kp, q.bplane, bx, by);
       } else if (q.kind == KIND_HANKEL) {
-        split_h_kernel_body<KIND_HANKEL>(q.K, q.N, q.g, q.r0, q.c0, q.eh, q.abits, cp, q.bres,
+        split_h_kernel_body<KIND_HANKEL>(q.K, q.N, q.g, q.r0, q.c0, q.eh, q.hbits, cp, q.bres,
                                          q.kp, q.bplane, bx, by);
       } else {
-        split_h_kernel_body<KIND_CAUCHYEIG>(q.K, q.N, q.g, q.r0, q.c0, q.eh, q.abits, cp, q.bres,
+        split_h_kernel_body<KIND_CAUCHYEIG>(q.K, q.N, q.g, q.r0, q.c0, q.eh, q.hbits, cp, q.bres,
                                             q.kp, q.bplane, bx, by);
       }
       break;
@@ -1931,7 +1956,7 @@ __global__ __launch_bounds__(256) void gsplit_crt_kernel(const SplitProblem* __r
   const size_t e0 = (size_t)o.y * (256 * kCrtQuads) + threadIdx.x;
 #pragma unroll 1
   for (int h = 0; h < kCrtQuads; ++h)
-    crt_unit<T>(q.M, q.N, q.ores, q.oplane, q.ldo, cp, q.ea, q.eh, q.abits, q.abits, q.alpha,
+    crt_unit<T>(q.M, q.N, q.ores, q.oplane, q.ldo, cp, q.ea, q.eh, q.abits, q.hbits, q.alpha,
                 q.beta, q.C, q.ldc, e0 + (size_t)h * 256);
 }
 
@@ -1947,7 +1972,7 @@ __global__ __launch_bounds__(256) void gsplit_crt_kernel(const SplitProblem* __r
 // y_l = (M / m_l)^-1 mod m_l (M = prod m_l).  By the Chinese remainder theorem
 // X / M = sum_l r_l y_l / m_l (mod 1), so after the last modulus
 //     S - sum_l 128 W_l = 2^65 X / M  (mod 2^64),
-// and that is X / M itself: abits keeps |X| <= K 2^(2 abits) < M / 4, so
+// and that is X / M itself: the operand bits keep |X| <= K 2^(a + h) < M / 4, so
 // |2^65 X / M| < 2^63 fits a signed 64-bit integer.  The rounding of the W_l
 // adds at most t * 128 * 1/2 units of 2^-65 to X / M: |dX| <= t 2^-59 M, about
 // 2^-45 of a typical |X| at K = 2048, below the 2^-abits operand rounding.
@@ -2000,7 +2025,8 @@ __device__ __forceinline__ void gemm_i8_fused_body(
     signed char* __restrict__ lds, int M, int N, int kp, int lpad,
     const signed char* __restrict__ Ares, const signed char* __restrict__ Bres, size_t aplane,
     size_t bplane, const FusedCrt& fc, const int* __restrict__ ea, const int* __restrict__ eh,
-    int abits, double alpha, double beta, double* __restrict__ C, int ldc, int bmi, int bni) {
+    int abits, int hbits, double alpha, double beta, double* __restrict__ C, int ldc, int bmi,
+    int bni) {
   constexpr bool WIN = BMODE != BMODE_DENSE;
   constexpr int BM = 128, BN = 128, BK = 64, NS = 5, D = 3;
   constexpr int WM = 64, WN = 64, TM = 4, TN = 4, NT = 256;
@@ -2190,7 +2216,7 @@ __device__ __forceinline__ void gemm_i8_fused_body(
   for (int j = 0; j < TN; ++j) {
     const int gc = n0 + wn0 + 16 * j + lr;
     if (gc >= N) continue;
-    const int sj = abits - eh[gc];
+    const int sj = hbits - eh[gc];
     double* cc = C + (size_t)gc * ldc;
 #pragma unroll
     for (int i = 0; i < TM; ++i) {
@@ -2227,15 +2253,16 @@ __global__ __launch_bounds__(256, 2) void gsplit_fused_kernel(const SplitProblem
   const int bmi = o.y / ntn, bni = o.y - bmi * ntn;
   if (q.bmode == BMODE_TOE)
     gemm_i8_fused_body<BMODE_TOE>(lds, q.M, q.N, q.kp, q.lpad, q.ares, q.bres, q.aplane, q.bplane,
-                                  fc, q.ea, q.eh, q.abits, q.alpha, q.beta, q.C, q.ldc, bmi, bni);
+                                  fc, q.ea, q.eh, q.abits, q.hbits, q.alpha, q.beta, q.C, q.ldc,
+                                  bmi, bni);
   else if (q.bmode == BMODE_HANK)
     gemm_i8_fused_body<BMODE_HANK>(lds, q.M, q.N, q.kp, q.lpad, q.ares, q.bres, q.aplane,
-                                   q.bplane, fc, q.ea, q.eh, q.abits, q.alpha, q.beta, q.C, q.ldc,
-                                   bmi, bni);
+                                   q.bplane, fc, q.ea, q.eh, q.abits, q.hbits, q.alpha, q.beta,
+                                   q.C, q.ldc, bmi, bni);
   else
     gemm_i8_fused_body<BMODE_DENSE>(lds, q.M, q.N, q.kp, q.lpad, q.ares, q.bres, q.aplane,
-                                    q.bplane, fc, q.ea, q.eh, q.abits, q.alpha, q.beta, q.C, q.ldc,
-                                    bmi, bni);
+                                    q.bplane, fc, q.ea, q.eh, q.abits, q.hbits, q.alpha, q.beta,
+                                    q.C, q.ldc, bmi, bni);
 }
 
 // Host plan of a grouped split batch (table, unit totals, workspace size).
@@ -2284,7 +2311,11 @@ void split_batch_layout(const BatchProblem* probs, int nprob, int t, char* ws, S
     q.kp = ((b.k + 255) / 256) * 256;
     if (q.bmode != BMODE_DENSE && q.kp > (sb.fused ? kFusedWinKmax : kBatchKmax))
       q.bmode = BMODE_DENSE;
-    q.abits = choose_abits(t, b.k);
+    {
+      const OpBits bits = choose_bits(t, b.k, b.kind);
+      q.abits = bits.a;
+      q.hbits = bits.h;
+    }
     q.alpha = b.alpha; q.beta = b.beta; q.A = b.A; q.lda = b.lda; q.g = b.g;
     q.r0 = b.r0; q.c0 = b.c0; q.C = b.C; q.ldc = b.ldc;
     const bool toe = b.kind == KIND_TOEPLITZ;
