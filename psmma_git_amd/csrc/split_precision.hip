@@ -311,53 +311,55 @@ __global__ void col_exp_kernel(int K, int N, GenPtrs g, int r0, int c0, int kchu
   col_exp_kernel_body<KIND>(K, N, g, r0, c0, kchunk, eh, blockIdx.x, blockIdx.y);
 }
 
-// Residues of A: a unit is 64 rows x kSplitAK k of column-major A, taken as 64 x 64
-// tiles transposed through LDS; the next tile's global loads are in flight while
-// the current tile's residues are formed and stored.  Writes t row-major int8
-// planes Ares[l][i * kp + k] (zero for k >= K; kp a multiple of kSplitAK).
+// Residues of A: a unit is 64 rows x kSplitAK k of column-major A, taken as four
+// 16 x 256 sub-tiles transposed through LDS (each wave then stores runs of 256
+// bytes of four rows per modulus: 275 us against 362 us for 64 x 64 sub-tiles,
+// whose 64-byte runs cost the stores, at M = K = 8192, t = 13 on MI355X); the
+// next sub-tile's global loads are in flight while the current one's residues
+// are formed and stored.  Writes t row-major int8 planes Ares[l][i * kp + k]
+// (zero for k >= K; kp a multiple of kSplitAK).
 constexpr int kSplitAK = 256;
 __device__ __forceinline__ void split_a_kernel_body(int M, int K, const double* __restrict__ A,
                                                       int lda, const int* __restrict__ ea,
                                                       int abits, CrtParams cp,
                                                       signed char* __restrict__ res, int kp,
                                                       size_t plane, int bx_, int by_) {
-  __shared__ double tile[64][65];
+  __shared__ double tile[16 * 273];  // 16 rows x 256 k: groups of 16 k padded to 17
   const int i0 = bx_ * 64, kbase = by_ * kSplitAK;
   const int t = threadIdx.x;
-  // thread -> row i0 + t/4, 16 consecutive k of each tile
-  const int ii = t / 4, kb = (t % 4) * 16;
-  const int gi = i0 + ii;
-  const bool rowok = gi < M;
-  const int sh = rowok ? abits - gld(ea + gi) : 0;
+  // thread -> row t / 16 of each 16-row sub-tile, 16 consecutive k from (t % 16) * 16
+  const int ir = t / 16, kg = t % 16, kb = kg * 16;
   double pre[16];
-  auto load = [&](int k0) {
+  auto load = [&](int r0) {
 #pragma unroll
     for (int q = 0; q < 16; ++q) {
-      const int iq = t % 64, kk = t / 64 + 4 * q;
-      const int gr = i0 + iq, gk = k0 + kk;
+      const int iq = t % 16, kk = t / 16 + 16 * q;
+      const int gr = i0 + r0 + iq, gk = kbase + kk;
       pre[q] = (gr < M && gk < K) ? gld(A + (size_t)gk * lda + gr) : 0.0;
     }
   };
-  load(kbase);
+  load(0);
 #pragma unroll 1
-  for (int st = 0; st < kSplitAK / 64; ++st) {
-    const int k0 = kbase + st * 64;
+  for (int st = 0; st < 4; ++st) {
+    const int r0 = st * 16;
 #pragma unroll
-    for (int q = 0; q < 16; ++q) tile[t / 64 + 4 * q][t % 64] = pre[q];
+    for (int q = 0; q < 16; ++q) tile[(t % 16) * 273 + q * 17 + t / 16] = pre[q];
     __syncthreads();
-    if (st + 1 < kSplitAK / 64) load(k0 + 64);
-    if (rowok) {
+    if (st + 1 < 4) load(r0 + 16);
+    const int gi = i0 + r0 + ir;
+    if (gi < M) {
+      const int sh = abits - gld(ea + gi);
       double v[16];
 #pragma unroll
-      for (int e = 0; e < 16; ++e) v[e] = rint(ldexp(tile[kb + e][ii], sh));
+      for (int e = 0; e < 16; ++e) v[e] = rint(ldexp(tile[ir * 273 + kg * 17 + e], sh));
       for (int l = 0; l < cp.t; ++l) {
         const int m = cp.m[l];
         const double rm = cp.rm[l];
-        v4u* dst = (v4u*)(res + l * plane + (size_t)gi * kp + k0 + kb);
+        v4u* dst = (v4u*)(res + l * plane + (size_t)gi * kp + kbase + kb);
         gst(dst, m > 253 ? residue_words<true>(v, m, rm) : residue_words<false>(v, m, rm));
       }
     }
-    __syncthreads();  // every row view is read before the next tile is stored
+    __syncthreads();  // every row view is read before the next sub-tile is stored
   }
 }
 
