@@ -36,6 +36,11 @@ class BatchStages {
   int launch(size_t i);
   // Synchronises the stream: the tables stay leased until the launches are done.
   int finish();
+  // Stage i's device problem table (after upload) and its problem count, for a
+  // caller that runs that stage on a kernel of its own.
+  const BatchProblem* table(size_t i) const { return (const BatchProblem*)(d_ + offp_[i]); }
+  int problems(size_t i) const { return nprob_[i]; }
+  hipStream_t stream() const { return s_; }
 
  private:
   std::vector<int> ntiles_, nprob_, kset_;

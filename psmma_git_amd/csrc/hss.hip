@@ -303,14 +303,197 @@ struct Hss {
 
 namespace {
 
-// Uploads the stage tables and runs each stage as one persistent launch (in order on s).
+// Panel products of the HSS passes: C_i = alpha A_i H_i + beta C_i with A_i the
+// m-row panel of A or of the pass's G / F workspace (column-major), H_i an explicit
+// n_i x k_i factor (U, B or V^T; n_i a rank or the sum of two, <= kPanelNmax) and
+// short contractions.  The batch kernel's square tiles pad such an n to 64 or 128
+// columns and, at four waves per 64 x 64 tile, stream the panel at 1.5 TB/s (up
+// pass over the leaves of the N = 16384 merge: 1.39 ms for 2.1 GB of A).  Here one
+// workgroup owns 64 rows and every column of one problem (NB 16-column blocks, the
+// stage's widest problem rounded up to an even count); each wave holds 16 rows.
+// Every element of A feeds one wave only, so each wave loads its own A fragments
+// straight into registers, two 16-deep slabs ahead; H's slab, which all waves
+// share, goes through LDS.  Measured on MI355X, N = 16384 merge: the product 7.59 ->
+// 6.15 ms (up pass over the leaves 0.80 ms, the other passes 1.32 ms against 2.06);
+// 128-row workgroups (two row blocks per wave) hold twice the registers and spill
+// from six column blocks up, and staging A through LDS with one slab of register
+// prefetch left too few bytes in flight (12.4 ms).
+constexpr int kPanelBM = 64, kPanelBK = 16, kPanelNmax = 160;
+template <int KIND, int NB>  // NB: 16-column blocks (the stage's widest problem, rounded up)
+__global__ __launch_bounds__(256, 2) void hss_panel_kernel(const BatchProblem* __restrict__ P) {
+  static_assert(KIND == KIND_DENSE || KIND == KIND_DENSE_T, "explicit factors only");
+  constexpr int BM = kPanelBM, BK = kPanelBK, RB = BM / 64, NW = 16 * NB;
+  constexpr int HLD = NW + 4, HREG = NW * BK / 256, KQ = BK / 4;
+  static_assert(NW * BK % 256 == 0 && NW <= kPanelNmax, "H slab over the threads");
+  __shared__ double Hs[2][BK * HLD];
+  const BatchProblem& q = P[blockIdx.y];
+  const int M = q.m, N = q.n, K = q.k;
+  const int m0 = blockIdx.x * BM;
+  if (m0 >= M || N <= 0) return;  // uniform: the grid covers the tallest problem
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int lr = lane & 15, lk = lane >> 4;
+  const int wm0 = wave * 16 * RB;
+  const double* __restrict__ A = q.A;
+  const double* __restrict__ H = q.g.p0;
+  const size_t lda = (size_t)q.lda, ldh = (size_t)q.g.korder;
+
+  v4d acc[RB][NB];
+#pragma unroll
+  for (int i = 0; i < RB; ++i)
+#pragma unroll
+    for (int j = 0; j < NB; ++j) acc[i][j] = (v4d){0.0, 0.0, 0.0, 0.0};
+
+  // the MFMA A operand of lane (lr, lk): row wm0 + 16 i + lr, k = 4 kq + lk of the slab
+  double a0[KQ][RB], a1[KQ][RB];
+  auto load_a = [&](double(&af)[KQ][RB], int k0) {
+#pragma unroll
+    for (int kq = 0; kq < KQ; ++kq)
+#pragma unroll
+      for (int i = 0; i < RB; ++i) {
+        const int gr = m0 + wm0 + 16 * i + lr, gk = k0 + 4 * kq + lk;
+        af[kq][i] = (gr < M && gk < K) ? A[(size_t)gk * lda + gr] : 0.0;
+      }
+  };
+  // H's slab k0, every column (along k for a column-major H, along the columns
+  // for its transpose)
+  double hr[HREG];
+  auto load_h = [&](int k0) {
+#pragma unroll
+    for (int r = 0; r < HREG; ++r) {
+      const int idx = tid + 256 * r;
+      int kk, col;
+      if constexpr (KIND == KIND_DENSE) {
+        kk = idx % BK;
+        col = idx / BK;
+      } else {
+        col = idx % NW;
+        kk = idx / NW;
+      }
+      const int gk = k0 + kk;
+      double v = 0.0;
+      if (col < N && gk < K)
+        v = (KIND == KIND_DENSE) ? H[(size_t)col * ldh + gk] : H[(size_t)gk * ldh + col];
+      hr[r] = v;
+    }
+  };
+  auto store_h = [&](int b) {
+#pragma unroll
+    for (int r = 0; r < HREG; ++r) {
+      const int idx = tid + 256 * r;
+      if constexpr (KIND == KIND_DENSE)
+        Hs[b][(idx % BK) * HLD + idx / BK] = hr[r];
+      else
+        Hs[b][(idx / NW) * HLD + idx % NW] = hr[r];
+    }
+  };
+  auto mfma_slab = [&](const double(&af)[KQ][RB], int b) {
+#pragma unroll
+    for (int kq = 0; kq < KQ; ++kq) {
+      const double* hk = &Hs[b][(4 * kq + lk) * HLD + lr];
+#pragma unroll
+      for (int j = 0; j < NB; ++j) {
+        const double bv = hk[16 * j];
+#pragma unroll
+        for (int i = 0; i < RB; ++i)
+          acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[kq][i], bv, acc[i][j], 0, 0, 0);
+      }
+    }
+  };
+
+  // slab kt: A from a0 (kt even) or a1 (kt odd), H from Hs[kt & 1]; after its
+  // MFMAs the same registers take slab kt + 2 and Hs[(kt + 1) & 1] slab kt + 1
+  // (that buffer's last reads were in step kt - 1, before the barrier closing it)
+  const int nkt = (K + BK - 1) / BK;
+  load_a(a0, 0);
+  load_a(a1, BK);
+  load_h(0);
+  store_h(0);
+  __syncthreads();
+  for (int kt = 0; kt < nkt; kt += 2) {
+    if (kt + 1 < nkt) load_h((kt + 1) * BK);
+    mfma_slab(a0, 0);
+    load_a(a0, (kt + 2) * BK);
+    if (kt + 1 < nkt) store_h(1);
+    __syncthreads();
+    if (kt + 1 >= nkt) break;
+    if (kt + 2 < nkt) load_h((kt + 2) * BK);
+    mfma_slab(a1, 1);
+    load_a(a1, (kt + 3) * BK);
+    if (kt + 2 < nkt) store_h(0);
+    __syncthreads();
+  }
+
+  const double alpha = q.alpha, beta = q.beta;
+#pragma unroll
+  for (int j = 0; j < NB; ++j) {
+    const int gc = 16 * j + lr;
+    if (gc >= N) continue;
+    double* cc = q.C + (size_t)gc * q.ldc;
+#pragma unroll
+    for (int i = 0; i < RB; ++i)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int gr = m0 + wm0 + 16 * i + lk + 4 * r;
+        if (gr >= M) continue;
+        double v = alpha * acc[i][j][r];
+        if (beta != 0.0) v += beta * cc[gr];
+        cc[gr] = v;
+      }
+  }
+}
+
+// A stage the panel kernel takes: explicit factors of one layout, no tail, at
+// most kPanelNmax columns.  Returns the kind (KIND_DENSE / KIND_DENSE_T) or -1.
+int panel_kind(const std::vector<BatchProblem>& stage) {
+  if (stage.empty()) return -1;
+  const int kind = stage[0].kind;
+  if (kind != KIND_DENSE && kind != KIND_DENSE_T) return -1;
+  for (const BatchProblem& b : stage)
+    if (b.kind != kind || b.n > kPanelNmax || b.k2 > 0 || b.m <= 0) return -1;
+  return kind;
+}
+
+// Uploads the stage tables and runs each stage as one launch, in order on s: the
+// panel kernel where the stage is a set of panel products (and panels is set),
+// else the persistent batch kernel.
 // big_min: the smaller side from which a problem takes the 128 x 128 tile class;
 // ksets: the batch kernel's kind set per stage (default: the HSS set for all).
 int run_stages(const std::vector<std::vector<BatchProblem>>& stages, hipStream_t s,
-               int big_min = 256, const std::vector<int>& ksets = {}) {
+               int big_min = 256, const std::vector<int>& ksets = {}, bool panels = false) {
   BatchStages bs;
   int st = bs.upload(stages, ksets, big_min, WS_HSS_TABLES, s);
-  for (size_t i = 0; i < stages.size() && st == 0; ++i) st = bs.launch(i);
+  for (size_t i = 0; i < stages.size() && st == 0; ++i) {
+    const int pk = panels ? panel_kind(stages[i]) : -1;
+    if (pk < 0 || bs.problems(i) == 0) {
+      st = bs.launch(i);
+      continue;
+    }
+    int mmax = 0, nmax = 0;
+    for (const BatchProblem& b : stages[i]) {
+      mmax = std::max(mmax, b.m);
+      nmax = std::max(nmax, b.n);
+    }
+    const dim3 grid((mmax + kPanelBM - 1) / kPanelBM, bs.problems(i));
+    const int nb2 = (nmax + 31) / 32;  // pairs of 16-column blocks, 1..5
+    const BatchProblem* t = bs.table(i);
+#define PSMMA_PANEL(NB)                                                                  \
+  do {                                                                                   \
+    if (pk == KIND_DENSE)                                                                \
+      hipLaunchKernelGGL((hss_panel_kernel<KIND_DENSE, NB>), grid, dim3(256), 0, s, t);   \
+    else                                                                                 \
+      hipLaunchKernelGGL((hss_panel_kernel<KIND_DENSE_T, NB>), grid, dim3(256), 0, s, t); \
+  } while (0)
+    switch (nb2) {
+      case 1: PSMMA_PANEL(2); break;
+      case 2: PSMMA_PANEL(4); break;
+      case 3: PSMMA_PANEL(6); break;
+      case 4: PSMMA_PANEL(8); break;
+      default: PSMMA_PANEL(10); break;
+    }
+#undef PSMMA_PANEL
+    const hipError_t e = hipGetLastError();
+    st = e == hipSuccess ? 0 : 1000 + (int)e;
+  }
   const int fin = bs.finish();  // the tables must outlive the launches
   return st ? st : fin;
 }
@@ -671,6 +854,12 @@ int build(Hss& h, double tol, unsigned long long seed, hipStream_t strm) {
   return done(0);
 }
 
+// PSMMA_HSS_PANELS=0 runs the passes' panel products on the batch kernel too.
+bool panels_on() {
+  const char* e = getenv("PSMMA_HSS_PANELS");
+  return !(e && atoi(e) == 0);
+}
+
 int product(const Hss& h, int m, double alpha, const double* a, int lda, double beta, double* c,
             int ldc, hipStream_t strm) {
   const int L = h.L;
@@ -758,7 +947,7 @@ int product(const Hss& h, int m, double alpha, const double* a, int lda, double 
                                          h.H + nd.offV, nd.wc, 1.0, c + (size_t)nd.lo * ldc, ldc));
     }
   }
-  return run_stages(stg, strm, 256, ksets);
+  return run_stages(stg, strm, 256, ksets, panels_on());
 }
 
 }  // namespace

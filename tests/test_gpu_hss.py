@@ -180,3 +180,33 @@ def test_hss_explicit_sample_counts(gpu, samples):
     else:  # 40 samples are too few for these ranks: flagged, not silently accepted
         assert h.stats["saturated"] > 0
     h.close()
+
+
+@pytest.mark.parametrize("kind,n,m", [("eig", 4096, 1024), ("eig", 3000, 333),
+                                      ("cauchy", 2048, 65)])
+def test_hss_panel_kernel_matches_batch_kernel(gpu, monkeypatch, kind, n, m):
+    """The passes' panel products (one workgroup per 64-row panel and every column of one
+    problem, A's fragments loaded straight into registers) against the same stages on the
+    persistent batch kernel (PSMMA_HSS_PANELS=0): both sum each entry over k in the same
+    order, so they agree to rounding; ragged panels (m = 333, 65) and uneven leaves
+    (n = 3000) included, beta != 0."""
+    import torch
+
+    print(f"case hss panels {kind} N={n} m={m}", flush=True)
+    okind, gens, c = (eig_case if kind == "eig" else cauchy_case)(n, seed=n + 7)
+    h = api.Hss(okind, gens, leaf=LEAF, tol=TOL, seed=5)
+    rng = np.random.default_rng(m)
+    a = dev_cm(rng.uniform(-1, 1, (m, n)))
+    x0 = rng.standard_normal((m, n))
+    outs = {}
+    for pan in ("0", "1"):
+        monkeypatch.setenv("PSMMA_HSS_PANELS", pan)
+        x = dev_cm(x0)
+        h.product(0.75, a, x, beta=-0.5)
+        torch.cuda.synchronize()
+        outs[pan] = x.cpu().numpy()
+    d = ora.rel_fro(outs["1"], outs["0"])
+    err = ora.rel_fro(outs["1"], 0.75 * (a.cpu().numpy() @ c) - 0.5 * x0)
+    print(f"  panels vs batch kernel {d:.3e}; product rel err {err:.3e}", flush=True)
+    assert d <= 1e-15 and err < TOL
+    h.close()
