@@ -783,150 +783,6 @@ class Hss:
                     vdim=vd.reshape(k, 2), bdim=bd.reshape(k, 2), off=off, H=h)
 
     def close(self):
-        if self._h:
-            lib().psmma_batched_plan_destroy(self._h)
-            self._h = None
-
-    def __del__(self):
-        try:
-            self.close()
-        except Exception:
-            pass
-
-
-# ---------------------------------------------------------------------------
-# The reference's P x Q grid, one rank's share at a time (include/psmma.h 3b).
-
-def constindex(n: int, nb: int, mycol: int, npcol: int) -> np.ndarray:
-    """ConstIndex (SRC/Auxil_2p5D.f90:2-66): 1-based global indices of process column mycol."""
-    cnt = lib().psmma_constindex(n, nb, mycol, npcol, None)
-    if cnt < 0:
-        raise ValueError(f"constindex({n}, {nb}, {mycol}, {npcol}): invalid arguments")
-    out = np.zeros(max(cnt, 1), dtype=np.int32)
-    lib().psmma_constindex(n, nb, mycol, npcol, out.ctypes.data_as(ctypes.c_void_p))
-    return out[:cnt]
-
-
-def is_intersect(a, b) -> bool:
-    """is_intersect (SRC/Auxil_2p5D.f90:752-792), the reference's sense: True iff disjoint."""
-    a = np.ascontiguousarray(a, dtype=np.int32)
-    b = np.ascontiguousarray(b, dtype=np.int32)
-    return bool(lib().psmma_is_intersect(a.size, a.ctypes.data_as(ctypes.c_void_p), b.size,
-                                         b.ctypes.data_as(ctypes.c_void_p)))
-
-
-def ring_step_dev(problem: "Problem", rindex, cindex, stream=None):
-    """One ring step C = alpha * A * H(rindex, cindex) + beta * C (problem.r0 / c0 ignored);
-    rindex, cindex: 1-based global indices (host arrays)."""
-    r = np.ascontiguousarray(rindex, dtype=np.int32)
-    c = np.ascontiguousarray(cindex, dtype=np.int32)
-    if r.size != problem.k or c.size != problem.n:
-        raise ValueError("rindex / cindex lengths must equal the step's k / n")
-    _check(lib().psmma_ring_step_dev(ctypes.byref(problem), r.ctypes.data_as(ctypes.c_void_p),
-                                     c.ctypes.data_as(ctypes.c_void_p), _stream_ptr(stream)),
-           "psmma_ring_step_dev")
-
-
-def local_offsets(m, n, nprow, npcol, mb, nb) -> np.ndarray:
-    """Offsets (elements) of the P*Q local arrays of a block-cyclic layout, rank-major."""
-    out = np.zeros(nprow * npcol + 1, dtype=np.int64)
-    tot = lib().psmma_local_offsets(m, n, nprow, npcol, mb, nb, out.ctypes.data_as(ctypes.c_void_p))
-    if tot < 0:
-        raise ValueError("local_offsets: invalid arguments")
-    return out
-
-
-def redistribute_dev(m, n, nprow, npcol, src_mb, src_nb, a, dst_mb, dst_nb, b, stream=None):
-    """PDGEMR2D between two block-cyclic layouts of one grid; a, b: 1-D float64 device tensors
-    holding all ranks' local arrays (local_offsets order)."""
-    na = int(local_offsets(m, n, nprow, npcol, src_mb, src_nb)[-1])
-    nb_ = int(local_offsets(m, n, nprow, npcol, dst_mb, dst_nb)[-1])
-    _check(lib().psmma_redistribute_dev(m, n, nprow, npcol, src_mb, src_nb, _vec(a, "a", na),
-                                        dst_mb, dst_nb, _vec(b, "b", nb_), _stream_ptr(stream)),
-           "psmma_redistribute_dev")
-
-
-# ---------------------------------------------------------------------------
-# Autotuner (include/psmma.h 3c).
-
-def autotune_f64(kind: int, m: int, n: int, k: int):
-    """Times the FP64 tile configurations for this (kind, size bucket) and caches the fastest.
-    Returns (cfg, ms)."""
-    cfg, ms = ctypes.c_int(-1), ctypes.c_float(0.0)
-    _check(lib().psmma_autotune_f64(kind, m, n, k, ctypes.byref(cfg), ctypes.byref(ms)),
-           "psmma_autotune_f64")
-    return cfg.value, ms.value
-
-
-def split_slices_dev(problem: "Problem", prec=PREC_SPLIT_I8, target=1e-12, stream=None):
-    """Fewest moduli / slices whose error on a sampled column block is <= target.
-    Returns (slices, measured error, reached)."""
-    t, err = ctypes.c_int(0), ctypes.c_double(0.0)
-    st = lib().psmma_split_slices_dev(ctypes.byref(problem), prec, float(target), ctypes.byref(t),
-                                      ctypes.byref(err), _stream_ptr(stream))
-    if st < 0 or st > 1:
-        raise PsmmaError(f"psmma_split_slices_dev returned {st}")
-    return t.value, err.value, st == 0
-
-
-# ---------------------------------------------------------------------------
-# Nested HSS representation (include/psmma.h 6b).
-
-class Hss:
-    """HSS representation of the n x n eigen-Cauchy (kind KIND_CAUCHYEIG, gens = fpole, u, v,
-    difl, difr) or plain Cauchy (KIND_CAUCHY, gens = d, f, u, v) matrix on the device."""
-
-    STAT_KEYS = ("levels", "rank_max", "rank_sum", "saturated", "storage", "samples")
-
-    def __init__(self, kind, gens, leaf=256, tol=1e-12, samples=0, seed=0, stream=None):
-        n = gens[0].numel()
-        ptrs = [_vec(t, f"gens[{i}]", n) for i, t in enumerate(gens)] + [None] * (5 - len(gens))
-        h = ctypes.c_void_p()
-        st = (ctypes.c_longlong * 6)()
-        self._gens = gens  # the generators must outlive the handle
-        _check(lib().psmma_hss_build_dev(kind, n, *ptrs, leaf, float(tol), samples, seed,
-                                         ctypes.byref(h), st, _stream_ptr(stream)),
-               "psmma_hss_build_dev")
-        self._h, self.n = h, n
-        self.stats = dict(zip(self.STAT_KEYS, list(st)))
-
-    def product(self, alpha, a, c, beta=0.0, stream=None):
-        """X = alpha * A * C + beta * X; a, c column-major (m x n) device views."""
-        pa, m, k, lda = _colmajor(a, "a")
-        if k != self.n:
-            raise ValueError(f"a has {k} columns, the HSS matrix is {self.n} x {self.n}")
-        pc, ldc = _out(c, m, self.n)
-        _check(lib().psmma_hss_product_dev(self._h, m, float(alpha), pa, lda, float(beta), pc, ldc,
-                                           _stream_ptr(stream)), "psmma_hss_product_dev")
-
-    def layout(self):
-        """Post-order layout and a host copy of the generator array H."""
-        import torch
-        nn = ctypes.c_int()
-        lib().psmma_hss_layout(self._h, ctypes.byref(nn), None, None, None, None, None, None,
-                               None, None)
-        k = nn.value
-        parent, lo, size = (np.zeros(k, np.int32) for _ in range(3))
-        ud, vd, bd = (np.zeros(2 * k, np.int32) for _ in range(3))
-        off = np.zeros(3 * k, np.int64)
-        hp = ctypes.c_void_p()
-        p_ = lambda x: x.ctypes.data_as(ctypes.c_void_p)
-        lib().psmma_hss_layout(self._h, ctypes.byref(nn), p_(parent), p_(lo), p_(size), p_(ud),
-                               p_(vd), p_(bd), p_(off), ctypes.byref(hp))
-        total = int(max(off.max() + 1, 1)) if k else 1
-        # the generator regions end at the last B region plus its fixed size
-        total = int(off.max() + self.stats["samples"] ** 2 + 2 * 256 * 64)
-        hipmod = ctypes.CDLL("libamdhip64.so")
-        torch.cuda.synchronize()
-        buf = np.zeros(total, np.float64)
-        nbytes = 8 * total
-        # H is allocated for every region; copy only what the regions cover
-        hipmod.hipMemcpy(p_(buf), ctypes.c_void_p(hp.value), ctypes.c_size_t(
-            8 * int(off.max() + self.stats["samples"] ** 2)), 2)
-        return dict(parent=parent, lo=lo, size=size, udim=ud.reshape(k, 2),
-                    vdim=vd.reshape(k, 2), bdim=bd.reshape(k, 2), off=off.reshape(k, 3), H=buf)
-
-    def close(self):
         if getattr(self, "_h", None):
             lib().psmma_hss_destroy(self._h)
             self._h = None
