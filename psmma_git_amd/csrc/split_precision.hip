@@ -347,7 +347,7 @@ __device__ __forceinline__ void split_a_kernel_body(int M, int K, const double* 
     __syncthreads();
     if (st + 1 < 4) load(r0 + 16);
     const int gi = i0 + r0 + ir;
-    if (gi < M) {
+    if (gi < M && kbase + kb < kp) {  // kp may end half-way into the last unit
       const int sh = abits - gld(ea + gi);
       double v[16];
 #pragma unroll
@@ -2191,7 +2191,7 @@ __device__ __forceinline__ void gemm_i8_fused_body(
   int sc = 0;  // stage of the current step: kt % NS
   auto next_stage = [](int s) { return s == NS - 1 ? 0 : s + 1; };
   for (int l = 0; l < t; ++l) {
-    for (int kk = 0; kk < nks; kk += 2) {  // nks is a multiple of 4
+    for (int kk = 0; kk < nks; kk += 2) {  // nks is even (kp a multiple of 128)
       // steps kt and kt+1 have landed once only step kt+2 may be in flight;
       // after the barrier every wave has read the stages of steps <= kt-1,
       // which the DMA of steps kt+3 and kt+4 overwrite
@@ -2310,7 +2310,10 @@ void split_batch_layout(const BatchProblem* probs, int nprob, int t, char* ws, S
     q.kind = b.kind;
     q.bmode = (b.kind == KIND_TOEPLITZ) ? BMODE_TOE : (b.kind == KIND_HANKEL) ? BMODE_HANK : BMODE_DENSE;
     q.M = b.m; q.N = b.n; q.K = b.k;
-    q.kp = ((b.k + 255) / 256) * 256;
+    // the fused GEMM steps in pairs of 64-deep k steps; the residue-plane
+    // GEMMs need a multiple of 256
+    const int kq = sb.fused ? 128 : 256;
+    q.kp = ((b.k + kq - 1) / kq) * kq;
     if (q.bmode != BMODE_DENSE && q.kp > (sb.fused ? kFusedWinKmax : kBatchKmax))
       q.bmode = BMODE_DENSE;
     {
@@ -2350,8 +2353,8 @@ void split_batch_layout(const BatchProblem* probs, int nprob, int t, char* ws, S
     if (dense) { gx[PH_COLEXP] = (q.N + 255) / 256; gy[PH_COLEXP] = kc; }
     else { gx[PH_COLEXP] = (int)((L + 255) / 256); gy[PH_COLEXP] = 1; }
     if (!dense) { gx[PH_EHFILL] = (q.N + 255) / 256; gy[PH_EHFILL] = 1; }
-    gx[PH_SPLITA] = (q.M + 63) / 64; gy[PH_SPLITA] = q.kp / kSplitAK;
-    if (dense) { gx[PH_SPLITB] = (q.N + 15) / 16; gy[PH_SPLITB] = q.kp / 256; }
+    gx[PH_SPLITA] = (q.M + 63) / 64; gy[PH_SPLITA] = (q.kp + kSplitAK - 1) / kSplitAK;
+    if (dense) { gx[PH_SPLITB] = (q.N + 15) / 16; gy[PH_SPLITB] = (q.kp + 255) / 256; }
     else { gx[PH_SPLITB] = (q.lpad + 255) / 256; gy[PH_SPLITB] = 1; }
     // the large class: the direct-to-LDS tiles, whose residue planes must be
     // addressable with 32-bit offsets
