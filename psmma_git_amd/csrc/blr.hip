@@ -14,8 +14,9 @@
 // whose diagonal block is generated in LDS and whose low-rank columns follow as an
 // explicit tail of the same contraction (T'_J R_J, each block column at its own
 // width).
-// The T columns are regrouped from (I, J) to (J, I) order between the two
-// stages, so each block column's tail is one contiguous panel of T'.
+// The first stage stores each column of T_I straight at its place in the block
+// column order (J, I) through a column map, so each block column's tail is one
+// contiguous panel of T' and no regrouping pass runs between the stages.
 // This is the reference's off-diagonal ring step (PSTOEPLITZ_COMPUTE's
 // RRQR branch, SRC/pstoeplitz_compute.f90:263-288, which compresses every
 // off-diagonal tile with Compress_RRQR and multiplies A by [Q | R P^T] with two
@@ -60,7 +61,7 @@ constexpr int kKmax = 32;
 struct PackEntry {
   int slot;   // block index of the RRQR outputs
   int r;      // rank
-  int offa;   // column offset in U and T (block-row order (I, J))
+  int offa;   // column offset in U (block-row order (I, J))
   int offb;   // column offset in T' and row offset in R (block-column order (J, I))
 };
 
@@ -80,29 +81,6 @@ __global__ __launch_bounds__(256) void blr_pack_kernel(const PackEntry* __restri
     U[(size_t)i * S + e.offa + c] = q[(size_t)c * leaf + i];
     // R P^T is stored column-major with leading dimension kKmax
     R[(size_t)(e.offb + c) * leaf + i] = rp[(size_t)i * kKmax + c];
-  }
-}
-
-// T'(:, offb + c) = T(:, offa + c) for every low-rank block: grid (blocks, row chunks).
-// With an even leading dimension every column starts 16-byte aligned (the buffers are
-// 256-byte aligned) and moves as pairs of doubles; an odd m leaves one last element.
-__global__ __launch_bounds__(256) void blr_regroup_kernel(const PackEntry* __restrict__ tab, int m,
-                                                          const double* __restrict__ T,
-                                                          double* __restrict__ Tp, int ldt) {
-  const PackEntry e = tab[blockIdx.x];
-  const int step = 256 * gridDim.y, i0 = blockIdx.y * 256 + threadIdx.x;
-  for (int c = 0; c < e.r; ++c) {
-    const double* src = T + (size_t)(e.offa + c) * ldt;
-    double* dst = Tp + (size_t)(e.offb + c) * ldt;
-    if ((ldt & 1) == 0) {
-      const int m2 = m >> 1;
-      const double2* s2 = (const double2*)src;
-      double2* d2 = (double2*)dst;
-      for (int i = i0; i < m2; i += step) d2[i] = s2[i];
-      if ((m & 1) && i0 == 0) dst[m - 1] = src[m - 1];
-    } else {
-      for (int i = i0; i < m; i += step) dst[i] = src[i];
-    }
   }
 }
 
@@ -249,6 +227,11 @@ int blr_product(int kind, const GenPtrs& g, int m, int nglob, double alpha, cons
   std::vector<PackEntry> packed;
   for (int I = 0; I < nb; ++I)
     for (int b : rowlist[I]) packed.push_back(tab[b]);
+  // column map of the first stage: column offa + c of the block-row order goes to
+  // column offb + c of T'
+  std::vector<int> cmap((size_t)std::max<long long>(S, 1), 0);
+  for (const PackEntry& pe : packed)
+    for (int c = 0; c < pe.r; ++c) cmap[(size_t)pe.offa + c] = pe.offb + c;
   if (stats) {
     stats[0] = nblk;
     stats[1] = (long long)packed.size();
@@ -257,24 +240,26 @@ int blr_product(int kind, const GenPtrs& g, int m, int nglob, double alpha, cons
     stats[4] = ranksum;
   }
 
-  // second workspace part: pack table, U (leaf x S, row-major), R (SR x leaf, row-major),
-  // T (m x S), T' (m x SR)
+  // second workspace part: pack table, column map (S entries), U (leaf x S, row-major),
+  // R (SR x leaf, row-major), T' (m x SR)
   const size_t off_tab = 0;
-  const size_t off_u = align_up(off_tab + sizeof(PackEntry) * std::max<size_t>(1, packed.size()));
+  const size_t off_map = align_up(off_tab + sizeof(PackEntry) * std::max<size_t>(1, packed.size()));
+  const size_t off_u = align_up(off_map + sizeof(int) * cmap.size());
   const size_t off_rr = align_up(off_u + sizeof(double) * (size_t)leaf * S);
-  const size_t off_t = align_up(off_rr + sizeof(double) * (size_t)SR * leaf);
-  const size_t off_tp = align_up(off_t + sizeof(double) * (size_t)m * S);
+  const size_t off_tp = align_up(off_rr + sizeof(double) * (size_t)SR * leaf);
   const size_t total = align_up(off_tp + sizeof(double) * (size_t)m * SR);
   char* ws2 = (char*)ws_lease(WS_BLR_GEMM, total, s, lk_gemm);
   if (!ws2) return 1002;
   PackEntry* dtab = (PackEntry*)(ws2 + off_tab);
+  int* dmap = (int*)(ws2 + off_map);
   double* U = (double*)(ws2 + off_u);
   double* R = (double*)(ws2 + off_rr);
-  double* T = (double*)(ws2 + off_t);
   double* Tp = (double*)(ws2 + off_tp);
-  // the table is copied before any launch of this call (the stream is idle here)
+  // the tables are copied before any launch of this call (the stream is idle here)
   if (!packed.empty()) {
     e = hipMemcpy(dtab, packed.data(), sizeof(PackEntry) * packed.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+      e = hipMemcpy(dmap, cmap.data(), sizeof(int) * cmap.size(), hipMemcpyHostToDevice);
     if (e != hipSuccess) return 1000 + (int)e;
   }
 
@@ -305,8 +290,8 @@ int blr_product(int kind, const GenPtrs& g, int m, int nglob, double alpha, cons
   const int tail_kind = window ? KIND_CAUCHY : kind;
   if (lowrank)
     for (int I = 0; I < nb; ++I) {
-      BatchProblem b = problem(tail_kind, wrow[I], 0, 1.0, a, lda, 0.0,
-                               T + (size_t)I * m * wA, m);
+      BatchProblem b = problem(tail_kind, wrow[I], 0, 1.0, a, lda, 0.0, Tp, m);
+      b.cmap = dmap + (size_t)I * wA;  // stored in block-column order
       if (!window) b.g = g;
       b.A2 = a + (size_t)I * leaf * lda;
       b.lda2 = lda;
@@ -352,10 +337,7 @@ int blr_product(int kind, const GenPtrs& g, int m, int nglob, double alpha, cons
       fprintf(stderr, "compressed structured product: %s\n", hipGetErrorString(e));
       return 1000 + (int)e;
     }
-    if ((st = bs.launch(0))) return st;  // T_I = A(:, I) U_I
-    dim3 grid((unsigned)packed.size(), (unsigned)std::min(64, (m / 2 + 255) / 256 + 1));
-    blr_regroup_kernel<<<grid, 256, 0, s>>>(dtab, m, T, Tp, m);
-    if ((e = hipGetLastError()) != hipSuccess) return 1000 + (int)e;
+    if ((st = bs.launch(0))) return st;  // T_I = A(:, I) U_I, stored as T'
   }
   if ((st = bs.launch(1))) return st;  // the diagonal blocks carry beta (and the tails)
   if (window && (st = bs.launch(2))) return st;  // the window kinds' tails, after beta

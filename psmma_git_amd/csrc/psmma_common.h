@@ -170,6 +170,9 @@ struct BatchProblem {
   const double* H2;
   int ldh2;
   int k2;
+  // Optional column map of C (tail form only): output column c of the product is
+  // stored in column cmap[c] of C.
+  const int* cmap;
 };
 
 // The explicit tail segment as the tile body takes it.
@@ -179,6 +182,7 @@ struct TailSeg {
   const double* H = nullptr;
   int ldh = 1;
   int K = 0;
+  const int* cmap = nullptr;  // column map of C (BatchProblem::cmap)
 };
 
 }  // namespace psmma

@@ -393,6 +393,16 @@ __device__ __forceinline__ void right_tile(int M, int N, int K, double alpha,
   }
 
   // Epilogue: C = alpha*acc + beta*C.  C is not read when beta == 0 (BLAS rule).
+  // (The tail form may store output column c in column cmap[c] of C.)
+  int oc[TN];
+#pragma unroll
+  for (int j = 0; j < TN; ++j) {
+    const int gc = n0 + wn0 + j * 16 + lr;
+    oc[j] = gc;
+    if constexpr (TAIL) {
+      if (tl.cmap && gc < N) oc[j] = tl.cmap[gc];
+    }
+  }
 #pragma unroll
   for (int i = 0; i < TM; ++i) {
 #pragma unroll
@@ -403,7 +413,7 @@ __device__ __forceinline__ void right_tile(int M, int N, int K, double alpha,
       for (int r = 0; r < 4; ++r) {
         int gr = m0 + wm0 + i * 16 + lk + 4 * r;
         if (gr < M) {
-          double* p = C + (size_t)gc * ldc + gr;
+          double* p = C + (size_t)oc[j] * ldc + gr;
           double v = alpha * acc[i][j][r];
           if (beta != 0.0) v += beta * (*p);
           *p = v;
@@ -708,7 +718,7 @@ template <int KIND, int BM, int BN, int BK, int WGM, int WGN, bool TAIL = false>
 __device__ __forceinline__ void batch_tile(const BatchProblem& p, int local, double* lds) {
   using L = RightLds<KIND, BM, BN, BK>;
   TailSeg tl;
-  if constexpr (TAIL) tl = TailSeg{p.A2, p.lda2, p.H2, p.ldh2, p.k2};
+  if constexpr (TAIL) tl = TailSeg{p.A2, p.lda2, p.H2, p.ldh2, p.k2, p.cmap};
   right_tile<KIND, BM, BN, BK, WGM, WGN, false, TAIL>(
       p.m, p.n, p.k, p.alpha, p.A, p.lda, p.g, p.r0, p.c0, p.beta, p.C, p.ldc,
       (local / p.ntn) * BM, (local % p.ntn) * BN, lds, lds + 2 * L::ASZ, tl);
