@@ -466,6 +466,11 @@ __global__ __launch_bounds__(WGM* WGN * 64, 2) void sgemm_f64_indexed_kernel(
 // entry is correctly rounded instead of accumulating the error of Wn**PowK.
 // B and C are complex128, interleaved (re, im), column-major, ld in complex
 // elements.  alpha and beta are real, as in the reference's argument list.
+// The complex product runs in the three-product form (Gauss): P1 = sum Fr Br,
+// P2 = sum Fi Bi, P3 = sum (Fr + Fi)(Br + Bi), Re = P1 - P2, Im = P3 - P1 - P2:
+// three real MFMAs per complex multiply-add instead of four.  Measured on MI355X,
+// F(4096 x 4096) B(4096 x 512): 1.185 -> 1.051 ms, 3.03e-15 -> 3.04e-15 against
+// the FFT of the same columns.
 template <int BM, int BN, int BK, int WGM, int WGN>
 __global__ __launch_bounds__(WGM* WGN * 64, 2) void dft_f64_left_kernel(
     int M, int N, int K, int Nd, double alpha, const double2* __restrict__ tw,
@@ -480,6 +485,7 @@ __global__ __launch_bounds__(WGM* WGN * 64, 2) void dft_f64_left_kernel(
   constexpr int BREG = BK * BN / NT;
   static_assert(BM * BK % NT == 0 && BK * BN % NT == 0, "tile/threads mismatch");
   static_assert(NT % BM == 0, "F generator assumes fixed row per thread");
+  static_assert(BK % 8 == 0, "the slab runs as two halves of whole 4-deep MFMA steps");
 
   __shared__ double Fr[2][BK * FLD];
   __shared__ double Fi[2][BK * FLD];
@@ -497,13 +503,15 @@ __global__ __launch_bounds__(WGM* WGN * 64, 2) void dft_f64_left_kernel(
   const int wm0 = (wave / WGN) * WM, wn0 = (wave % WGN) * WN;
   const int lr = lane & 15, lk = lane >> 4;
 
-  v4d accr[TM][TN], acci[TM][TN];
+  // P1, P2, P3 of the three-product form
+  v4d accr[TM][TN], acci[TM][TN], accs[TM][TN];
 #pragma unroll
   for (int i = 0; i < TM; ++i)
 #pragma unroll
     for (int j = 0; j < TN; ++j) {
       accr[i][j] = (v4d){0.0, 0.0, 0.0, 0.0};
       acci[i][j] = (v4d){0.0, 0.0, 0.0, 0.0};
+      accs[i][j] = (v4d){0.0, 0.0, 0.0, 0.0};
     }
 
   // Each thread generates F for one fixed row (tid % BM) and FREG columns
@@ -584,26 +592,29 @@ __global__ __launch_bounds__(WGM* WGN * 64, 2) void dft_f64_left_kernel(
       constexpr int KK0 = decltype(k0c)::value, KK1 = decltype(k1c)::value;
 #pragma unroll
       for (int kk = KK0; kk < KK1; kk += 4) {
-        double ar[TM], ai[TM], an[TM], br[TN], bi[TN];
+        double ar[TM], ai[TM], br[TN], bi[TN];
 #pragma unroll
         for (int i = 0; i < TM; ++i) {
           ar[i] = Fr[cur][(kk + lk) * FLD + wm0 + i * 16 + lr];
           ai[i] = Fi[cur][(kk + lk) * FLD + wm0 + i * 16 + lr];
-          an[i] = -ai[i];
         }
 #pragma unroll
         for (int j = 0; j < TN; ++j) {
           br[j] = Br[cur][(kk + lk) * BLD + wn0 + j * 16 + lr];
           bi[j] = Bi[cur][(kk + lk) * BLD + wn0 + j * 16 + lr];
         }
+        double as[TM], bs[TN];
+#pragma unroll
+        for (int i = 0; i < TM; ++i) as[i] = ar[i] + ai[i];
+#pragma unroll
+        for (int j = 0; j < TN; ++j) bs[j] = br[j] + bi[j];
 #pragma unroll
         for (int i = 0; i < TM; ++i)
 #pragma unroll
           for (int j = 0; j < TN; ++j) {
             accr[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[i], br[j], accr[i][j], 0, 0, 0);
-            accr[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(an[i], bi[j], accr[i][j], 0, 0, 0);
-            acci[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[i], bi[j], acci[i][j], 0, 0, 0);
-            acci[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[i], br[j], acci[i][j], 0, 0, 0);
+            acci[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[i], bi[j], acci[i][j], 0, 0, 0);
+            accs[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(as[i], bs[j], accs[i][j], 0, 0, 0);
           }
       }
     };
@@ -631,7 +642,9 @@ __global__ __launch_bounds__(WGM* WGN * 64, 2) void dft_f64_left_kernel(
         int gr = m0 + wm0 + i * 16 + lk + 4 * r;
         if (gr < M) {
           double2* p = C + (size_t)gc * ldc + gr;
-          double2 v = make_double2(alpha * accr[i][j][r], alpha * acci[i][j][r]);
+          const double re = accr[i][j][r] - acci[i][j][r];
+          const double im = accs[i][j][r] - accr[i][j][r] - acci[i][j][r];
+          double2 v = make_double2(alpha * re, alpha * im);
           if (beta != 0.0) {
             double2 o = *p;
             v.x += beta * o.x;
@@ -979,7 +992,10 @@ hipError_t launch_indexed_f64(int kind, int M, int N, int K, double alpha, const
 hipError_t launch_dft_f64(int M, int N, int K, int Nd, double alpha, const double2* tw,
                           const double2* B, int ldb, int r0, int c0, double beta, double2* C, int ldc,
                           hipStream_t s) {
-  constexpr int BM = 64, BN = 64, BK = 16, WGM = 2, WGN = 2;
+  // 128 x 64 tiles of 8 waves with 8-deep slabs (57 KB of LDS): F(4096 x 4096) B(4096 x 512)
+  // 0.967 ms on MI355X, against 0.975 for 64 x 64 with BK = 8, 1.051 for 64 x 64 with
+  // BK = 16, 1.007 for 64 x 128 with BK = 8, 1.051 for 128 x 64 with BK = 16
+  constexpr int BM = 128, BN = 64, BK = 8, WGM = 4, WGN = 2;
   int nb = ((M + BM - 1) / BM) * ((N + BN - 1) / BN);
   hipLaunchKernelGGL((dft_f64_left_kernel<BM, BN, BK, WGM, WGN>), dim3(nb), dim3(WGM * WGN * 64), 0,
                      s, M, N, K, Nd, alpha, tw, B, ldb, r0, c0, beta, C, ldc);
