@@ -87,6 +87,10 @@ def _declare(L):
     L.ora_compress_cauchy.restype = I
     L.ora_compress_cauchyeig2.argtypes = [P, P, P, P, P, P, P, D, I, I, P, P]
     L.ora_compress_cauchyeig2.restype = I
+    L.ora_rrlu_cauchy.argtypes = [P, P, P, P, D, P, P, P, P, I, I]
+    L.ora_rrlu_cauchy.restype = I
+    L.ora_rrlu_cauchyeig_row.argtypes = [P, P, P, P, P, P, D, P, P, P, P, P, I, I, P]
+    L.ora_rrlu_cauchyeig_row.restype = I
 
 
 def _f64(x):
@@ -392,6 +396,41 @@ def compress_cauchyeig2(fpole, difl, difr, u, v, rows, cols, tol: float = RRLU_T
     q = h[:m * rk].reshape((m, rk), order="F").copy()
     y = h[m * rk:m * rk + rk * n].reshape((rk, n), order="F").copy()
     return rk, q, y, int(nsw[0])
+
+
+def rrlu_cauchy(d, f, a, v, tol: float = RRLU_TOL):
+    """SRC/Cauchylowrank.f90:7-134 (rrluCauchy) on one block: every output.
+    Returns dict(rk, pl, pu (1-based), d, f, a, v (permuted / updated in place),
+    u (m), w (min(m, n)))."""
+    d, f, a, v = (np.array(x, dtype=np.float64) for x in (d, f, a, v))
+    m, n = len(d), len(f)
+    u = np.zeros(m)
+    w = np.zeros(max(1, min(m, n)))
+    pl = np.zeros(m, np.int32)
+    pu = np.zeros(n, np.int32)
+    rk = lib().ora_rrlu_cauchy(_ptr(d), _ptr(f), _ptr(a), _ptr(v), tol, _ptr(u), _ptr(w),
+                               _ptr(pl), _ptr(pu), m, n)
+    return dict(rk=rk, pl=pl, pu=pu, d=d, f=f, a=a, v=v, u=u, w=w[:min(m, n)])
+
+
+def rrlu_cauchyeig_row(fpole, difl, difr, a, v, pl, pu, tol: float = RRLU_TOL):
+    """SRC/CauchyHssEig.f90:7-212 (rrluCauchyEigrow) on one block with global 1-based
+    row / column indices pl, pu and the block's generators a (rows), v (columns).
+    Returns dict(rk, pl, pu, pq, nswap, f (the block's row poles), a, v, u, w)."""
+    ff, dl, dr = (np.ascontiguousarray(x, dtype=np.float64) for x in (fpole, difl, difr))
+    a, v = np.array(a, np.float64), np.array(v, np.float64)
+    pl, pu = np.array(pl, np.int32), np.array(pu, np.int32)
+    m, n = len(pl), len(pu)
+    f = np.ascontiguousarray(ff[pl - 1])
+    u = np.zeros(m)
+    w = np.zeros(max(1, min(m, n)))
+    pq = np.zeros(m, np.int32)
+    nsw = np.zeros(1, np.int32)
+    rk = lib().ora_rrlu_cauchyeig_row(_ptr(ff), _ptr(dl), _ptr(dr), _ptr(f), _ptr(a), _ptr(v),
+                                      tol, _ptr(u), _ptr(w), _ptr(pl), _ptr(pu), _ptr(pq), m, n,
+                                      _ptr(nsw))
+    return dict(rk=rk, pl=pl, pu=pu, pq=pq, nswap=int(nsw[0]), f=f, a=a, v=v, u=u,
+                w=w[:min(m, n)])
 
 
 def rel_fro(x, ref) -> float:
