@@ -98,6 +98,13 @@ def cauchy_case(kind: str, m: int, n: int, seed: int):
     return d, f, a, v
 
 
+def cauchy_tile_generators(nglob: int, seed: int):
+    """(d, f, u, v): the benchmark's interlaced plain Cauchy generators of size nglob."""
+    from oracle import inputs
+    b = inputs.cauchy_generators(nglob, np.random.default_rng(seed))
+    return (np.ascontiguousarray(b[:, i]) for i in range(4))
+
+
 def cauchyeig_case(nglob: int, m: int, n: int, seed: int):
     """(fpole, difl, difr, u, v, rows, cols) of the first-block-row block rows 0..m-1,
     columns m..m+n-1 (0-based) of the N = nglob merge matrix."""
@@ -107,3 +114,28 @@ def cauchyeig_case(nglob: int, m: int, n: int, seed: int):
     zh = inputs.loewner_z(f, e["u"], dl, dr)
     v = inputs.column_normalisers(f, zh, dl, dr)
     return f, dl, dr, zh, v, np.arange(m), np.arange(m, m + n)
+
+
+def cauchy_tile(d, f, u, v, timeout: float = 60.0) -> np.ndarray:
+    """Cauchylike (SRC/Cauchylowrank.f90:509-521): the m x n block U_i V_j / (D_i - F_j)."""
+    d, f, u, v = (np.ascontiguousarray(x, np.float64) for x in (d, f, u, v))
+    m, n = len(d), len(f)
+    head = np.array([2, m, n], np.int32).tobytes() + np.zeros(1).tobytes()
+    out = _run(head + b"".join(x.tobytes() for x in (d, f, u, v)), timeout)
+    return np.frombuffer(out, np.float64, m * n).reshape((m, n), order="F").copy()
+
+
+def cauchyeig_tile(fpole, difl, difr, u, v, rows, cols, timeout: float = 60.0) -> np.ndarray:
+    """CauchylikeEig (SRC/CauchyHssEig.f90:1693-1774): the block of the eigen-Cauchy matrix
+    at global 0-based rows / cols; u, v are the global row / column generators."""
+    ff, dl, dr = (np.ascontiguousarray(x, np.float64) for x in (fpole, difl, difr))
+    rows, cols = np.asarray(rows), np.asarray(cols)
+    m, n = len(rows), len(cols)
+    ul = np.ascontiguousarray(np.asarray(u, np.float64)[rows])
+    vl = np.ascontiguousarray(np.asarray(v, np.float64)[cols])
+    pl = np.ascontiguousarray(rows + 1, np.int32)
+    pu = np.ascontiguousarray(cols + 1, np.int32)
+    head = (np.array([3, m, n], np.int32).tobytes() + np.zeros(1).tobytes() +
+            np.array([len(ff)], np.int32).tobytes())
+    out = _run(head + b"".join(x.tobytes() for x in (ff, dl, dr, ul, vl, pl, pu)), timeout)
+    return np.frombuffer(out, np.float64, m * n).reshape((m, n), order="F").copy()

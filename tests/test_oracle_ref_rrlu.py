@@ -8,7 +8,8 @@ generators, the Schur-complement generators U / V and the interpolation
 generators W (and for the eigen-Cauchy kind the row permutation PQ and the swap
 count).  The golden fixture (tests/golden/rrlu_reference.npz, written by
 tests/golden/make_rrlu_golden.py from the compiled reference) holds these
-outputs for 18 blocks at four tolerances, so the comparison runs wherever the
+outputs for 17 blocks at four tolerances, and three explicit tiles of the
+reference's Cauchylike / CauchylikeEig, so the comparison runs wherever the
 oracle builds; where the reference driver is built as well, more blocks are
 compared live.  The eigen-Cauchy blocks lie in the first block row (rows 1..M):
 elsewhere CauchyPivtEig's early-return test reads the block's row-pole array at
@@ -94,3 +95,38 @@ def test_live_reference_more_blocks(ora, seed):
                                       cols + 1, tol)
         _same(got, want, FLOAT_KEYS_E, ("pl", "pu", "pq"))
         assert got["nswap"] == want["nswap"]
+
+
+# Explicit tiles: Cauchylike (SRC/Cauchylowrank.f90:509-521) and CauchylikeEig
+# (SRC/CauchyHssEig.f90:1693-1774), the formulas every structured tile of the
+# Cauchy kinds comes from (both branches of the eigen-Cauchy pole difference).
+
+def _tile_specs(golden):
+    return json.loads(str(golden["tile_specs"]))
+
+
+def test_cauchy_tiles_equal_the_reference(ora, golden):
+    for i, (ng, seed, r0, m, c0, n) in enumerate(_tile_specs(golden)["cauchy"]):
+        d, f, u, v = ref.cauchy_tile_generators(ng, seed)
+        got = ora.cauchy_matrix(d, f, u, v, r0, m, c0, n)
+        assert np.array_equal(got, golden[f"tc{i}"])
+
+
+def test_cauchyeig_tiles_equal_the_reference(ora, golden):
+    for i, (ng, seed, r0, m, c0, n) in enumerate(_tile_specs(golden)["eig"]):
+        f, dl, dr, zh, v, _, _ = ref.cauchyeig_case(ng, 1, 1, seed)
+        rows, cols = np.arange(r0, r0 + m)[:, None], np.arange(c0, c0 + n)[None, :]
+        assert np.any(rows <= cols) and np.any(rows > cols)  # both branches
+        got = ora.cauchyeig_matrix(f, zh, v, dl, dr, r0, m, c0, n)
+        assert np.array_equal(got, golden[f"te{i}"])
+
+
+@pytest.mark.skipif(not ref.available(), reason="reference driver not built (make -C oracle ref)")
+def test_live_reference_tiles(ora):
+    f, dl, dr, zh, v, _, _ = ref.cauchyeig_case(600, 1, 1, 21)
+    for r0, m, c0, n in ((0, 600, 0, 600), (250, 90, 10, 300)):
+        want = ref.cauchyeig_tile(f, dl, dr, zh, v, np.arange(r0, r0 + m), np.arange(c0, c0 + n))
+        assert np.array_equal(ora.cauchyeig_matrix(f, zh, v, dl, dr, r0, m, c0, n), want)
+    d, ff, u, vv = ref.cauchy_tile_generators(500, 22)
+    want = ref.cauchy_tile(d, ff, u, vv)
+    assert np.array_equal(ora.cauchy_matrix(d, ff, u, vv, 0, 500, 0, 500), want)
