@@ -1,10 +1,13 @@
 """Runs the reference's own generator-only RRLU routines (rrluCauchy,
-rrluCauchyEigrow) through the driver oracle/ref_rrlu.f90, built from the
-reference's module sources by `make -C oracle ref` into oracle/_ref.
+rrluCauchyEigrow) and explicit tile generators (Cauchylike, CauchylikeEig)
+through the driver oracle/ref_rrlu.f90, built from the reference's module
+sources by `make -C oracle ref` into oracle/_ref.
 
-Test infrastructure only: the CPU tests compare the oracle's C restatement of
-the same routines with these outputs.  Absent where the reference sources are
-not (the driver is then not built and `available()` is False)."""
+Test infrastructure only: the CPU tests compare the oracle's restatements of
+the same routines with these outputs, and tests/golden/make_rrlu_golden.py
+writes them into the committed fixture the GPU tests use.  Absent where the
+reference sources are not (the driver is then not built and `available()` is
+False)."""
 from __future__ import annotations
 
 import subprocess
