@@ -12,18 +12,22 @@
 //    x_1 = rint(x 2^8) 2^-8, x_p = rint(r_{p-1} 2^(8p)) 2^(-8p) with r_p the
 //    remainder, so |x - sum_p x_p| <= 2^(-8S-1) and every slice has at most 9
 //    significant bits (exact in FP32);
-//  * products x_p y_q are exact in FP32; all pairs with p + q = d lie on the
-//    grid 2^(-8d), and for d <= 7 the real or imaginary part of one k step
-//    contributes at most 2^18 grid units, so the FP32 sum of a K-slab of 16
-//    stays below 2^24 units and is exact.  Pairs with p + q <= S + 1 are kept,
-//    one FP32 accumulator per class d and part, flushed into FP64 accumulators
-//    after every slab (FP64 recombination);
+//  * the complex product runs in the three-product form: per class, P1 = sum
+//    Fr_p Br_q, P2 = sum Fi_p Bi_q and P3 = sum (Fr_p + Fi_p)(Br_q + Bi_q), then
+//    Re = P1 - P2 and Im = P3 - P1 - P2.  Slices on one grid sum exactly (at most
+//    10 significant bits), so every product is exact in FP32; all pairs with
+//    p + q = d lie on the grid 2^(-8d), and for d <= 7 one k step adds at most
+//    2^17 grid units to P1 or P2 and 2^19 to P3, so the FP32 sum of a K-slab of
+//    16 stays below 2^24 units and is exact.  Pairs with p + q <= S + 1 are kept,
+//    three FP32 accumulators per class, flushed into FP64 accumulators after
+//    every slab (FP64 recombination, where P1 - P2 and P3 - P1 - P2 are exact:
+//    the result equals the four-product form's bit for bit);
 //  * the twiddles come from the FP64 table, so F's slices are those of the
 //    correctly rounded entries; they are generated straight into LDS.
 //
 // Error: about 2^(-8S) relative to the row/column maxima (normwise), plus the
-// FP64 recombination.  Cost: S(S+1)/2 slice pairs of 4 real FP32 products,
-// against 4 FP64 products for the plain form; FP32 MFMA runs at twice the FP64
+// FP64 recombination.  Cost: S(S+1)/2 slice pairs of 3 real FP32 products,
+// against 3 FP64 products for the plain form; FP32 MFMA runs at twice the FP64
 // MFMA rate on MI355X, so S = 2 costs 1.5x the plain form and S = 5 (about
 // 1e-12) 7.5x.  The form exists for callers that want FP32 MFMA; the plain FP64
 // kernel is the fast exact one on this GPU.
@@ -123,7 +127,8 @@ __global__ __launch_bounds__(WGM* WGN * 64, 1) void dft_split_f32_kernel(
   const int wm0 = (wave / WGN) * WM, wn0 = (wave % WGN) * WN;
   const int lr = lane & 15, lk = lane >> 4;
 
-  v4f accr[NCLS][TM][TN], acci[NCLS][TM][TN];
+  // P1, P2, P3 of each class (three-product form)
+  v4f accr[NCLS][TM][TN], acci[NCLS][TM][TN], accs[NCLS][TM][TN];
   double sr[TM][TN][4], si[TM][TN][4];
 #pragma unroll
   for (int i = 0; i < TM; ++i)
@@ -133,6 +138,7 @@ __global__ __launch_bounds__(WGM* WGN * 64, 1) void dft_split_f32_kernel(
       for (int d = 0; d < NCLS; ++d) {
         accr[d][i][j] = (v4f){0.f, 0.f, 0.f, 0.f};
         acci[d][i][j] = (v4f){0.f, 0.f, 0.f, 0.f};
+        accs[d][i][j] = (v4f){0.f, 0.f, 0.f, 0.f};
       }
 #pragma unroll
       for (int r = 0; r < 4; ++r) sr[i][j][r] = si[i][j][r] = 0.0;
@@ -230,37 +236,40 @@ __global__ __launch_bounds__(WGM* WGN * 64, 1) void dft_split_f32_kernel(
     }
 #pragma unroll
     for (int kk = 0; kk < BK; kk += 4) {
-      float far[S][TM], fai[S][TM], fan[S][TM], fbr[S][TN], fbi[S][TN];
+      // B's slices for every q, then F's slice p at a time (fewer live registers)
+      float fbr[S][TN], fbi[S][TN], fbs[S][TN];
 #pragma unroll
-      for (int p = 0; p < S; ++p) {
-#pragma unroll
-        for (int i = 0; i < TM; ++i) {
-          far[p][i] = Fs[cur][p][0][(kk + lk) * FLD + wm0 + i * 16 + lr];
-          fai[p][i] = Fs[cur][p][1][(kk + lk) * FLD + wm0 + i * 16 + lr];
-          fan[p][i] = -fai[p][i];
-        }
+      for (int q = 0; q < S; ++q)
 #pragma unroll
         for (int j = 0; j < TN; ++j) {
-          fbr[p][j] = Bs[cur][p][0][(kk + lk) * BLD + wn0 + j * 16 + lr];
-          fbi[p][j] = Bs[cur][p][1][(kk + lk) * BLD + wn0 + j * 16 + lr];
+          fbr[q][j] = Bs[cur][q][0][(kk + lk) * BLD + wn0 + j * 16 + lr];
+          fbi[q][j] = Bs[cur][q][1][(kk + lk) * BLD + wn0 + j * 16 + lr];
+          fbs[q][j] = fbr[q][j] + fbi[q][j];  // exact: one grid, at most 10 bits
         }
-      }
       // class d = p + q (0-based: dd = p + q, kept when dd <= S - 1)
 #pragma unroll
-      for (int p = 0; p < S; ++p)
+      for (int p = 0; p < S; ++p) {
+        float far[TM], fai[TM], fas[TM];
+#pragma unroll
+        for (int i = 0; i < TM; ++i) {
+          far[i] = Fs[cur][p][0][(kk + lk) * FLD + wm0 + i * 16 + lr];
+          fai[i] = Fs[cur][p][1][(kk + lk) * FLD + wm0 + i * 16 + lr];
+          fas[i] = far[i] + fai[i];
+        }
 #pragma unroll
         for (int q = 0; q + p < S; ++q)
 #pragma unroll
           for (int i = 0; i < TM; ++i)
 #pragma unroll
             for (int j = 0; j < TN; ++j) {
-              v4f& ar = accr[p + q][i][j];
-              v4f& ai = acci[p + q][i][j];
-              ar = __builtin_amdgcn_mfma_f32_16x16x4f32(far[p][i], fbr[q][j], ar, 0, 0, 0);
-              ar = __builtin_amdgcn_mfma_f32_16x16x4f32(fan[p][i], fbi[q][j], ar, 0, 0, 0);
-              ai = __builtin_amdgcn_mfma_f32_16x16x4f32(far[p][i], fbi[q][j], ai, 0, 0, 0);
-              ai = __builtin_amdgcn_mfma_f32_16x16x4f32(fai[p][i], fbr[q][j], ai, 0, 0, 0);
+              v4f& p1 = accr[p + q][i][j];
+              v4f& p2 = acci[p + q][i][j];
+              v4f& p3 = accs[p + q][i][j];
+              p1 = __builtin_amdgcn_mfma_f32_16x16x4f32(far[i], fbr[q][j], p1, 0, 0, 0);
+              p2 = __builtin_amdgcn_mfma_f32_16x16x4f32(fai[i], fbi[q][j], p2, 0, 0, 0);
+              p3 = __builtin_amdgcn_mfma_f32_16x16x4f32(fas[i], fbs[q][j], p3, 0, 0, 0);
             }
+      }
     }
     // flush the exact slab sums into FP64
 #pragma unroll
@@ -271,11 +280,13 @@ __global__ __launch_bounds__(WGM* WGN * 64, 1) void dft_split_f32_kernel(
         for (int d = 0; d < NCLS; ++d) {
 #pragma unroll
           for (int r = 0; r < 4; ++r) {
-            sr[i][j][r] += (double)accr[d][i][j][r];
-            si[i][j][r] += (double)acci[d][i][j][r];
+            const double p1 = (double)accr[d][i][j][r], p2 = (double)acci[d][i][j][r];
+            sr[i][j][r] += p1 - p2;                                // exact differences
+            si[i][j][r] += ((double)accs[d][i][j][r] - p1) - p2;
           }
           accr[d][i][j] = (v4f){0.f, 0.f, 0.f, 0.f};
           acci[d][i][j] = (v4f){0.f, 0.f, 0.f, 0.f};
+          accs[d][i][j] = (v4f){0.f, 0.f, 0.f, 0.f};
         }
     if (more) store(cur ^ 1);
     __syncthreads();
@@ -312,7 +323,10 @@ template <int S>
 static void launch_dft_split_s(int M, int N, int K, int Nd, double alpha, const double2* tw,
                                const double2* B, int ldb, const int* eb, int r0, int c0,
                                double beta, double2* C, int ldc, hipStream_t s) {
-  constexpr int BM = 64, BN = 64, BK = 8, WGM = 2, WGN = 2;
+  // 8 waves of 32 x 16: the three accumulators per class stay in registers up to 6
+  // slices (4 waves of 32 x 32 spill from 4); F(4096 x 4096) B(4096 x 512) on MI355X,
+  // 2 / 3 / 5 slices: 1.88 / 3.44 / 7.25 ms (four-product form on 4 waves: 2.14 / 3.97 / 8.65)
+  constexpr int BM = 64, BN = 64, BK = 8, WGM = 2, WGN = 4;
   int nb = ((M + BM - 1) / BM) * ((N + BN - 1) / BN);
   hipLaunchKernelGGL((dft_split_f32_kernel<S, BM, BN, BK, WGM, WGN>), dim3(nb),
                      dim3(WGM * WGN * 64), 0, s, M, N, K, Nd, alpha, tw, B, ldb, eb, r0, c0, beta,
